@@ -1,0 +1,161 @@
+"""Benchmark: Mrays/s of the plain HIP path tracer on the full-shading workload.
+
+Workload (one "step"): the 256^3 terrain+boxes scene with 70/20/10
+Lambert/metal/emissive IDs, the 1024x512 FP32 sky+sun environment
+(importance sampled), a 1024x1024 thin-lens camera, 16 spp and 8 bounces,
+rendered from a cleared accumulator.  Inputs are resident in HBM before the
+timed region; every step repeats the same 16 passes, so the ray count of a
+step is the count of the (untimed) statistics launch.
+
+Rays = every camera sample + every shadow and continuation traversal.
+The roofline object prices the render kernel at its algorithmic HBM bytes
+(1 B per DDA voxel visit, 32 B per surface interaction, 32 B per sample
+accumulator read-modify-write, binary-search and texel bytes per image
+environment lookup) over the kernel's HIP-event time.  The CPU baseline is
+the C oracle (OpenMP over the host cores) on a bounded row band of the same
+workload, one pass.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s; time-to-RMSE 1e-2 vs FP64 ref, plain vs batched+radiance-cache"
+HBM_PEAK_GBS = 8000.0
+
+
+def env_lookup_bytes(cu_w: int, cv_n: int) -> tuple[int, int]:
+    """Bytes of one importance-sampled environment draw (two binary searches
+    of 4-byte CDF entries, 4 bracket reads, 4 RGB32F texels) and of one
+    lookup by direction (4 RGB32F texels)."""
+    search = 4 * (math.ceil(math.log2(cv_n)) + math.ceil(math.log2(cu_w)) + 4)
+    return search + 48, 48
+
+
+def algorithmic_bytes(st: dict, cu_w: int, cv_n: int) -> int:
+    sample_b, eval_b = env_lookup_bytes(cu_w, cv_n)
+    return (st["visits"] * 1 + st["interactions"] * 32 + st["samples"] * 32 + st["env_samples"] * sample_b
+            + st["env_evals"] * eval_b)
+
+
+def cpu_baseline(wl, budget_s: float = 15.0):
+    """Oracle on a row band of the same workload (1 pass), all host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.build()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    sc = wl.scene
+    o = oracle.OracleScene(sc.offsets(), sc.mat, sc.emissive)
+    o.set_camera(**wl.camera_arrays())
+    cu, cv, integ = wl.cdf
+    o.set_environment(wl.env_top, wl.env_bottom, wl.env_image, cu, cv, integ, wl.env_rotation)
+    o.set_settings(max_bounces=wl.bounces)
+    # calibrate on 8 rows through the image centre, then size the band to the budget
+    y0 = wl.height // 2 - 4
+    t = time.perf_counter()
+    _, cnt = o.render(wl.width, wl.height, wl.seed, 0, 1, y0=y0, y1=y0 + 8, threads=cores)
+    dt = time.perf_counter() - t
+    rows = int(max(8, min(wl.height, 8 * budget_s / max(dt, 1e-3))))
+    y0 = max(0, wl.height // 2 - rows // 2)
+    t = time.perf_counter()
+    _, cnt = o.render(wl.width, wl.height, wl.seed, 0, 1, y0=y0, y1=y0 + rows, threads=cores)
+    dt = time.perf_counter() - t
+    return {"value": cnt["rays"] / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"rows {y0}-{y0 + rows} of {wl.width}x{wl.height}, 1 pass of 16, {cnt['rays']} rays "
+                      f"in {dt:.2f} s (C oracle, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the path shards by "
+              "image tiles and scenes, which is not exercised here", file=sys.stderr)
+        sys.exit(2)
+
+    from voxeltoy_amd import api, workloads
+
+    wl = workloads.full()
+    ctx = api.Context(0)
+    workloads.load(ctx, wl)
+    cu_w, cv_n = wl.cdf[0].shape[1], len(wl.cdf[1])
+
+    # statistics launch (untimed): exact work of one step
+    ctx.set_stats(True)
+    ctx.reset_accum()
+    ctx.render(wl.seed, 0, wl.spp)
+    st = ctx.stats()
+    ctx.set_stats(False)
+
+    for _ in range(args.warmup):
+        ctx.reset_accum()
+        ctx.render(wl.seed, 0, wl.spp)
+    ctx.synchronize()
+
+    kernel_ms = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.reset_accum()
+        ctx.render(wl.seed, 0, wl.spp)
+        kernel_ms.append(ctx.stats()["kernel_ms"])
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000.0 / args.steps
+    k_ms = sum(kernel_ms) / len(kernel_ms)
+
+    rays = st["rays"]
+    value = rays / (ms_per_step / 1000.0) / 1e6
+    bytes_step = algorithmic_bytes(st, cu_w, cv_n)
+    achieved = bytes_step / (k_ms / 1000.0) / 1e9
+    cpu = None if args.no_cpu_baseline else cpu_baseline(wl, args.cpu_budget)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "full-shading 256^3 terrain+boxes, 70/20/10 lambert/metal/emissive, "
+                               "1024x512 fp32 sky+sun env (importance sampled), 1024x1024 thin lens, "
+                               "16 spp, 8 bounces (plain tracer)",
+                   "grid": 256, "width": wl.width, "height": wl.height, "spp": wl.spp, "bounces": wl.bounces,
+                   "emissive_voxels": int(len(wl.scene.emissive)), "rays_per_step": int(rays),
+                   "kernel_ms": round(k_ms, 3), "counters": {k: int(st[k]) for k in
+                                                             ("visits", "interactions", "samples", "env_samples",
+                                                              "env_evals")}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(bytes_step)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,196 @@
+/*
+ * vt_api.h - C ABI of the MI355X voxel path tracer.
+ *
+ * This is the boundary that replaces the GLSL program "ABI" of voxelToy:
+ * the uniforms and textures bound by Renderer (src/shaders/shader.h:43-81,
+ * src/renderer/glResources.h:9-24, src/renderer/renderer.cpp:215-312) and
+ * the draw calls that launch pathTracer.fs, accumulation.fs and
+ * voxelize.{vs,gs} (src/renderer/renderer.cpp:584-611,
+ * src/voxelize/gpuVoxelizer.cpp:46-72).  Plain C types only; every
+ * function returns VT_OK or a negative status and leaves a message for
+ * vt_last_error().
+ *
+ * Conventions:
+ *   - a context owns all device memory and one HIP stream on one device;
+ *     host arrays passed in are copied (glTexImage* semantics);
+ *   - calls on one context are not reentrant; use one context per host
+ *     thread;
+ *   - images are in GL row order: row 0 is the bottom row;
+ *   - matrices are row-major float[16] (m[r*4+c]), as Imath stores them
+ *     and as the reference uploads them with transpose = GL_TRUE.
+ */
+#ifndef VT_API_H_
+#define VT_API_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VT_OK 0
+#define VT_ERR_ARG -1
+#define VT_ERR_HIP -2
+#define VT_ERR_STATE -3
+#define VT_ERR_LIMIT -4
+
+typedef struct vt_context vt_context;
+
+/* Camera uniforms (pathTracer.fs:14-23, generateRay.h).  lens_model:
+ * 0 pinhole, 1 thin lens, 2 orthographic (CameraParameters::CameraLensModel,
+ * src/camera/cameraParameters.h:23-28).  focal_distance is
+ * FocalDistanceData.focalDistance (focalDistanceDevice.h:5-8). */
+typedef struct vt_camera {
+    float inv_modelview[16];
+    float proj[16];
+    float inv_proj[16];
+    float viewport[4];      /* x, y, w, h */
+    float win_z;            /* gl_FragCoord.z of the full-screen quad */
+    float lens_radius;
+    float focal_distance;
+    int32_t lens_model;
+} vt_camera;
+
+/* Environment uniforms (pathTracer.fs:25-33).  With use_image != 0 the RGB
+ * float image (row 0 = top of the equirect map, v = 0) is sampled
+ * bilinearly with GL_CLAMP borders, and cdf_u ((w'+1) x h'), cdf_v (h'+1)
+ * and integral come from vt_env_build_cdf or the caller. */
+typedef struct vt_environment {
+    int32_t use_image;
+    float top[3];
+    float bottom[3];
+    int32_t image_w, image_h;
+    const float* image_rgb;
+    int32_t cdf_u_w, cdf_u_h;
+    const float* cdf_u;
+    int32_t cdf_v_n;
+    const float* cdf_v;
+    float integral;
+    float rotation_radians;
+} vt_environment;
+
+/* Integrator settings (RenderSettings src/renderer/renderSettings.h:6-29
+ * and the pathTracer.fs uniforms :37-44, SelectVoxelData). */
+typedef struct vt_settings {
+    int32_t max_bounces;
+    float exposure;
+    float gamma;
+    int32_t selected_voxel[3];   /* out of range = no selection */
+    float wireframe_opacity;
+    float wireframe_thickness;
+} vt_settings;
+
+/* Work counters of the last vt_render (filled when stats are enabled). */
+typedef struct vt_stats {
+    uint64_t rays;          /* camera samples + shadow + continuation traversals */
+    uint64_t visits;        /* voxel fetches made by the DDA */
+    uint64_t interactions;  /* surface shading events */
+    uint64_t samples;
+    uint64_t env_samples;   /* importance-sampled image-environment draws */
+    uint64_t env_evals;     /* image-environment lookups by direction */
+    double kernel_ms;       /* HIP-event time of the last render launch */
+} vt_stats;
+
+/* ---- context ------------------------------------------------------------ */
+const char* vt_last_error(void);
+int vt_create(int device, vt_context** out);
+void vt_destroy(vt_context* ctx);
+int vt_synchronize(vt_context* ctx);
+/* Number of compute units of the context's device. */
+int vt_device_cus(vt_context* ctx);
+
+/* ---- volume -------------------------------------------------------------
+ * Replaces Renderer::createVoxelDataTexture (src/renderer/renderer.cpp:833-940):
+ * the GL_R32I material-offset grid (x fastest, -1 = empty), the GL_R32F
+ * material records and the GL_R32I emissive-voxel list.  Bounds follow
+ * the reference (cubic voxels, longest axis 1000 units, centred) unless
+ * bmin/bmax are given (non-NULL).  At most 255 distinct occupied offsets;
+ * offsets must be -1 or lie in [0, mat_size). */
+int vt_set_volume(vt_context* ctx, int32_t nx, int32_t ny, int32_t nz, const int32_t* material_offsets,
+                  const float* mat_data, int32_t mat_size, const int32_t* emissive, int32_t n_emissive,
+                  const float* bmin, const float* bmax);
+/* Same volume from a uint8 ID grid: id_offset[256] gives the material
+ * offset of each ID (-1 = empty). */
+int vt_set_volume_u8(vt_context* ctx, int32_t nx, int32_t ny, int32_t nz, const uint8_t* ids,
+                     const int32_t* id_offset, const float* mat_data, int32_t mat_size,
+                     const int32_t* emissive, int32_t n_emissive, const float* bmin, const float* bmax);
+/* Volume bounds and voxel size actually in use (6 + 3 floats). */
+int vt_get_bounds(vt_context* ctx, float* bmin, float* bmax, float* voxel_size);
+/* Dense readback of the grid as material offsets (-1 empty), x fastest. */
+int vt_read_volume(vt_context* ctx, int32_t* material_offsets);
+/* Material record edits: Renderer::updateMaterialColor/Value
+ * (src/renderer/renderer.cpp:1205-1235). */
+int vt_update_material(vt_context* ctx, int32_t offset, const float* values, int32_t count);
+int vt_read_materials(vt_context* ctx, float* mat_data, int32_t capacity, int32_t* size);
+
+/* ---- render state ------------------------------------------------------- */
+int vt_set_camera(vt_context* ctx, const vt_camera* cam);
+int vt_set_environment(vt_context* ctx, const vt_environment* env);
+int vt_set_settings(vt_context* ctx, const vt_settings* st);
+/* Allocate the W x H accumulation target (sampleTexture/averageTexture,
+ * src/renderer/renderer.cpp:763-795) and clear it.  fp64 != 0 also keeps
+ * double-precision running sums of every sample. */
+int vt_set_frame(vt_context* ctx, int32_t width, int32_t height, int32_t fp64);
+int vt_reset_accum(vt_context* ctx);
+int vt_set_stats(vt_context* ctx, int32_t enabled);
+
+/* ---- rendering ----------------------------------------------------------
+ * Render `count` progressive passes: pass k uses accumulation index
+ * n = n0 + k (accumulation.fs: avg = (s + avg*n)/(n+1)) and RNG frame
+ * min(n, rng_frame_max) (src/renderer/renderer.cpp:594).  Equivalent to
+ * `count` calls of Renderer::render(). */
+int vt_render(vt_context* ctx, uint64_t seed, uint32_t n0, uint32_t count, uint32_t rng_frame_max);
+/* RGBA32F running mean, W*H*4 floats, row 0 = bottom. */
+int vt_read_accum(vt_context* ctx, float* rgba);
+/* FP64 running sums (W*H*3 doubles) when vt_set_frame(fp64=1). */
+int vt_read_sum64(vt_context* ctx, double* rgb);
+int vt_get_stats(vt_context* ctx, vt_stats* out);
+
+/* ---- probes -------------------------------------------------------------
+ * First hit of every pixel's primary ray of frame `frame`: rec[8] per
+ * pixel = kind (0 AABB miss, 1 traversal miss, 2 voxel, 3 ground), voxel
+ * x, y, z, DDA steps (-1 if the march started outside the grid), face
+ * normal nx, ny, nz. */
+int vt_trace_first_hit(vt_context* ctx, uint64_t seed, uint32_t frame, int32_t* rec);
+/* Injected rays (origin[3], dir[3] each), same record. */
+int vt_trace_rays(vt_context* ctx, int32_t n, const float* origins, const float* dirs, int32_t* rec);
+
+/* ---- voxelizer ----------------------------------------------------------
+ * GPUVoxelizer::voxelizeMesh (src/voxelize/gpuVoxelizer.cpp:46-72 with
+ * voxelize.vs/gs).  Vertices are float xyz, triangles uint32 index
+ * triples; model_transform is the row-major mesh transform
+ * (computeMeshTransform, src/renderer/import.cpp:46-64).  mode 0 = THIN
+ * (reference default), 1 = FAT.  The grid must already exist
+ * (vt_set_volume); covered voxels are set to material offset
+ * `material_offset` (must be a known offset).  If occupancy != NULL the
+ * dense byte occupancy (1 = covered) of this call alone is also written
+ * there. */
+int vt_voxelize(vt_context* ctx, const float* vertices, int64_t n_vertices, const uint32_t* indices,
+                int64_t n_triangles, const float* model_transform, int32_t mode, int32_t material_offset,
+                uint8_t* occupancy);
+
+/* ---- host helpers (same arithmetic as the reference host code) ---------
+ * updateCamera matrices (src/renderer/renderer.cpp:343-446). */
+int vt_camera_matrices(const float* eye, const float* target, float fov_y, float near_d, float far_d,
+                       int32_t lens_model, int32_t width, int32_t height, float* modelview,
+                       float* inv_modelview, float* proj, float* inv_proj);
+/* calculateCDF (src/renderer/image.cpp:104-319) over an RGB float image:
+ * writes the function-image size, CDF U ((w'+1) x h'), CDF V (h'+1) and the
+ * integral.  cdf_u/cdf_v may be NULL to query sizes. */
+int vt_env_build_cdf(const float* rgb, int32_t w, int32_t h, int32_t* cdf_u_w, int32_t* cdf_u_h,
+                     float* cdf_u, int32_t* cdf_v_n, float* cdf_v, float* integral);
+/* pruneInteriorEmissiveVoxels (src/renderer/import.cpp:227-250); returns
+ * the new count. */
+int32_t vt_prune_emissive(const int32_t* material_offsets, int32_t nx, int32_t ny, int32_t nz,
+                          int32_t* emissive, int32_t n_emissive);
+
+/* Device evaluation of the elementary functions used by the kernels, for
+ * checking that device and host arithmetic agree bit for bit.  fn: 0 sqrt,
+ * 1 a/b, 2 sin, 3 cos, 4 acos, 5 atan2(a,b), 6 log2, 7 exp2, 8 pow(a,b). */
+int vt_device_math(vt_context* ctx, int32_t fn, int32_t n, const float* a, const float* b, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1225 @@
+/*
+ * vt_oracle.c - CPU restatement of voxelToy's rendering hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see vt_oracle.h).  Built with
+ * -O2 -ffp-contract=off -fno-fast-math so every expression rounds exactly as
+ * written.  Expressions keep the reference's operand order; where GLSL leaves
+ * an order or a built-in open, the choice made here is the documented
+ * assumption shared with the GPU kernels:
+ *   - dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z, no fused multiply-add;
+ *   - normalize(v) = v * (1/sqrt(dot(v,v))), length(v) = sqrt(dot(v,v));
+ *   - mat4 * vec4 sums column products left to right;
+ *   - min/max/step/sign/clamp/mix as the GLSL specification defines them;
+ *   - sin, cos, acos, atan, pow from vtmath.h (shared with the kernels);
+ *   - out-of-range texelFetch returns 0;
+ *   - texture() on the environment is GL_LINEAR with GL_CLAMP: coordinates
+ *     clamped to [0,1], texels outside the image read the black border.
+ * The elementary functions come from the product's vtmath.h so both sides
+ * round identically; they are pinned against libm separately.
+ */
+#include "vt_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <limits.h>
+
+#define VT_FN static inline
+#include "../voxeltoy_amd/csrc/vtmath.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { float x, y, z; } v3;
+
+static inline v3 mk(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+static inline v3 ld(const float* p) { return mk(p[0], p[1], p[2]); }
+static inline v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 mul(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 mulf(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline v3 divf(v3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+static inline v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline float dot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline v3 cross(v3 a, v3 b)
+{
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline v3 normalize(v3 v) { const float inv = 1.0f / sqrtf(dot(v, v)); return mulf(v, inv); }
+static inline float length3(v3 v) { return sqrtf(dot(v, v)); }
+
+/* mat4 (row-major m[r*4+c]) times (x,y,z,w): column products left to right. */
+static inline void mat4_mul(const float* m, float x, float y, float z, float w, float* out)
+{
+    for (int r = 0; r < 4; ++r)
+        out[r] = ((m[r * 4 + 0] * x + m[r * 4 + 1] * y) + m[r * 4 + 2] * z) + m[r * 4 + 3] * w;
+}
+
+/* ---------------------------------------------------------------------------
+ * exposed elementary functions
+ * ------------------------------------------------------------------------- */
+float orc_powf(float x, float y) { return vt_powf(x, y); }
+void orc_sincosf(float x, float* s, float* c) { vt_sincosf(x, s, c); }
+float orc_atan2f(float y, float x) { return vt_atan2f(y, x); }
+float orc_acosf(float x) { return vt_acosf(x); }
+float orc_log2f(float x) { return vt_log2f(x); }
+float orc_exp2f(float x) { return vt_exp2f(x); }
+
+void orc_pcg32_raw(uint64_t initstate, uint64_t initseq, int n, uint32_t* out)
+{
+    vt_pcg32 r;
+    vt_pcg32_seed(&r, initstate, initseq);
+    for (int i = 0; i < n; ++i) out[i] = vt_pcg32_next(&r);
+}
+
+void orc_pcg32_stream(uint64_t seed, uint32_t pixel, uint32_t frame, int n, float* out)
+{
+    vt_pcg32 r;
+    vt_rng_init(&r, seed, pixel, frame);
+    for (int i = 0; i < n; ++i) out[i] = vt_u01(vt_pcg32_next(&r));
+}
+
+/* ---------------------------------------------------------------------------
+ * texel fetches (out of range reads 0, as texelFetch does on llvmpipe)
+ * ------------------------------------------------------------------------- */
+static inline int32_t fetch_offset(const orc_scene* s, int32_t x, int32_t y, int32_t z)
+{
+    if (x < 0 || y < 0 || z < 0 || x >= s->res[0] || y >= s->res[1] || z >= s->res[2]) return 0;
+    return s->offsets[(size_t)x + (size_t)y * (size_t)s->res[0] + (size_t)z * (size_t)s->res[0] * (size_t)s->res[1]];
+}
+
+static inline float fetch_mat(const orc_scene* s, int32_t i)
+{
+    return (i < 0 || i >= s->mat_size) ? 0.0f : s->mat[i];
+}
+
+/* int(float) of the material type: anything that is not 0, 1, 2 falls
+ * through to the default branch. */
+static inline int material_type(const orc_scene* s, int32_t off)
+{
+    const float t = fetch_mat(s, off);
+    if (!(t >= 0.0f && t < 3.0f)) return -1;
+    return (int)t;
+}
+
+/* ---------------------------------------------------------------------------
+ * rayAABBIntersection  (src/shaders/shared/aabb.h:1-32)
+ * ------------------------------------------------------------------------- */
+static float aabb(v3 o, v3 d, v3 bmin, v3 bmax)
+{
+    const v3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const v3 t0 = mul(sub(bmin, o), inv);
+    const v3 t1 = mul(sub(bmax, o), inv);
+    const v3 tmin = mk(vt_minf(t0.x, t1.x), vt_minf(t0.y, t1.y), vt_minf(t0.z, t1.z));
+    const float tminf = vt_maxf(tmin.x, vt_maxf(tmin.y, tmin.z));
+    const v3 tmax = mk(vt_maxf(t0.x, t1.x), vt_maxf(t0.y, t1.y), vt_maxf(t0.z, t1.z));
+    const float tmaxf = vt_minf(tmax.x, vt_minf(tmax.y, tmax.z));
+    if (tmaxf < 0.0f) return -1.0f;
+    if (tminf > tmaxf) return -1.0f;
+    return vt_maxf(0.0f, tminf);
+}
+
+float orc_aabb(const float o[3], const float d[3], const float bmin[3], const float bmax[3])
+{
+    return aabb(ld(o), ld(d), ld(bmin), ld(bmax));
+}
+
+/* ---------------------------------------------------------------------------
+ * raymarch / traverse  (src/shaders/shared/dda.h:7-100)
+ * ------------------------------------------------------------------------- */
+int orc_max_steps(const int32_t res[3])
+{
+    /* int(2 * ceil(length(vec3(voxelResolution))))  dda.h:98 */
+    const v3 r = mk((float)res[0], (float)res[1], (float)res[2]);
+    return (int)(2.0f * ceilf(length3(r)));
+}
+
+static int raymarch(const orc_scene* s, v3 o, v3 d, int max_steps, float vp_out[3], int32_t* steps_out,
+                    int32_t* wrote, uint64_t* visits)
+{
+    const v3 bmin = ld(s->bmin), bmax = ld(s->bmax);
+    const v3 res = mk((float)s->res[0], (float)s->res[1], (float)s->res[2]);
+    const v3 ext = mk(1.0f / (bmax.x - bmin.x), 1.0f / (bmax.y - bmin.y), 1.0f / (bmax.z - bmin.z)); /* :16 */
+    /* :19 wsRayOrigin += sign(wsRayDir) * ISECT_EPSILON */
+    o = add(o, mulf(mk(vt_signf(d.x), vt_signf(d.y), vt_signf(d.z)), 0.001f));
+    /* :20 */
+    const v3 vo = mul(mul(sub(o, bmin), ext), res);
+    float vp[3] = { floorf(vo.x), floorf(vo.y), floorf(vo.z) };               /* :22 */
+    if (vp[0] < 0.0f || vp[1] < 0.0f || vp[2] < 0.0f ||
+        vp[0] >= res.x || vp[1] >= res.y || vp[2] >= res.z) {                  /* :24-25 */
+        *wrote = 0;
+        *steps_out = -1;
+        return 0;
+    }
+    /* :29 mix(d, vec3(1e-5), step(abs(d), vec3(1e-5))) */
+    const float dd[3] = { d.x, d.y, d.z };
+    float dc[3], inc[3], sgn[3], dis[3];
+    const float vo3[3] = { vo.x, vo.y, vo.z };
+    for (int i = 0; i < 3; ++i) {
+        const float t = vt_stepf(vt_absf(dd[i]), 1e-5f);
+        dc[i] = t != 0.0f ? 1e-5f : dd[i];
+        inc[i] = 1.0f / dc[i];                                                  /* :31 */
+        sgn[i] = vt_signf(dc[i]);                                               /* :32 */
+        dis[i] = (((vp[i] - vo3[i]) + 0.5f) + sgn[i] * 0.5f) * inc[i];          /* :34 */
+    }
+    const float resa[3] = { res.x, res.y, res.z };
+    int isect = 0;
+    int steps = 0;
+    while (steps < max_steps) {                                                 /* :38 */
+        if (vp[0] < 0.0f || vp[1] < 0.0f || vp[2] < 0.0f ||
+            vp[0] >= resa[0] || vp[1] >= resa[1] || vp[2] >= resa[2]) break;    /* :41-42 */
+        if (visits) (*visits)++;
+        if (fetch_offset(s, (int32_t)vp[0], (int32_t)vp[1], (int32_t)vp[2]) >= 0) { /* :44-45 */
+            isect = 1;
+            break;
+        }
+        /* :51 mask = step(dis.xyz, dis.yxy) * step(dis.xyz, dis.zzx) */
+        const float m0 = vt_stepf(dis[0], dis[1]) * vt_stepf(dis[0], dis[2]);
+        const float m1 = vt_stepf(dis[1], dis[0]) * vt_stepf(dis[1], dis[2]);
+        const float m2 = vt_stepf(dis[2], dis[1]) * vt_stepf(dis[2], dis[0]);
+        const float m[3] = { m0, m1, m2 };
+        for (int i = 0; i < 3; ++i) {
+            dis[i] = dis[i] + (m[i] * sgn[i]) * inc[i];                         /* :52 */
+            vp[i] = vp[i] + m[i] * sgn[i];                                      /* :53 */
+        }
+        steps++;
+    }
+    vp_out[0] = vp[0]; vp_out[1] = vp[1]; vp_out[2] = vp[2];
+    *steps_out = steps;
+    *wrote = 1;
+    return isect;
+}
+
+/* traverse (dda.h:63-79): a march that ends below the grid (y < 0) counts
+ * as a ground hit.  A march that never started (origin outside the grid)
+ * leaves vsHitPos unwritten; that case is taken as a miss. */
+static int traverse(const orc_scene* s, v3 o, v3 d, float vp[3], int32_t* steps, int32_t* ground,
+                    uint64_t* visits)
+{
+    int32_t wrote = 0;
+    const int max_steps = orc_max_steps(s->res);
+    if (raymarch(s, o, d, max_steps, vp, steps, &wrote, visits)) { *ground = 0; return 1; }
+    if (!wrote) { *ground = 0; return 0; }
+    *ground = vp[1] < 0.0f;
+    return *ground;
+}
+
+int orc_raymarch(const orc_scene* s, const float o[3], const float d[3], int max_steps,
+                 int32_t vp[3], int32_t* steps, int32_t* wrote)
+{
+    float v[3] = { 0, 0, 0 };
+    const int r = raymarch(s, ld(o), ld(d), max_steps, v, steps, wrote, 0);
+    vp[0] = (int32_t)v[0]; vp[1] = (int32_t)v[1]; vp[2] = (int32_t)v[2];
+    return r;
+}
+
+int orc_traverse(const orc_scene* s, const float o[3], const float d[3], int32_t vp[3], int32_t* steps,
+                 int32_t* kind)
+{
+    float v[3] = { 0, 0, 0 };
+    int32_t ground = 0;
+    const int r = traverse(s, ld(o), ld(d), v, steps, &ground, 0);
+    vp[0] = (int32_t)v[0]; vp[1] = (int32_t)v[1]; vp[2] = (int32_t)v[2];
+    *kind = r ? (ground ? 3 : 2) : 0;
+    return r;
+}
+
+/* ---------------------------------------------------------------------------
+ * voxelSpaceToWorldSpace  (src/shaders/shared/coordinates.h:35-82)
+ * ------------------------------------------------------------------------- */
+typedef struct { v3 p, t, n, b; } basis;
+
+static basis voxel_basis(const orc_scene* s, const float vsP[3], v3 o, v3 d)
+{
+    basis B;
+    const v3 vs = ld(s->vsize), bmin = ld(s->bmin);
+    const v3 vmin = add(mul(ld(vsP), vs), bmin);                 /* :65 */
+    const v3 vmax = add(vmin, vs);                                /* :66 */
+    const float t = aabb(o, d, vmin, vmax);                       /* :67 */
+    B.p = add(o, mulf(d, t));                                     /* :69 */
+    const v3 c = add(vmin, mulf(vs, 0.5f));                       /* :71 */
+    const v3 c2h = sub(B.p, c);                                   /* :72 */
+    const v3 a = mk(vt_absf(c2h.x), vt_absf(c2h.y), vt_absf(c2h.z));
+    /* :74 mask = step(abs.yxx, abs.xyz) * step(abs.zzy, abs.xyz) */
+    const v3 m = mk(vt_stepf(a.y, a.x) * vt_stepf(a.z, a.x),
+                    vt_stepf(a.x, a.y) * vt_stepf(a.z, a.y),
+                    vt_stepf(a.x, a.z) * vt_stepf(a.y, a.z));
+    B.n = mul(m, mk(vt_signf(c2h.x), vt_signf(c2h.y), vt_signf(c2h.z)));  /* :75 */
+    const float k = 0.57735026919f;
+    const v3 tan0 = cross(B.n, mk(k, k, k));                      /* :79 */
+    B.b = normalize(cross(tan0, B.n));                            /* :80 */
+    B.t = normalize(cross(B.b, B.n));                             /* :81 */
+    return B;
+}
+
+void orc_basis(const orc_scene* s, const int32_t vp[3], const float o[3], const float d[3], float out[12])
+{
+    const float v[3] = { (float)vp[0], (float)vp[1], (float)vp[2] };
+    const basis B = voxel_basis(s, v, ld(o), ld(d));
+    const v3 arr[4] = { B.p, B.t, B.n, B.b };
+    for (int i = 0; i < 4; ++i) { out[3 * i] = arr[i].x; out[3 * i + 1] = arr[i].y; out[3 * i + 2] = arr[i].z; }
+}
+
+static inline v3 local_to_world(v3 v, const basis* B)   /* coordinates.h:43-49 */
+{
+    return add(add(mulf(B->t, v.x), mulf(B->n, v.y)), mulf(B->b, v.z));
+}
+static inline v3 world_to_local(v3 v, const basis* B)   /* coordinates.h:51-57 */
+{
+    return mk(dot(v, B->t), dot(v, B->n), dot(v, B->b));
+}
+
+/* ---------------------------------------------------------------------------
+ * camera rays  (src/shaders/shared/generateRay.h:1-123, coordinates.h:1-33)
+ * ------------------------------------------------------------------------- */
+static void screen_to_eye_persp(const orc_camera* c, float wx, float wy, float wz, float eye[4])
+{
+    float ndc[3];
+    ndc[0] = ((2.0f * wx) - (2.0f * c->viewport[0])) / c->viewport[2] - 1.0f;   /* coordinates.h:16 */
+    ndc[1] = ((2.0f * wy) - (2.0f * c->viewport[1])) / c->viewport[3] - 1.0f;
+    ndc[2] = ((2.0f * wz) - 0.0f - 1.0f) / (1.0f - 0.0f);                       /* :17, DepthRange [0,1] */
+    /* cameraProj[3][2] = P row 2 col 3; [2][2] = row 2 col 2; [2][3] = row 3 col 2 */
+    const float P32 = c->proj[2 * 4 + 3], P22 = c->proj[2 * 4 + 2], P23 = c->proj[3 * 4 + 2];
+    const float w = P32 / (ndc[2] - (P22 / P23));                                 /* :22 */
+    mat4_mul(c->inv_proj, ndc[0] * w, ndc[1] * w, ndc[2] * w, w, eye);            /* :23-25 */
+}
+
+static void gen_ray(const orc_camera* c, float fx, float fy, const float u[4], v3* o, v3* d)
+{
+    if (c->lens_model == 0) {
+        /* generateRay_Pinhole :10-26 */
+        const float jx = u[0] - 0.5f, jy = u[1] - 0.5f;
+        float es[4], tmp[4];
+        screen_to_eye_persp(c, fx + jx, fy + jy, c->win_z + 0.0f, es);
+        mat4_mul(c->inv_mv, 0.0f, 0.0f, 0.0f, 1.0f, tmp);
+        *o = mk(tmp[0], tmp[1], tmp[2]);
+        const v3 ed = normalize(mk(es[0], es[1], es[2]));
+        mat4_mul(c->inv_mv, ed.x, ed.y, ed.z, 0.0f, tmp);
+        /* normalize of the vec4 (w = 0 term included) */
+        const float l2 = ((tmp[0] * tmp[0] + tmp[1] * tmp[1]) + tmp[2] * tmp[2]) + tmp[3] * tmp[3];
+        const float inv = 1.0f / sqrtf(l2);
+        *d = mk(tmp[0] * inv, tmp[1] * inv, tmp[2] * inv);
+    } else if (c->lens_model == 1) {
+        /* generateRay_ThinLens :30-101; uniformlySampleDisk sampling.h:2-7 */
+        const float r = sqrtf(u[0]);
+        float sn, cs;
+        vt_sincosf((2.0f * VT_PI_F) * u[1], &sn, &cs);
+        const float lx = (r * cs) * c->lens_radius, ly = (r * sn) * c->lens_radius;
+        float eo[4], tmp[4];
+        screen_to_eye_persp(c, fx, fy, c->win_z, eo);
+        const v3 esO = mk(eo[0], eo[1], eo[2]);
+        const v3 esD = normalize(esO);
+        const float t = c->focal_distance / -(esD.z);
+        const v3 esF = add(esO, mulf(esD, t));
+        mat4_mul(c->inv_mv, esF.x, esF.y, esF.z, 1.0f, tmp);
+        const v3 wsF = mk(tmp[0], tmp[1], tmp[2]);
+        mat4_mul(c->inv_mv, lx, ly, 0.0f, 1.0f, tmp);
+        *o = mk(tmp[0], tmp[1], tmp[2]);
+        *d = normalize(sub(wsF, *o));
+    } else {
+        /* generateRay_Orthographic :1-7 with screenToEyeSpaceOrthographic */
+        float ndc[3], eye[4], tmp[4];
+        ndc[0] = (fx / c->viewport[2]) * 2.0f - 1.0f;
+        ndc[1] = (fy / c->viewport[3]) * 2.0f - 1.0f;
+        ndc[2] = ((2.0f * c->win_z) - 0.0f - 1.0f) / (1.0f - 0.0f);
+        mat4_mul(c->inv_proj, ndc[0], ndc[1], ndc[2], 1.0f, eye);
+        mat4_mul(c->inv_mv, eye[0], eye[1], eye[2], eye[3], tmp);
+        *o = mk(tmp[0], tmp[1], tmp[2]);
+        mat4_mul(c->inv_mv, 0.0f, 0.0f, -1.0f, 0.0f, tmp);
+        *d = mk(tmp[0], tmp[1], tmp[2]);
+    }
+}
+
+void orc_generate_ray(const orc_camera* cam, float fx, float fy, const float u[4], float o[3], float d[3])
+{
+    v3 oo, dd;
+    gen_ray(cam, fx, fy, u, &oo, &dd);
+    o[0] = oo.x; o[1] = oo.y; o[2] = oo.z;
+    d[0] = dd.x; d[1] = dd.y; d[2] = dd.z;
+}
+
+/* ---------------------------------------------------------------------------
+ * sampling utilities  (src/shaders/shared/sampling.h)
+ * ------------------------------------------------------------------------- */
+static inline float power_heuristic(float f, float g) { return (f * f) / ((f * f) + (g * g)); }  /* :40-43 */
+
+/* ---------------------------------------------------------------------------
+ * environment  (src/shaders/shared/lights.h, coordinates.h:84-116,
+ *               src/shaders/envLight/envMapSample.h)
+ * ------------------------------------------------------------------------- */
+static void tex_bilinear(const orc_env* e, float u, float v, float rgb[3])
+{
+    if (vt_isnan(u) || vt_isnan(v)) { rgb[0] = rgb[1] = rgb[2] = vt_nanf(); return; }
+    u = vt_minf(vt_maxf(u, 0.0f), 1.0f);
+    v = vt_minf(vt_maxf(v, 0.0f), 1.0f);
+    const float s = u * (float)e->img_w - 0.5f;
+    const float t = v * (float)e->img_h - 0.5f;
+    const float fs = floorf(s), ft = floorf(t);
+    const int i0 = (int)fs, j0 = (int)ft;
+    const float a = s - fs, b = t - ft;
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    for (int ch = 0; ch < 3; ++ch) {
+        float tx[4];
+        const int ii[4] = { i0, i0 + 1, i0, i0 + 1 };
+        const int jj[4] = { j0, j0, j0 + 1, j0 + 1 };
+        for (int k = 0; k < 4; ++k) {
+            const int i = ii[k], j = jj[k];
+            tx[k] = (i >= 0 && j >= 0 && i < e->img_w && j < e->img_h)
+                        ? e->img[((size_t)j * (size_t)e->img_w + (size_t)i) * 3 + ch] : 0.0f;
+        }
+        rgb[ch] = ((w00 * tx[0] + w10 * tx[1]) + w01 * tx[2]) + w11 * tx[3];
+    }
+}
+
+void orc_env_texture(const orc_env* e, float u, float v, float rgb[3]) { tex_bilinear(e, u, v, rgb); }
+
+static void uv_from_dir(v3 d, float rot, float uv[2])   /* coordinates.h:97-109 */
+{
+    const float theta = vt_acosf(d.y);
+    const float phi = (vt_atan2f(d.z, d.x) + VT_PI_F) + rot;
+    const float a = phi / VT_TWO_PI_F;
+    uv[0] = a - floorf(a);
+    uv[1] = theta / VT_PI_F;
+}
+
+void orc_uv_from_dir(const float d[3], float rotation, float uv[2]) { uv_from_dir(ld(d), rotation, uv); }
+
+static v3 dir_from_uv(float u, float v, float rot)   /* coordinates.h:111-116, 85-89 */
+{
+    const float phi = ((u * VT_TWO_PI_F) - VT_PI_F) - rot;
+    const float theta = v * VT_PI_F;
+    float st, ct, sp, cp;
+    vt_sincosf(theta, &st, &ct);
+    vt_sincosf(phi, &sp, &cp);
+    return mk(st * cp, ct, st * sp);
+}
+
+static inline float luminance(v3 c)   /* color.h:1-8 */
+{
+    return (c.x * 0.2126f + c.y * 0.7152f) + c.z * 0.0722f;
+}
+
+static v3 background(const orc_env* e, v3 v, orc_counters* cnt)   /* lights.h:4-17 */
+{
+    if (e->use_image) {
+        float uv[2], rgb[3];
+        uv_from_dir(v, e->rotation, uv);
+        tex_bilinear(e, uv[0], uv[1], rgb);
+        if (cnt) cnt->env_evals++;
+        return mk(rgb[0], rgb[1], rgb[2]);
+    }
+    const float bias = vt_maxf(0.0f, v.y);
+    return add(mulf(ld(e->bottom), 1.0f - bias), mulf(ld(e->top), bias));
+}
+
+static v3 eval_env(const orc_env* e, v3 wi, float* pdf, orc_counters* cnt)   /* lights.h:20-33 */
+{
+    if (e->use_image) {
+        const v3 L = background(e, wi, cnt);
+        *pdf = luminance(L) / e->integral;
+        return L;
+    }
+    *pdf = 1.0f / (4.0f * VT_PI_F);
+    return background(e, wi, cnt);
+}
+
+static int bsearch_cdf(const float* cdf, int hi, float x)   /* envMapSample.h:70-123 */
+{
+    int lo = 0;
+    if (hi < 1) return 0;
+    while (lo != hi - 1) {
+        const int mid = (lo + hi) / 2;
+        if (x < cdf[mid]) hi = mid; else lo = mid;
+    }
+    return lo;
+}
+
+static inline float cdf_at(const float* cdf, int n, int i) { return (i < 0 || i >= n) ? 0.0f : cdf[i]; }
+
+static void env_sample_uv(const orc_env* e, float su, float sv, float uv[2], int32_t idx[2])  /* :23-65 */
+{
+    const int sizeU = e->cdfu_w, sizeV = e->cdfv_n;
+    const int row = bsearch_cdf(e->cdfv, sizeV - 1, sv);
+    const int col = bsearch_cdf(e->cdfu + (size_t)row * (size_t)sizeU, sizeU - 1, su);
+    const float* rowp = e->cdfu + (size_t)row * (size_t)sizeU;
+    const float loU = cdf_at(rowp, sizeU, col), upU = cdf_at(rowp, sizeU, col + 1);
+    const float du = (su - loU) / (upU - loU);
+    const float loV = cdf_at(e->cdfv, sizeV, row), upV = cdf_at(e->cdfv, sizeV, row + 1);
+    const float dv = (sv - loV) / (upV - loV);
+    uv[0] = ((float)col + du) / (float)(sizeU - 1);
+    uv[1] = ((float)row + dv) / (float)(sizeV - 1);
+    if (idx) { idx[0] = col; idx[1] = row; }
+}
+
+void orc_env_sample_uv(const orc_env* e, float su, float sv, float uv[2], int32_t idx[2])
+{
+    env_sample_uv(e, su, sv, uv, idx);
+}
+
+/* sampleEnvironmentRadiance lights.h:36-55 */
+static v3 sample_env(const orc_env* e, const basis* B, float u0, float u1, v3* wi, float* pdf,
+                     orc_counters* cnt)
+{
+    if (e->use_image) {
+        float uv[2], rgb[3];
+        env_sample_uv(e, u0, u1, uv, 0);
+        *wi = dir_from_uv(uv[0], uv[1], e->rotation);
+        tex_bilinear(e, uv[0], uv[1], rgb);
+        const v3 L = mk(rgb[0], rgb[1], rgb[2]);
+        *pdf = luminance(L) / e->integral;
+        if (cnt) cnt->env_samples++;
+        return L;
+    }
+    /* uniformlySampledHemisphere sampling.h:27-36 */
+    const float y = u0;
+    const float r = sqrtf(vt_maxf(0.0f, 1.0f - y * y));
+    const float phi = (u1 * 2.0f) * VT_PI_F;
+    float sn, cs;
+    vt_sincosf(phi, &sn, &cs);
+    const v3 ls = mk(r * cs, y, r * sn);
+    *pdf = 1.0f / (2.0f * VT_PI_F);
+    *wi = local_to_world(ls, B);
+    return background(e, *wi, cnt);
+}
+
+/* ---------------------------------------------------------------------------
+ * BSDFs  (src/shaders/bsdf/lambertian.h, microfacet.h)
+ * ------------------------------------------------------------------------- */
+typedef struct { v3 f; float pdf; } fpdf;
+
+static float G_term(v3 wo, v3 wi, v3 wh)   /* microfacet.h:5-15 */
+{
+    const float NdotWh = wh.y, NdotWo = wo.y, NdotWi = wi.y;
+    const float WoDotWh = vt_absf(dot(wo, wh));
+    return vt_minf(1.0f, vt_minf(((2.0f * NdotWh) * NdotWo) / WoDotWh, ((2.0f * NdotWh) * NdotWi) / WoDotWh));
+}
+
+static fpdf D_term(v3 refl, float e, v3 wo, v3 wh)   /* :19-37 */
+{
+    fpdf r;
+    const float pc = vt_powf(wh.y, e);
+    const float f = ((e + 2.0f) * VT_INV_TWOPI_F) * pc;
+    const float woDotWh = dot(wo, wh);
+    r.pdf = (woDotWh <= 0.0f) ? 0.0f : ((e + 1.0f) * pc) / ((VT_TWO_PI_F * 4.0f) * woDotWh);
+    r.f = mulf(refl, f);
+    return r;
+}
+
+static float F_term(float c)   /* :64-69 */
+{
+    const float ior = 7.3f;
+    const float sq = (1.0f - ior) / (1.0f + ior);
+    const float r0 = sq * sq;
+    return r0 + (1.0f - r0) * vt_powf(1.0f - c, 5.0f);
+}
+
+static fpdf eval_microfacet(v3 refl, float e, v3 wo, v3 wi)   /* :73-91 */
+{
+    const float cWi = wi.y, cWo = wo.y;
+    const v3 wh = normalize(add(wi, wo));
+    const float cH = dot(wi, wh);
+    fpdf r = D_term(refl, e, wo, wh);
+    const float k = (G_term(wo, wi, wh) * F_term(cH)) / ((4.0f * cWo) * cWi);
+    r.f = mulf(r.f, k);
+    return r;
+}
+
+static v3 sample_microfacet(v3 refl, float e, v3 wo, vt_pcg32* rng, fpdf* out)   /* :102-123, 41-61 */
+{
+    float u[4];
+    vt_rand4(rng, u);
+    const float cosT = vt_powf(u[0], 1.0f / (e + 1.0f));
+    const float sinT = sqrtf(vt_maxf(0.0f, 1.0f - cosT * cosT));
+    const float phi = (u[1] * 2.0f) * VT_PI_F;
+    float sp, cp;
+    vt_sincosf(phi, &sp, &cp);
+    const v3 wh = mk(sinT * cp, cosT, sinT * sp);
+    const v3 wi = add(neg(wo), mulf(wh, 2.0f * dot(wo, wh)));
+    *out = D_term(refl, e, wo, wh);
+    const float cWi = wi.y, cWo = wo.y;
+    const v3 wh2 = normalize(add(wi, wo));
+    const float cH = dot(wi, wh2);
+    const v3 t1 = mul(refl, out->f);
+    const v3 t2 = mulf(t1, G_term(wo, wi, wh2));
+    const v3 t3 = mulf(t2, F_term(cH));
+    out->f = divf(t3, (4.0f * cWo) * cWi);
+    return wi;
+}
+
+/* ---------------------------------------------------------------------------
+ * materials  (src/shaders/materials/materials.h, matte.h, metal.h, plastic.h)
+ * ------------------------------------------------------------------------- */
+static v3 mat_vec(const orc_scene* s, int32_t base)
+{
+    return mk(fetch_mat(s, base), fetch_mat(s, base + 1), fetch_mat(s, base + 2));
+}
+
+static v3 emission(const orc_scene* s, int32_t off)   /* materials.h:46-56 */
+{
+    const int t = material_type(s, off);
+    if (t < 0) return mk(0, 0, 0);
+    return mat_vec(s, off + 1);
+}
+
+static fpdf eval_material(const orc_scene* s, int32_t off, v3 wo, v3 wi)   /* materials.h:7-19 */
+{
+    fpdf r;
+    switch (material_type(s, off)) {
+        case 0: {   /* matte.h:1-10, lambertian.h:23-29 */
+            const v3 albedo = mat_vec(s, off + 1 + 3);
+            r.pdf = wi.y / VT_PI_F;
+            r.f = divf(albedo, VT_PI_F);
+            return r;
+        }
+        case 1:     /* metal.h:1-17 */
+            return eval_microfacet(mat_vec(s, off + 1 + 3), fetch_mat(s, off + 1 + 6), wo, wi);
+        case 2:     /* plastic.h:1-17 */
+            return eval_microfacet(mk(1, 1, 1), fetch_mat(s, off + 1 + 6), wo, wi);
+        default:
+            r.f = mk(0, 0, 0); r.pdf = 0.0f;
+            return r;
+    }
+}
+
+static v3 sample_material(const orc_scene* s, int32_t off, v3 wo, vt_pcg32* rng, fpdf* out)  /* :30-43 */
+{
+    switch (material_type(s, off)) {
+        case 0: {   /* matte.h:12-22, lambertian.h:10-19, sampling.h:2-22 */
+            const v3 albedo = mat_vec(s, off + 1 + 3);
+            float u[4];
+            vt_rand4(rng, u);
+            const float r = sqrtf(u[0]);
+            float sn, cs;
+            vt_sincosf((2.0f * VT_PI_F) * u[1], &sn, &cs);
+            const float px = r * cs, py = r * sn;
+            const float y = sqrtf(vt_maxf(0.0f, (1.0f - px * px) - py * py));
+            out->f = divf(albedo, VT_PI_F);
+            out->pdf = y / VT_PI_F;
+            return mk(px, y, py);
+        }
+        case 1:
+            return sample_microfacet(mat_vec(s, off + 1 + 3), fetch_mat(s, off + 1 + 6), wo, rng, out);
+        case 2:
+            return sample_microfacet(mk(1, 1, 1), fetch_mat(s, off + 1 + 6), wo, rng, out);
+        default:
+            out->f = mk(0, 0, 0); out->pdf = 0.0f;
+            return mk(0, 0, 0);
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * directLighting  (src/shaders/integrator/pathTracer.fs:64-170)
+ * ------------------------------------------------------------------------- */
+static v3 direct_lighting(const orc_scene* s, const orc_env* e, int32_t off, const basis* B, v3 wo,
+                          vt_pcg32* rng, orc_counters* cnt)
+{
+    float u[4];
+    vt_rand4(rng, u);                                                 /* :77 */
+    const int numLights = s->n_emissive + 1;                          /* :78 */
+    const int lightIndex = (int)(u[0] * (float)numLights);            /* :79 */
+    const int sampling_emissive = lightIndex < numLights - 1;         /* :81 */
+    v3 L, wi;
+    float pdf;
+    int32_t vsE[3] = { 0, 0, 0 };
+    if (sampling_emissive) {
+        int32_t idx = s->emissive[lightIndex];                        /* :85 */
+        const int32_t dz = s->res[0] * s->res[1], dy = s->res[0];      /* coordinates.h:118-128 */
+        const int32_t z = idx / dz; idx -= z * dz;
+        const int32_t y = idx / dy; idx -= y * dy;
+        vsE[0] = idx; vsE[1] = y; vsE[2] = z;
+        const int32_t offE = fetch_offset(s, vsE[0], vsE[1], vsE[2]);  /* :87 */
+        L = mulf(emission(s, offE), 10.0f);                           /* :88 vec3(10) * emission */
+        const v3 res = mk((float)s->res[0], (float)s->res[1], (float)s->res[2]);
+        const v3 vsEf = mk((float)vsE[0], (float)vsE[1], (float)vsE[2]);
+        v3 wsE = add(mul(mk(vsEf.x / res.x, vsEf.y / res.y, vsEf.z / res.z), sub(ld(s->bmax), ld(s->bmin))),
+                     ld(s->bmin));                                    /* :90 */
+        wsE = add(wsE, mul(mk(u[1], u[2], u[3]), ld(s->vsize)));      /* :92 */
+        const v3 toLight = sub(wsE, B->p);                            /* :94 */
+        const float r = length3(toLight);                             /* :95 */
+        wi = divf(toLight, r);                                        /* :96 */
+        const float vsEa[3] = { vsEf.x, vsEf.y, vsEf.z };
+        const basis LB = voxel_basis(s, vsEa, B->p, wi);              /* :103-106 */
+        const float area = (6.0f * s->vsize[0]) * s->vsize[1];        /* :113 */
+        const float jac = (r * r) / vt_absf(dot(neg(wi), LB.n));      /* :114 */
+        pdf = jac / area;                                             /* :115 */
+    } else {
+        L = sample_env(e, B, u[1], u[2], &wi, &pdf, cnt);             /* :120 */
+    }
+    pdf = pdf / (float)numLights;                                     /* :124 */
+    float vp[3];
+    int32_t steps, ground;
+    if (cnt) cnt->rays++;
+    const int missed = !traverse(s, B->p, wi, vp, &steps, &ground, cnt ? &cnt->visits : 0);   /* :133 */
+    if (sampling_emissive) {
+        if (missed || vp[0] != (float)vsE[0] || vp[1] != (float)vsE[1] || vp[2] != (float)vsE[2])
+            return mk(0, 0, 0);                                       /* :138-142 */
+    } else {
+        if (!missed) return mk(0, 0, 0);                              /* :148-152 */
+    }
+    const v3 lsWo = world_to_local(wo, B);                            /* :159 */
+    const v3 lsWi = world_to_local(wi, B);                            /* :160 */
+    const fpdf fp = eval_material(s, off, lsWo, lsWi);                /* :161 */
+    const float mis = power_heuristic(pdf, fp.pdf);                   /* :163 */
+    const float cosv = vt_absf(dot(wi, B->n));
+    /* :164 f * L * |cos| * mis / pdf, left to right */
+    return divf(mulf(mulf(mul(fp.f, L), cosv), mis), pdf);
+}
+
+/* ---------------------------------------------------------------------------
+ * main  (src/shaders/integrator/pathTracer.fs:172-296)
+ * ------------------------------------------------------------------------- */
+static v3 tonemap(v3 L, const orc_settings* st)   /* :294 */
+{
+    const float e = 1.0f / st->gamma;
+    return mk(vt_powf(L.x * st->exposure, e), vt_powf(L.y * st->exposure, e), vt_powf(L.z * st->exposure, e));
+}
+
+void orc_sample(const orc_scene* s, const orc_env* e, const orc_camera* cam, const orc_settings* st,
+                int32_t W, int32_t H, int32_t px, int32_t py, uint64_t seed, uint32_t frame,
+                uint32_t rng_frame, float rgb[3], orc_counters* cnt)
+{
+    (void)H;
+    (void)frame;
+    vt_pcg32 rng;
+    vt_rng_init(&rng, seed, (uint32_t)(py * W + px), rng_frame);     /* :174 */
+    v3 radiance = mk(0, 0, 0);
+    v3 o, d;
+    float u[4] = { 0, 0, 0, 0 };
+    if (cam->lens_model == 0 || cam->lens_model == 1) vt_rand4(&rng, u);  /* generateRay.h:15,79 */
+    gen_ray(cam, (float)px + 0.5f, (float)py + 0.5f, u, &o, &d);    /* :179 */
+    if (cnt) { cnt->samples++; cnt->rays++; }
+    const float t = aabb(o, d, ld(s->bmin), ld(s->bmax));            /* :183 */
+    v3 out;
+    if (t < 0.0f) {
+        out = tonemap(background(e, d, cnt), st);                     /* :187-194 */
+        rgb[0] = out.x; rgb[1] = out.y; rgb[2] = out.z;
+        return;
+    }
+    const v3 entry = add(o, mulf(d, t));                              /* :196 */
+    float vp[3];
+    int32_t steps, ground;
+    if (!traverse(s, entry, d, vp, &steps, &ground, cnt ? &cnt->visits : 0)) {   /* :202 */
+        out = tonemap(background(e, d, cnt), st);
+        rgb[0] = out.x; rgb[1] = out.y; rgb[2] = out.z;
+        return;
+    }
+    v3 throughput = mk(1, 1, 1);
+    int bounces = 0;
+    while (bounces < st->max_bounces) {                               /* :214 */
+        if (cnt) cnt->interactions++;
+        const basis B = voxel_basis(s, vp, o, d);                      /* :221 */
+        const int32_t ix = (int32_t)vp[0], iy = (int32_t)vp[1], iz = (int32_t)vp[2];
+        const int32_t off = fetch_offset(s, ix, iy, iz);              /* :225-226 */
+        if (ix == st->selected[0] && iy == st->selected[1] && iz == st->selected[2]) {
+            radiance = add(radiance, mk(1, 0, 0));                    /* :228-233 */
+            break;
+        }
+        const v3 wo = neg(d);                                         /* :237 */
+        const v3 lsWo = world_to_local(wo, &B);
+        if (bounces == 0) radiance = add(radiance, mul(throughput, emission(s, off)));   /* :241-245 */
+        radiance = add(radiance, mul(throughput, direct_lighting(s, e, off, &B, wo, &rng, cnt)));  /* :248 */
+        fpdf fp;
+        const v3 lsWi = sample_material(s, off, lsWo, &rng, &fp);     /* :255 */
+        if (st->wire_opacity > 0.0f) {                                /* :260-270 */
+            const v3 bmin = ld(s->bmin), bmax = ld(s->bmax);
+            const v3 res = mk((float)s->res[0], (float)s->res[1], (float)s->res[2]);
+            const v3 q = sub(B.p, bmin), ext = sub(bmax, bmin);
+            const v3 vc = mul(mk(q.x / ext.x, q.y / ext.y, q.z / ext.z), res);
+            const v3 uvw = sub(mk(vp[0], vp[1], vp[2]), vc);
+            const float ux = vt_absf(dot(mk(B.n.y, B.n.z, B.n.x), uvw));
+            const float uy = vt_absf(dot(mk(B.n.z, B.n.x, B.n.y), uvw));
+            const float th = st->wire_thickness;
+            float w = ((vt_stepf(th, ux) * vt_stepf(ux, 1.0f - th)) * vt_stepf(th, uy)) * vt_stepf(uy, 1.0f - th);
+            w = (1.0f - st->wire_opacity) + st->wire_opacity * w;
+            fp.f = mulf(fp.f, w);
+        }
+        const v3 wi = local_to_world(lsWi, &B);                        /* :273 */
+        const float cosv = vt_absf(dot(wi, B.n));
+        throughput = mul(throughput, divf(mulf(fp.f, cosv), fp.pdf)); /* :276 */
+        o = B.p;                                                      /* :278 */
+        d = wi;                                                       /* :279 */
+        if (cnt) cnt->rays++;
+        if (!traverse(s, o, d, vp, &steps, &ground, cnt ? &cnt->visits : 0)) {   /* :282 */
+            float lpdf;
+            const v3 L = eval_env(e, d, &lpdf, cnt);                  /* :285 */
+            const float mis = power_heuristic(fp.pdf, lpdf);          /* :286 */
+            radiance = add(radiance, mulf(mul(throughput, L), mis));  /* :287 */
+            break;
+        }
+        bounces++;
+    }
+    out = tonemap(radiance, st);
+    rgb[0] = out.x; rgb[1] = out.y; rgb[2] = out.z;
+}
+
+/* Progressive accumulation: accumulation.fs:10-18 per frame, in order. */
+void orc_render(const orc_scene* s, const orc_env* e, const orc_camera* cam, const orc_settings* st,
+                int32_t W, int32_t H, int32_t y0, int32_t y1, uint64_t seed, uint32_t n0, uint32_t count,
+                uint32_t rng_frame_max, float* accum, double* sum64, orc_counters* cnt, int32_t nthreads)
+{
+    const int64_t npix = (int64_t)(y1 - y0) * W;
+    uint64_t c_rays = 0, c_vis = 0, c_int = 0, c_smp = 0, c_es = 0, c_ee = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1) \
+    reduction(+ : c_rays, c_vis, c_int, c_smp, c_es, c_ee)
+#endif
+    for (int64_t i = 0; i < npix; ++i) {
+        const int32_t py = y0 + (int32_t)(i / W), px = (int32_t)(i % W);
+        const size_t p = (size_t)py * (size_t)W + (size_t)px;
+        orc_counters lc;
+        memset(&lc, 0, sizeof(lc));
+        float* a = accum + 4 * p;
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t n = n0 + k;
+            const uint32_t rf = n < rng_frame_max ? n : rng_frame_max;
+            float rgb[3];
+            orc_sample(s, e, cam, st, W, H, px, py, seed, n, rf, rgb, cnt ? &lc : 0);
+            const float fn = (float)n, fn1 = (float)(n + 1);
+            const float smp[4] = { rgb[0], rgb[1], rgb[2], 1.0f };
+            for (int c = 0; c < 4; ++c) a[c] = (smp[c] + a[c] * fn) / fn1;
+            if (sum64) for (int c = 0; c < 3; ++c) sum64[3 * p + c] += (double)rgb[c];
+        }
+        c_rays += lc.rays; c_vis += lc.visits; c_int += lc.interactions;
+        c_smp += lc.samples; c_es += lc.env_samples; c_ee += lc.env_evals;
+    }
+    if (cnt) {
+        cnt->rays += c_rays; cnt->visits += c_vis; cnt->interactions += c_int;
+        cnt->samples += c_smp; cnt->env_samples += c_es; cnt->env_evals += c_ee;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * first-hit probes
+ * ------------------------------------------------------------------------- */
+static void hit_record(const orc_scene* s, v3 o, v3 d, v3 basis_origin, int32_t* rec)
+{
+    float vp[3] = { 0, 0, 0 };
+    int32_t steps = 0, ground = 0;
+    memset(rec, 0, 8 * sizeof(int32_t));
+    const int hit = traverse(s, o, d, vp, &steps, &ground, 0);
+    rec[1] = (int32_t)vp[0]; rec[2] = (int32_t)vp[1]; rec[3] = (int32_t)vp[2];
+    rec[4] = steps;
+    if (!hit) { rec[0] = 1; return; }
+    rec[0] = ground ? 3 : 2;
+    const basis B = voxel_basis(s, vp, basis_origin, d);
+    rec[5] = (int32_t)B.n.x; rec[6] = (int32_t)B.n.y; rec[7] = (int32_t)B.n.z;
+}
+
+void orc_first_hit_image(const orc_scene* s, const orc_camera* cam, int32_t W, int32_t H,
+                         uint64_t seed, uint32_t frame, int32_t* rec)
+{
+    for (int32_t py = 0; py < H; ++py) {
+        for (int32_t px = 0; px < W; ++px) {
+            int32_t* r = rec + 8 * ((size_t)py * W + px);
+            vt_pcg32 rng;
+            vt_rng_init(&rng, seed, (uint32_t)(py * W + px), frame);
+            float u[4] = { 0, 0, 0, 0 };
+            if (cam->lens_model == 0 || cam->lens_model == 1) vt_rand4(&rng, u);
+            v3 o, d;
+            gen_ray(cam, (float)px + 0.5f, (float)py + 0.5f, u, &o, &d);
+            const float t = aabb(o, d, ld(s->bmin), ld(s->bmax));
+            if (t < 0.0f) { memset(r, 0, 8 * sizeof(int32_t)); continue; }
+            hit_record(s, add(o, mulf(d, t)), d, o, r);
+        }
+    }
+}
+
+void orc_trace_rays(const orc_scene* s, int32_t n, const float* o, const float* d, int32_t* rec)
+{
+    for (int32_t i = 0; i < n; ++i) hit_record(s, ld(o + 3 * i), ld(d + 3 * i), ld(o + 3 * i), rec + 8 * i);
+}
+
+/* ---------------------------------------------------------------------------
+ * voxelizer  (src/shaders/shared/voxelize.vs:20-28, voxelize.gs:55-251,
+ *             twin src/voxelize/cpuVoxelizer.cpp:55-291)
+ * ------------------------------------------------------------------------- */
+/* int(float) as x86 cvttss2si performs it: non-finite or out of range gives
+ * INT_MIN.  Sums wrap as 32-bit two's complement. */
+static inline int32_t cvt_i32(float f)
+{
+    if (!(f >= -2147483648.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)f;
+}
+static inline int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
+typedef struct { float x, y; } v2;
+static inline float dot2(v2 a, float bx, float by) { return a.x * bx + a.y * by; }
+
+void orc_voxelize(const float* verts, const uint32_t* idx, int64_t ntri, const float M[16],
+                  const int32_t res[3], int32_t mode, uint8_t value, uint8_t* occ)
+{
+    const float rf[3] = { (float)res[0], (float)res[1], (float)res[2] };
+    for (int64_t tri = 0; tri < ntri; ++tri) {
+        v3 v[3];
+        int finite = 1;
+        for (int k = 0; k < 3; ++k) {
+            const float* p = verts + 3 * (size_t)idx[3 * tri + k];
+            float ls[4];
+            mat4_mul(M, p[0], p[1], p[2], 1.0f, ls);                         /* voxelize.vs:23 */
+            v[k] = mk(ls[0] * rf[0], ls[1] * rf[1], ls[2] * rf[2]);         /* :25 */
+            if (!isfinite(v[k].x) || !isfinite(v[k].y) || !isfinite(v[k].z)) finite = 0;
+        }
+        if (!finite) continue;
+        /* swizzleTri gs:83-137 */
+        v3 n = cross(sub(v[1], v[0]), sub(v[2], v[1]));
+        const v3 an = mk(vt_absf(n.x), vt_absf(n.y), vt_absf(n.z));
+        int lut;
+        if (an.x >= an.y && an.x >= an.z) {
+            for (int k = 0; k < 3; ++k) v[k] = mk(v[k].y, v[k].z, v[k].x);
+            n = mk(n.y, n.z, n.x);
+            lut = 0;
+        } else if (an.y >= an.x && an.y >= an.z) {
+            for (int k = 0; k < 3; ++k) v[k] = mk(v[k].z, v[k].x, v[k].y);
+            n = mk(n.z, n.x, n.y);
+            lut = 1;
+        } else {
+            lut = 2;
+        }
+        /* main gs:272-276: AABB clamped against the unswizzled resolution */
+        const v3 amin = mk(vt_minf(vt_minf(v[0].x, v[1].x), v[2].x), vt_minf(vt_minf(v[0].y, v[1].y), v[2].y),
+                           vt_minf(vt_minf(v[0].z, v[1].z), v[2].z));
+        const v3 amax = mk(vt_maxf(vt_maxf(v[0].x, v[1].x), v[2].x), vt_maxf(vt_maxf(v[0].y, v[1].y), v[2].y),
+                           vt_maxf(vt_maxf(v[0].z, v[1].z), v[2].z));
+        int32_t mn[3], mx[3];
+        const float amn[3] = { amin.x, amin.y, amin.z }, amx[3] = { amax.x, amax.y, amax.z };
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = (int32_t)vt_minf(vt_maxf(floorf(amn[k]), 0.0f), rf[k]);
+            mx[k] = (int32_t)vt_minf(vt_maxf(ceilf(amx[k]), 0.0f), rf[k]);
+        }
+        /* voxelizeTriPostSwizzle gs:146-260 */
+        const v3 e0 = sub(v[1], v[0]), e1 = sub(v[2], v[1]), e2 = sub(v[0], v[2]);
+        const v3 ee[3] = { e0, e1, e2 };
+        v2 nxy[3], nyz[3], nzx[3];
+        float dxy[3], dyz[3], dzx[3];
+        for (int k = 0; k < 3; ++k) {
+            const v3 ek = ee[k];
+            nxy[k] = (n.z >= 0.0f) ? (v2){ -ek.y, ek.x } : (v2){ ek.y, -ek.x };
+            nyz[k] = (n.x >= 0.0f) ? (v2){ -ek.z, ek.y } : (v2){ ek.z, -ek.y };
+            nzx[k] = (n.y >= 0.0f) ? (v2){ -ek.x, ek.z } : (v2){ ek.x, -ek.z };
+            const v3 vk = v[k];
+            if (mode == 0) {
+                dxy[k] = dot2(nxy[k], 0.5f - vk.x, 0.5f - vk.y) + 0.5f * vt_maxf(vt_absf(nxy[k].x), vt_absf(nxy[k].y));
+                dyz[k] = dot2(nyz[k], 0.5f - vk.y, 0.5f - vk.z) + 0.5f * vt_maxf(vt_absf(nyz[k].x), vt_absf(nyz[k].y));
+                dzx[k] = dot2(nzx[k], 0.5f - vk.z, 0.5f - vk.x) + 0.5f * vt_maxf(vt_absf(nzx[k].x), vt_absf(nzx[k].y));
+            } else {
+                dxy[k] = (-dot2(nxy[k], vk.x, vk.y) + vt_maxf(0.0f, nxy[k].x)) + vt_maxf(0.0f, nxy[k].y);
+                dyz[k] = (-dot2(nyz[k], vk.y, vk.z) + vt_maxf(0.0f, nyz[k].x)) + vt_maxf(0.0f, nyz[k].y);
+                dzx[k] = (-dot2(nzx[k], vk.z, vk.x) + vt_maxf(0.0f, nzx[k].x)) + vt_maxf(0.0f, nzx[k].y);
+            }
+        }
+        const v3 np = (n.z < 0.0f) ? neg(n) : n;
+        const float dTri = dot(np, v[0]);
+        const float dMin = (mode == 0) ? dTri - (np.x * 0.5f + np.y * 0.5f)
+                                       : (dTri - vt_maxf(np.x, 0.0f)) - vt_maxf(np.y, 0.0f);
+        const float dMax = (mode == 0) ? dMin : (dTri - vt_minf(np.x, 0.0f)) - vt_minf(np.y, 0.0f);
+        const float nzInv = 1.0f / np.z;
+        for (int32_t x = mn[0]; x < mx[0]; ++x) {
+            for (int32_t y = mn[1]; y < mx[1]; ++y) {
+                const float fx = (float)x, fy = (float)y;
+                const float d0 = dxy[0] + dot2(nxy[0], fx, fy);
+                const float d1 = dxy[1] + dot2(nxy[1], fx, fy);
+                const float d2 = dxy[2] + dot2(nxy[2], fx, fy);
+                if (!(d0 >= 0.0f && d1 >= 0.0f && d2 >= 0.0f)) continue;
+                const float dnp = np.x * fx + np.y * fy;
+                const float zMinInt = (-dnp + dMin) * nzInv;
+                const float zMaxInt = (mode == 0) ? zMinInt : (-dnp + dMax) * nzInv;
+                const float zMinFloor = floorf(zMinInt), zMaxCeil = ceilf(zMaxInt);
+                int32_t zMin = wrap_add(cvt_i32(zMinFloor), -(int32_t)(zMinFloor == zMinInt));
+                int32_t zMax = wrap_add(cvt_i32(zMaxCeil), (int32_t)(zMaxCeil == zMaxInt));
+                if (zMin < mn[2]) zMin = mn[2];
+                if (zMax > mx[2]) zMax = mx[2];
+                for (int32_t z = zMin; z < zMax; ++z) {
+                    const float fz = (float)z;
+                    int ok = 1;
+                    for (int k = 0; k < 3 && ok; ++k) {
+                        if (!(dyz[k] + dot2(nyz[k], fy, fz) >= 0.0f)) ok = 0;
+                        if (!(dzx[k] + dot2(nzx[k], fz, fx) >= 0.0f)) ok = 0;
+                    }
+                    if (!ok) continue;
+                    int32_t c[3];
+                    if (lut == 0) { c[0] = z; c[1] = x; c[2] = y; }
+                    else if (lut == 1) { c[0] = y; c[1] = z; c[2] = x; }
+                    else { c[0] = x; c[1] = y; c[2] = z; }
+                    if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= res[0] || c[1] >= res[1] || c[2] >= res[2])
+                        continue;   /* imageStore outside the image is dropped */
+                    occ[(size_t)c[0] + (size_t)c[1] * res[0] + (size_t)c[2] * (size_t)res[0] * res[1]] = value;
+                }
+            }
+        }
+    }
+}
+
+/* computeMeshTransform  (src/renderer/import.cpp:46-64) with Imath's
+ * Box3f::majorAxis (largest extent, first on ties). */
+void orc_mesh_transform(const float bmin[3], const float bmax[3], const int32_t res[3], float M[16])
+{
+    const float size[3] = { bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2] };
+    const float margin[3] = { 1.0f / (float)res[0], 1.0f / (float)res[1], 1.0f / (float)res[2] };
+    int major = 0;
+    if (size[1] > size[major]) major = 1;
+    if (size[2] > size[major]) major = 2;
+    const float s = (float)((1.0 - 2.0 * (double)margin[major]) / (double)size[major]);
+    float t[3];
+    for (int k = 0; k < 3; ++k) t[k] = -bmin[k] + margin[k] / s;
+    memset(M, 0, 16 * sizeof(float));
+    M[0] = s; M[5] = s; M[10] = s; M[15] = 1.0f;
+    M[3] = t[0] * s; M[7] = t[1] * s; M[11] = t[2] * s;
+}
+
+/* ---------------------------------------------------------------------------
+ * environment CDF  (src/renderer/image.cpp:104-425)
+ * ------------------------------------------------------------------------- */
+void orc_calculate_cdf(const float* fimg, int32_t w, int32_t h, float* cdfu, float* cdfv, float* integral)
+{
+    const unsigned W = (unsigned)w, Hh = (unsigned)h;
+    const unsigned cw = W + 1;
+    float* fu = (float*)malloc(sizeof(float) * (size_t)W * Hh);
+    float* fv = (float*)malloc(sizeof(float) * (Hh + 1));
+    const float iW = (float)W, iH = (float)Hh, iA = iW * iH;
+    /* calculateImageIntegral :385-425 */
+    float sum = 0.0f;
+    for (unsigned y = 0; y < Hh; ++y) {
+        const float sinTheta = (float)sin(M_PI * (double)((float)y + 0.5f) / (double)iH);
+        for (unsigned x = 0; x < W; ++x) {
+            const float value = vt_maxf(0.0f, fimg[(size_t)y * W + x]);
+            fu[(size_t)y * W + x] = value * sinTheta;
+            sum += value * sinTheta;
+        }
+    }
+    float integ = sum / iA;
+    integ = (float)((double)integ * (2.0 * M_PI * M_PI));   /* *= 2.0f * M_PI * M_PI */
+    *integral = integ;
+    /* :248-283 */
+    const float numStepsW = (float)(unsigned)iW;
+    for (unsigned y = 0; y < Hh; ++y) {
+        const unsigned row = y * cw;
+        cdfu[row] = 0.0f;
+        for (unsigned x = 1; x <= W; ++x) {
+            const float f = fu[(size_t)y * W + x - 1] / numStepsW;
+            cdfu[row + x] = cdfu[row + x - 1] + f;
+        }
+        const float rowInt = cdfu[row + W];
+        fv[y] = rowInt;
+        if (rowInt > 0.0f) {
+            for (unsigned x = 1; x <= W; ++x) cdfu[row + x] /= rowInt;
+        } else {
+            for (unsigned x = 1; x <= W; ++x) cdfu[row + x] = (float)x / numStepsW;
+        }
+    }
+    /* :287-316 */
+    cdfv[0] = 0.0f;
+    const float numStepsH = (float)Hh;
+    for (unsigned y = 1; y <= Hh; ++y) {
+        const float f = fv[y - 1] / numStepsH;
+        cdfv[y] = cdfv[y - 1] + f;
+    }
+    const float imgInt = cdfv[Hh];
+    if (imgInt > 0.0f) {
+        for (unsigned y = 1; y <= Hh; ++y) cdfv[y] /= imgInt;
+    } else {
+        for (unsigned y = 1; y <= Hh; ++y) cdfv[y] = (float)y / iH;
+    }
+    free(fu);
+    free(fv);
+}
+
+/* Lanczos-3 kernel. */
+static double lanczos3(double x)
+{
+    x = fabs(x);
+    if (x >= 3.0) return 0.0;
+    if (x < 1e-9) return 1.0;
+    const double px = M_PI * x;
+    return 3.0 * sin(px) * sin(px / 3.0) / (px * px);
+}
+
+/* One separable resampling pass along rows (horizontal) of a 1-channel
+ * image: Lanczos-3 widened by the minification ratio, normalised weights,
+ * edge pixels clamped. */
+static void resample_rows(const float* src, int sw, int h, float* dst, int dw)
+{
+    const double ratio = (double)sw / (double)dw;
+    const double scale = ratio > 1.0 ? ratio : 1.0;
+    const double rad = 3.0 * scale;
+    for (int x = 0; x < dw; ++x) {
+        const double center = ((double)x + 0.5) * ratio;
+        const int i0 = (int)floor(center - rad), i1 = (int)ceil(center + rad);
+        double wsum = 0.0;
+        for (int i = i0; i <= i1; ++i) wsum += lanczos3(((double)i + 0.5 - center) / scale);
+        for (int y = 0; y < h; ++y) {
+            double acc = 0.0;
+            for (int i = i0; i <= i1; ++i) {
+                const int ci = i < 0 ? 0 : (i >= sw ? sw - 1 : i);
+                acc += lanczos3(((double)i + 0.5 - center) / scale) * (double)src[(size_t)y * sw + ci];
+            }
+            dst[(size_t)y * dw + x] = (float)(acc / wsum);
+        }
+    }
+}
+
+static void transpose_img(const float* src, int w, int h, float* dst)
+{
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) dst[(size_t)x * h + y] = src[(size_t)y * w + x];
+}
+
+/* generateImageFunction  (src/renderer/image.cpp:321-382).  The luminance
+ * is taken first (it commutes with the linear resize), then the image is
+ * resized when larger than max_size, then blurred by the normalised 3x3
+ * Gaussian exp(-2 (2x/3)^2) with black outside the image. */
+void orc_image_function(const float* rgb, int32_t w, int32_t h, int32_t max_size, float* out,
+                        int32_t* ow, int32_t* oh)
+{
+    float* lum = (float*)malloc(sizeof(float) * (size_t)w * h);
+    for (size_t i = 0; i < (size_t)w * h; ++i)
+        lum[i] = (rgb[3 * i] * 0.2126f + rgb[3 * i + 1] * 0.7152f) + rgb[3 * i + 2] * 0.0722f;
+    int nw = w, nh = h;
+    if ((w > h ? w : h) > max_size) {
+        const int mxd = w > h ? w : h;
+        nw = (int)((float)w / (float)mxd * (float)max_size);
+        nh = (int)((float)h / (float)mxd * (float)max_size);
+        float* t1 = (float*)malloc(sizeof(float) * (size_t)nw * h);
+        resample_rows(lum, w, h, t1, nw);
+        float* t2 = (float*)malloc(sizeof(float) * (size_t)nw * h);
+        transpose_img(t1, nw, h, t2);
+        float* t3 = (float*)malloc(sizeof(float) * (size_t)nw * nh);
+        resample_rows(t2, h, nw, t3, nh);
+        free(lum);
+        lum = (float*)malloc(sizeof(float) * (size_t)nw * nh);
+        transpose_img(t3, nh, nw, lum);
+        free(t1); free(t2); free(t3);
+    }
+    const float g1 = expf(-2.0f * ((2.0f / 3.0f) * (2.0f / 3.0f)));
+    const float k1[3] = { g1, 1.0f, g1 };
+    float ksum = 0.0f;
+    for (int j = 0; j < 3; ++j) for (int i = 0; i < 3; ++i) ksum += k1[i] * k1[j];
+    for (int y = 0; y < nh; ++y) {
+        for (int x = 0; x < nw; ++x) {
+            float acc = 0.0f;
+            for (int j = -1; j <= 1; ++j) {
+                for (int i = -1; i <= 1; ++i) {
+                    const int xx = x + i, yy = y + j;
+                    const float v = (xx < 0 || yy < 0 || xx >= nw || yy >= nh) ? 0.0f : lum[(size_t)yy * nw + xx];
+                    acc += ((k1[i + 1] * k1[j + 1]) / ksum) * v;
+                }
+            }
+            out[(size_t)y * nw + x] = acc;
+        }
+    }
+    free(lum);
+    *ow = nw;
+    *oh = nh;
+}
+
+/* pruneInteriorEmissiveVoxels  (src/renderer/import.cpp:180-250) */
+int32_t orc_prune_emissive(const int32_t* offsets, const int32_t res[3], int32_t* list, int32_t n)
+{
+    static const int nb[6][3] = { { 1, 0, 0 }, { -1, 0, 0 }, { 0, 1, 0 }, { 0, -1, 0 }, { 0, 0, 1 }, { 0, 0, -1 } };
+    for (int32_t i = 0; i < n; ++i) {
+        int32_t v = list[i];
+        const int32_t z = v / (res[0] * res[1]);
+        v -= z * res[0] * res[1];
+        const int32_t y = v / res[0];
+        const int32_t x = v - y * res[0];
+        int visible = 0;
+        for (int k = 0; k < 6 && !visible; ++k) {
+            const int32_t a = x + nb[k][0], b = y + nb[k][1], c = z + nb[k][2];
+            const int oob = a < 0 || a >= res[0] || b < 0 || b >= res[1] || c < 0 || c >= res[2];
+            if (oob) { visible = 1; break; }
+            if (offsets[(size_t)a + (size_t)b * res[0] + (size_t)c * res[0] * res[1]] < 0) visible = 1;
+        }
+        if (!visible) {
+            list[i] = list[n - 1];
+            n--;
+            i--;
+        }
+    }
+    return n;
+}
+
+/* ---------------------------------------------------------------------------
+ * updateCamera  (src/renderer/renderer.cpp:343-446,
+ *                src/camera/cameraParameters.cpp:43-50)
+ * ------------------------------------------------------------------------- */
+static void imath_normalize(float v[3])
+{
+    const float l = sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    if (l == 0.0f) { v[0] = v[1] = v[2] = 0.0f; return; }
+    v[0] = v[0] / l; v[1] = v[1] / l; v[2] = v[2] / l;
+}
+
+static void inverse4(const float* m, float* out)
+{
+    double a[16], inv[16];
+    for (int i = 0; i < 16; ++i) a[i] = (double)m[i];
+    inv[0] = a[5] * a[10] * a[15] - a[5] * a[11] * a[14] - a[9] * a[6] * a[15] + a[9] * a[7] * a[14] + a[13] * a[6] * a[11] - a[13] * a[7] * a[10];
+    inv[4] = -a[4] * a[10] * a[15] + a[4] * a[11] * a[14] + a[8] * a[6] * a[15] - a[8] * a[7] * a[14] - a[12] * a[6] * a[11] + a[12] * a[7] * a[10];
+    inv[8] = a[4] * a[9] * a[15] - a[4] * a[11] * a[13] - a[8] * a[5] * a[15] + a[8] * a[7] * a[13] + a[12] * a[5] * a[11] - a[12] * a[7] * a[9];
+    inv[12] = -a[4] * a[9] * a[14] + a[4] * a[10] * a[13] + a[8] * a[5] * a[14] - a[8] * a[6] * a[13] - a[12] * a[5] * a[10] + a[12] * a[6] * a[9];
+    inv[1] = -a[1] * a[10] * a[15] + a[1] * a[11] * a[14] + a[9] * a[2] * a[15] - a[9] * a[3] * a[14] - a[13] * a[2] * a[11] + a[13] * a[3] * a[10];
+    inv[5] = a[0] * a[10] * a[15] - a[0] * a[11] * a[14] - a[8] * a[2] * a[15] + a[8] * a[3] * a[14] + a[12] * a[2] * a[11] - a[12] * a[3] * a[10];
+    inv[9] = -a[0] * a[9] * a[15] + a[0] * a[11] * a[13] + a[8] * a[1] * a[15] - a[8] * a[3] * a[13] - a[12] * a[1] * a[11] + a[12] * a[3] * a[9];
+    inv[13] = a[0] * a[9] * a[14] - a[0] * a[10] * a[13] - a[8] * a[1] * a[14] + a[8] * a[2] * a[13] + a[12] * a[1] * a[10] - a[12] * a[2] * a[9];
+    inv[2] = a[1] * a[6] * a[15] - a[1] * a[7] * a[14] - a[5] * a[2] * a[15] + a[5] * a[3] * a[14] + a[13] * a[2] * a[7] - a[13] * a[3] * a[6];
+    inv[6] = -a[0] * a[6] * a[15] + a[0] * a[7] * a[14] + a[4] * a[2] * a[15] - a[4] * a[3] * a[14] - a[12] * a[2] * a[7] + a[12] * a[3] * a[6];
+    inv[10] = a[0] * a[5] * a[15] - a[0] * a[7] * a[13] - a[4] * a[1] * a[15] + a[4] * a[3] * a[13] + a[12] * a[1] * a[7] - a[12] * a[3] * a[5];
+    inv[14] = -a[0] * a[5] * a[14] + a[0] * a[6] * a[13] + a[4] * a[1] * a[14] - a[4] * a[2] * a[13] - a[12] * a[1] * a[6] + a[12] * a[2] * a[5];
+    inv[3] = -a[1] * a[6] * a[11] + a[1] * a[7] * a[10] + a[5] * a[2] * a[11] - a[5] * a[3] * a[10] - a[9] * a[2] * a[7] + a[9] * a[3] * a[6];
+    inv[7] = a[0] * a[6] * a[11] - a[0] * a[7] * a[10] - a[4] * a[2] * a[11] + a[4] * a[3] * a[10] + a[8] * a[2] * a[7] - a[8] * a[3] * a[6];
+    inv[11] = -a[0] * a[5] * a[11] + a[0] * a[7] * a[9] + a[4] * a[1] * a[11] - a[4] * a[3] * a[9] - a[8] * a[1] * a[7] + a[8] * a[3] * a[5];
+    inv[15] = a[0] * a[5] * a[10] - a[0] * a[6] * a[9] - a[4] * a[1] * a[10] + a[4] * a[2] * a[9] + a[8] * a[1] * a[6] - a[8] * a[2] * a[5];
+    const double det = a[0] * inv[0] + a[1] * inv[4] + a[2] * inv[8] + a[3] * inv[12];
+    for (int i = 0; i < 16; ++i) out[i] = (det == 0.0) ? 0.0f : (float)(inv[i] / det);
+}
+
+void orc_camera_matrices(const float eye[3], const float target[3], float fov_y, float near_d,
+                         float far_d, int32_t lens_model, int32_t W, int32_t H,
+                         float mv[16], float inv_mv[16], float proj[16], float inv_proj[16])
+{
+    float fwd[3] = { target[0] - eye[0], target[1] - eye[1], target[2] - eye[2] };
+    imath_normalize(fwd);
+    /* right = (0,1,0) x fwd */
+    float right[3] = { 1.0f * fwd[2] - 0.0f * fwd[1], 0.0f * fwd[0] - 0.0f * fwd[2], 0.0f * fwd[1] - 1.0f * fwd[0] };
+    imath_normalize(right);
+    const float up[3] = { fwd[1] * right[2] - fwd[2] * right[1], fwd[2] * right[0] - fwd[0] * right[2],
+                          fwd[0] * right[1] - fwd[1] * right[0] };
+    const float er = (eye[0] * right[0] + eye[1] * right[1]) + eye[2] * right[2];
+    const float eu = (eye[0] * up[0] + eye[1] * up[1]) + eye[2] * up[2];
+    const float ef = (eye[0] * fwd[0] + eye[1] * fwd[1]) + eye[2] * fwd[2];
+    const float m[16] = { right[0], right[1], right[2], -er,
+                          up[0], up[1], up[2], -eu,
+                          -fwd[0], -fwd[1], -fwd[2], ef,
+                          0.0f, 0.0f, 0.0f, 1.0f };
+    memcpy(mv, m, sizeof(m));
+    float p[16];
+    memset(p, 0, sizeof(p));
+    const float a = (float)W / (float)H;
+    if (lens_model == 2) {
+        const float dist = sqrtf(((target[0] - eye[0]) * (target[0] - eye[0]) + (target[1] - eye[1]) * (target[1] - eye[1])) +
+                                 (target[2] - eye[2]) * (target[2] - eye[2]));
+        const float left = -tanf(fov_y / 2.0f) * dist;
+        const float rightp = -left;
+        const float bottom = -tanf(fov_y / 2.0f) * dist / a;
+        const float top = -bottom;
+        const float n = -near_d, f = -far_d;
+        const float tx = -(rightp + left) / (rightp - left);
+        const float ty = -(top + bottom) / (top - bottom);
+        const float tz = -(f + n) / (f - n);
+        p[0] = 2.0f / (rightp - left); p[3] = tx;
+        p[5] = 2.0f / (top - bottom); p[7] = ty;
+        p[10] = -2.0f / (f - n); p[11] = tz;
+        p[15] = 1.0f;
+    } else {
+        const float n = near_d, f = far_d;
+        const float e = 1.0f / tanf(fov_y / 2.0f);
+        p[0] = e / a;
+        p[5] = e;
+        p[10] = (f + n) / (n - f);
+        p[11] = ((2.0f * f) * n) / (n - f);
+        p[14] = -1.0f;
+    }
+    memcpy(proj, p, sizeof(p));
+    inverse4(mv, inv_mv);
+    inverse4(p, inv_proj);
+}

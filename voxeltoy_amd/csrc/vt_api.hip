@@ -1,0 +1,751 @@
+/*
+ * vt_api.hip - host side of the C ABI declared in include/vt_api.h.
+ *
+ * A context owns one HIP stream (default priority) on one device and all
+ * device buffers: the Morton-bricked grid and its occupancy bits, 256
+ * decoded material records, the emissive list, the environment image and
+ * CDF tables, and the RGBA32F accumulation target (plus optional FP64
+ * sums).  Host arrays are copied on upload.
+ */
+#include "../../include/vt_api.h"
+#include "vt_kernels.h"
+#include "../host/vt_host_algos.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define VT_HIP(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(VT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct vt_context {
+    int device = 0;
+    int cus = 0;
+    size_t lds_limit = 65536;       /* dynamic LDS per workgroup */
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+
+    /* volume */
+    bool has_volume = false;
+    vt::BrickDims dims{};
+    float bmin[3] = { 0, 0, 0 }, bmax[3] = { 0, 0, 0 }, vs[3] = { 0, 0, 0 };
+    uint8_t* d_bricks = nullptr;
+    uint32_t* d_bits = nullptr;
+    uint32_t bit_words = 0;
+    int32_t* d_emissive = nullptr;
+    int32_t n_emissive = 0;
+    std::vector<float> mat;
+    int32_t slot_off[256];
+    int32_t n_slots = 0;            /* slots 1..n_slots used */
+    vt::MatRec* d_recs = nullptr;
+
+    /* environment */
+    vt_environment env{};
+    float* d_img = nullptr;
+    float* d_cdfu = nullptr;
+    float* d_cdfv = nullptr;
+
+    vt_camera cam{};
+    vt_settings st{};
+
+    /* frame */
+    int32_t W = 0, H = 0;
+    float4* d_accum = nullptr;
+    double* d_sum64 = nullptr;
+
+    uint32_t* d_tile = nullptr;
+    unsigned long long* d_stats = nullptr;
+    bool stats_enabled = false;
+    bool stats_valid = false;
+
+    vt_context() { for (int i = 0; i < 256; ++i) slot_off[i] = -1; }
+};
+
+namespace {
+
+float mat_at(const std::vector<float>& m, int64_t i) { return (i < 0 || i >= (int64_t)m.size()) ? 0.0f : m[(size_t)i]; }
+
+/* Decode the record a material offset points at (materials.h:7-56): type
+ * at off, emission at off+1..3, albedo/reflectance at off+4..6, roughness
+ * at off+7; reads past the array return 0 as texelFetch does. */
+vt::MatRec decode(const std::vector<float>& m, int32_t off)
+{
+    vt::MatRec r;
+    const float t = mat_at(m, off);
+    r.type = (t >= 0.0f && t < 3.0f) ? (int32_t)t : -1;
+    for (int k = 0; k < 3; ++k) {
+        r.e[k] = r.type < 0 ? 0.0f : mat_at(m, (int64_t)off + 1 + k);
+        r.r[k] = mat_at(m, (int64_t)off + 4 + k);
+    }
+    r.rough = mat_at(m, (int64_t)off + 7);
+    return r;
+}
+
+int upload_records(vt_context* c)
+{
+    vt::MatRec recs[256];
+    recs[0] = decode(c->mat, 0);
+    for (int s = 1; s < 256; ++s) recs[s] = decode(c->mat, c->slot_off[s] < 0 ? 0 : c->slot_off[s]);
+    if (!c->d_recs) VT_HIP(hipMalloc(&c->d_recs, sizeof(recs)));
+    VT_HIP(hipMemcpyAsync(c->d_recs, recs, sizeof(recs), hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
+vt::BrickDims make_dims(int32_t nx, int32_t ny, int32_t nz)
+{
+    vt::BrickDims d;
+    d.nx = nx; d.ny = ny; d.nz = nz;
+    d.bx = (nx + 7) / 8; d.by = (ny + 7) / 8; d.bz = (nz + 7) / 8;
+    int32_t m = std::max(d.bx, std::max(d.by, d.bz));
+    int32_t p = 1;
+    while (p < m) p <<= 1;
+    d.bdim = p;
+    d.n_bricks = (uint64_t)p * p * p;
+    return d;
+}
+
+/* Reference bounds (renderer.cpp:845-850): cubic voxels, longest axis 1000. */
+void reference_bounds(int32_t nx, int32_t ny, int32_t nz, float* bmin, float* bmax)
+{
+    const float sizeMultiplier = 1000.0f;
+    const float voxelSize = sizeMultiplier / (float)std::max(nx, std::max(ny, nz));
+    const float bs[3] = { voxelSize * (float)nx, voxelSize * (float)ny, voxelSize * (float)nz };
+    for (int k = 0; k < 3; ++k) {
+        bmin[k] = -bs[k] * 0.5f;
+        bmax[k] = bs[k] * 0.5f;
+    }
+}
+
+int common_volume_setup(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const float* mat_data, int32_t mat_size,
+                        const float* bmin, const float* bmax)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > 8192 || ny > 8192 || nz > 8192)
+        return fail(VT_ERR_ARG, "volume resolution must be in [1, 8192] per axis");
+    if (mat_size < 0 || (mat_size > 0 && !mat_data)) return fail(VT_ERR_ARG, "bad material data");
+    const vt::BrickDims d = make_dims(nx, ny, nz);
+    if (d.bdim > 1024) return fail(VT_ERR_LIMIT, "grid too large for 10-bit brick Morton codes");
+    c->has_volume = false;
+    dfree(c->d_bricks);
+    dfree(c->d_bits);
+    dfree(c->d_emissive);
+    c->dims = d;
+    if (bmin && bmax) {
+        for (int k = 0; k < 3; ++k) { c->bmin[k] = bmin[k]; c->bmax[k] = bmax[k]; }
+    } else {
+        reference_bounds(nx, ny, nz, c->bmin, c->bmax);
+    }
+    const int32_t res[3] = { nx, ny, nz };
+    for (int k = 0; k < 3; ++k) c->vs[k] = (c->bmax[k] - c->bmin[k]) / (float)res[k];
+    c->mat.assign(mat_data, mat_data + mat_size);
+    for (int i = 0; i < 256; ++i) c->slot_off[i] = -1;
+    c->n_slots = 0;
+    VT_HIP(hipMalloc(&c->d_bricks, d.n_bricks * 512u));
+    c->bit_words = (uint32_t)((d.n_bricks + 31) / 32);
+    VT_HIP(hipMalloc(&c->d_bits, (size_t)c->bit_words * 4u));
+    VT_HIP(vt::launch_fill_u8(c->d_bricks, d.n_bricks * 512u, 0, c->stream));
+    return VT_OK;
+}
+
+int finish_volume(vt_context* c, const int32_t* emissive, int32_t n_emissive, const std::vector<uint8_t>& occupied_of_linear,
+                  bool have_occ)
+{
+    (void)occupied_of_linear;
+    (void)have_occ;
+    c->n_emissive = n_emissive;
+    if (n_emissive > 0) {
+        VT_HIP(hipMalloc(&c->d_emissive, (size_t)n_emissive * 4u));
+        VT_HIP(hipMemcpyAsync(c->d_emissive, emissive, (size_t)n_emissive * 4u, hipMemcpyHostToDevice, c->stream));
+    }
+    VT_HIP(vt::launch_fill_u32(c->d_bits, c->bit_words, 0u, c->stream));
+    VT_HIP(vt::launch_brick_bits(c->dims, c->d_bricks, c->d_bits, c->stream));
+    int r = upload_records(c);
+    if (r != VT_OK) return r;
+    VT_HIP(hipStreamSynchronize(c->stream));
+    c->has_volume = true;
+    return VT_OK;
+}
+
+int check_emissive(const int32_t* emissive, int32_t n, int64_t nvox, const std::vector<uint8_t>& occ)
+{
+    if (n < 0 || (n > 0 && !emissive)) return fail(VT_ERR_ARG, "bad emissive list");
+    for (int32_t i = 0; i < n; ++i) {
+        if (emissive[i] < 0 || (int64_t)emissive[i] >= nvox)
+            return fail(VT_ERR_ARG, "emissive voxel index out of range at " + std::to_string(i));
+        if (!occ[(size_t)emissive[i]])
+            return fail(VT_ERR_ARG, "emissive voxel " + std::to_string(emissive[i]) + " is empty");
+    }
+    return VT_OK;
+}
+
+vt::GridView grid_view(const vt_context* c)
+{
+    vt::GridView g;
+    g.nx = c->dims.nx; g.ny = c->dims.ny; g.nz = c->dims.nz;
+    const int32_t res[3] = { g.nx, g.ny, g.nz };
+    {
+        /* int(2 * ceil(length(vec3(voxelResolution))))  dda.h:98 */
+        const float rx = (float)res[0], ry = (float)res[1], rz = (float)res[2];
+        g.max_steps = (int32_t)(2.0f * std::ceil(std::sqrt((rx * rx + ry * ry) + rz * rz)));
+    }
+    g.rx = (float)res[0]; g.ry = (float)res[1]; g.rz = (float)res[2];
+    for (int k = 0; k < 3; ++k) {
+        g.bmin[k] = c->bmin[k];
+        g.bmax[k] = c->bmax[k];
+        g.vs[k] = c->vs[k];
+        g.ext[k] = 1.0f / (c->bmax[k] - c->bmin[k]);
+    }
+    g.bricks = c->d_bricks;
+    g.bits = c->d_bits;
+    g.bit_words = c->bit_words;
+    g.n_emissive = c->n_emissive;
+    g.emissive = c->d_emissive;
+    return g;
+}
+
+vt::TraceArgs trace_args(const vt_context* c)
+{
+    vt::TraceArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = grid_view(c);
+    a.e.use_image = c->env.use_image;
+    for (int k = 0; k < 3; ++k) { a.e.top[k] = c->env.top[k]; a.e.bottom[k] = c->env.bottom[k]; }
+    a.e.w = c->env.image_w; a.e.h = c->env.image_h; a.e.img = c->d_img;
+    a.e.cu_w = c->env.cdf_u_w; a.e.cu_h = c->env.cdf_u_h; a.e.cdfu = c->d_cdfu;
+    a.e.cv_n = c->env.cdf_v_n; a.e.cdfv = c->d_cdfv;
+    a.e.integral = c->env.integral; a.e.rotation = c->env.rotation_radians;
+    std::memcpy(a.c.inv_mv, c->cam.inv_modelview, sizeof(a.c.inv_mv));
+    std::memcpy(a.c.inv_proj, c->cam.inv_proj, sizeof(a.c.inv_proj));
+    a.c.P32 = c->cam.proj[2 * 4 + 3];
+    a.c.P22 = c->cam.proj[2 * 4 + 2];
+    a.c.P23 = c->cam.proj[3 * 4 + 2];
+    std::memcpy(a.c.viewport, c->cam.viewport, sizeof(a.c.viewport));
+    a.c.win_z = c->cam.win_z;
+    a.c.lens_radius = c->cam.lens_radius;
+    a.c.focal_distance = c->cam.focal_distance;
+    a.c.lens_model = c->cam.lens_model;
+    a.s.max_bounces = c->st.max_bounces;
+    a.s.exposure = c->st.exposure;
+    a.s.inv_gamma = 1.0f / c->st.gamma;
+    for (int k = 0; k < 3; ++k) a.s.sel[k] = c->st.selected_voxel[k];
+    a.s.wire_opacity = c->st.wireframe_opacity;
+    a.s.wire_thickness = c->st.wireframe_thickness;
+    a.recs = c->d_recs;
+    a.W = c->W; a.H = c->H;
+    a.accum = c->d_accum;
+    a.sum64 = c->d_sum64;
+    a.tile_counter = c->d_tile;
+    a.tiles_x = (c->W + 7) / 8;
+    a.n_tiles = a.tiles_x * ((c->H + 7) / 8);
+    a.cdfv_lds_bytes = c->env.use_image ? ((c->env.cdf_v_n * 4 + 15) / 16) * 16 : 0;
+    const size_t base = 256 * sizeof(vt::MatRec) + (size_t)a.cdfv_lds_bytes;
+    a.lds_bits = (base + (size_t)c->bit_words * 4u <= c->lds_limit) ? 1 : 0;
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vt_last_error(void) { return g_err.c_str(); }
+
+int vt_create(int device, vt_context** out)
+{
+    if (!out) return fail(VT_ERR_ARG, "null output pointer");
+    int n = 0;
+    VT_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(VT_ERR_ARG, "no such device");
+    VT_HIP(hipSetDevice(device));
+    vt_context* c = new vt_context();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return fail(VT_ERR_HIP, "device query failed"); }
+    c->cus = prop.multiProcessorCount;
+    c->lds_limit = prop.sharedMemPerBlock;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipMalloc(&c->d_tile, 4) != hipSuccess || hipMalloc(&c->d_stats, 6 * sizeof(unsigned long long)) != hipSuccess) {
+        vt_destroy(c);
+        return fail(VT_ERR_HIP, "context resource creation failed");
+    }
+    /* defaults: pathTracer.fs uniform defaults and Renderer ctor (renderer.cpp:41-65) */
+    c->env.use_image = 0;
+    const float top[3] = { 153.0f / 255 * 2, 187.0f / 255 * 2, 201.0f / 255 * 2 };
+    const float bot[3] = { 77.0f / 255, 64.0f / 255, 50.0f / 255 };
+    for (int k = 0; k < 3; ++k) { c->env.top[k] = top[k]; c->env.bottom[k] = bot[k]; }
+    c->st.max_bounces = 1;
+    c->st.exposure = 1.0f;
+    c->st.gamma = 2.2f;
+    c->st.selected_voxel[0] = c->st.selected_voxel[1] = c->st.selected_voxel[2] = INT32_MIN;
+    c->st.wireframe_opacity = 0.0f;
+    c->st.wireframe_thickness = 0.01f;
+    *out = c;
+    return VT_OK;
+}
+
+void vt_destroy(vt_context* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->d_bricks); dfree(c->d_bits); dfree(c->d_emissive); dfree(c->d_recs);
+    dfree(c->d_img); dfree(c->d_cdfu); dfree(c->d_cdfv);
+    dfree(c->d_accum); dfree(c->d_sum64); dfree(c->d_tile); dfree(c->d_stats);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int vt_synchronize(vt_context* c)
+{
+    if (!c) return fail(VT_ERR_ARG, "null context");
+    VT_HIP(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
+int vt_device_cus(vt_context* c) { return c ? c->cus : 0; }
+
+int vt_set_volume(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const int32_t* offsets, const float* mat_data,
+                  int32_t mat_size, const int32_t* emissive, int32_t n_emissive, const float* bmin, const float* bmax)
+{
+    if (!c || !offsets) return fail(VT_ERR_ARG, "null argument");
+    VT_HIP(hipSetDevice(c->device));
+    int r = common_volume_setup(c, nx, ny, nz, mat_data, mat_size, bmin, bmax);
+    if (r != VT_OK) return r;
+    const int64_t nvox = (int64_t)nx * ny * nz;
+    /* distinct occupied offsets -> slots, in increasing offset order */
+    std::vector<uint8_t> used((size_t)std::max(mat_size, 1), 0);
+    std::vector<uint8_t> occ((size_t)nvox, 0);
+    for (int64_t i = 0; i < nvox; ++i) {
+        const int32_t o = offsets[i];
+        if (o < 0) continue;
+        if (o >= mat_size) return fail(VT_ERR_ARG, "material offset " + std::to_string(o) + " outside material data");
+        used[(size_t)o] = 1;
+        occ[(size_t)i] = 1;
+    }
+    std::vector<uint8_t> lut((size_t)std::max(mat_size, 1), 0);
+    int32_t ns = 0;
+    for (int32_t o = 0; o < mat_size; ++o) {
+        if (!used[(size_t)o]) continue;
+        if (ns == 255) return fail(VT_ERR_LIMIT, "more than 255 distinct material offsets in the grid");
+        ++ns;
+        lut[(size_t)o] = (uint8_t)ns;
+        c->slot_off[ns] = o;
+    }
+    c->n_slots = ns;
+    r = check_emissive(emissive, n_emissive, nvox, occ);
+    if (r != VT_OK) return r;
+    int32_t* d_dense = nullptr;
+    uint8_t* d_lut = nullptr;
+    uint32_t* d_bad = nullptr;
+    VT_HIP(hipMalloc(&d_dense, (size_t)nvox * 4u));
+    VT_HIP(hipMalloc(&d_lut, lut.size()));
+    VT_HIP(hipMalloc(&d_bad, 4));
+    VT_HIP(hipMemcpyAsync(d_dense, offsets, (size_t)nvox * 4u, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipMemcpyAsync(d_lut, lut.data(), lut.size(), hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_fill_u32(d_bad, 1, 0u, c->stream));
+    VT_HIP(vt::launch_brick_from_offsets(c->dims, d_dense, d_lut, mat_size, c->d_bricks, d_bad, c->stream));
+    uint32_t bad = 0;
+    VT_HIP(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    dfree(d_dense); dfree(d_lut); dfree(d_bad);
+    if (bad) return fail(VT_ERR_ARG, "grid conversion found an unmapped material offset");
+    return finish_volume(c, emissive, n_emissive, occ, true);
+}
+
+int vt_set_volume_u8(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const uint8_t* ids, const int32_t* id_offset,
+                     const float* mat_data, int32_t mat_size, const int32_t* emissive, int32_t n_emissive,
+                     const float* bmin, const float* bmax)
+{
+    if (!c || !ids || !id_offset) return fail(VT_ERR_ARG, "null argument");
+    VT_HIP(hipSetDevice(c->device));
+    int r = common_volume_setup(c, nx, ny, nz, mat_data, mat_size, bmin, bmax);
+    if (r != VT_OK) return r;
+    uint8_t remap[256];
+    int32_t ns = 0;
+    for (int id = 0; id < 256; ++id) {
+        remap[id] = 0;
+        if (id_offset[id] < 0) continue;
+        if (id_offset[id] >= mat_size) return fail(VT_ERR_ARG, "id " + std::to_string(id) + " offset outside material data");
+        if (ns == 255) return fail(VT_ERR_LIMIT, "more than 255 occupied ids");
+        ++ns;
+        remap[id] = (uint8_t)ns;
+        c->slot_off[ns] = id_offset[id];
+    }
+    c->n_slots = ns;
+    const int64_t nvox = (int64_t)nx * ny * nz;
+    std::vector<uint8_t> occ;
+    if (n_emissive > 0) {
+        occ.assign((size_t)nvox, 0);
+        for (int64_t i = 0; i < nvox; ++i) occ[(size_t)i] = remap[ids[i]] != 0;
+    }
+    r = check_emissive(emissive, n_emissive, nvox, occ);
+    if (r != VT_OK) return r;
+    uint8_t* d_dense = nullptr;
+    uint8_t* d_remap = nullptr;
+    VT_HIP(hipMalloc(&d_dense, (size_t)nvox));
+    VT_HIP(hipMalloc(&d_remap, 256));
+    VT_HIP(hipMemcpyAsync(d_dense, ids, (size_t)nvox, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipMemcpyAsync(d_remap, remap, 256, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_brick_from_ids(c->dims, d_dense, d_remap, c->d_bricks, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    dfree(d_dense); dfree(d_remap);
+    return finish_volume(c, emissive, n_emissive, occ, false);
+}
+
+int vt_get_bounds(vt_context* c, float* bmin, float* bmax, float* vsz)
+{
+    if (!c || !c->has_volume) return fail(VT_ERR_STATE, "no volume");
+    for (int k = 0; k < 3; ++k) {
+        if (bmin) bmin[k] = c->bmin[k];
+        if (bmax) bmax[k] = c->bmax[k];
+        if (vsz) vsz[k] = c->vs[k];
+    }
+    return VT_OK;
+}
+
+int vt_read_volume(vt_context* c, int32_t* offsets)
+{
+    if (!c || !offsets) return fail(VT_ERR_ARG, "null argument");
+    if (!c->has_volume) return fail(VT_ERR_STATE, "no volume");
+    VT_HIP(hipSetDevice(c->device));
+    const int64_t nvox = (int64_t)c->dims.nx * c->dims.ny * c->dims.nz;
+    int32_t table[256];
+    table[0] = -1;
+    for (int s = 1; s < 256; ++s) table[s] = c->slot_off[s];
+    int32_t* d_dense = nullptr;
+    int32_t* d_table = nullptr;
+    VT_HIP(hipMalloc(&d_dense, (size_t)nvox * 4u));
+    VT_HIP(hipMalloc(&d_table, sizeof(table)));
+    VT_HIP(hipMemcpyAsync(d_table, table, sizeof(table), hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_dense_from_bricks(c->dims, c->d_bricks, d_table, d_dense, c->stream));
+    VT_HIP(hipMemcpyAsync(offsets, d_dense, (size_t)nvox * 4u, hipMemcpyDeviceToHost, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    dfree(d_dense); dfree(d_table);
+    return VT_OK;
+}
+
+int vt_update_material(vt_context* c, int32_t offset, const float* values, int32_t count)
+{
+    if (!c || !values || count < 0) return fail(VT_ERR_ARG, "bad argument");
+    if (offset < 0 || (int64_t)offset + count > (int64_t)c->mat.size())
+        return fail(VT_ERR_ARG, "material update outside the material data");
+    for (int32_t i = 0; i < count; ++i) c->mat[(size_t)offset + i] = values[i];
+    VT_HIP(hipSetDevice(c->device));
+    return upload_records(c);
+}
+
+int vt_read_materials(vt_context* c, float* mat_data, int32_t capacity, int32_t* size)
+{
+    if (!c) return fail(VT_ERR_ARG, "null context");
+    if (size) *size = (int32_t)c->mat.size();
+    if (mat_data) std::memcpy(mat_data, c->mat.data(), sizeof(float) * std::min<size_t>(c->mat.size(), (size_t)std::max(capacity, 0)));
+    return VT_OK;
+}
+
+int vt_set_camera(vt_context* c, const vt_camera* cam)
+{
+    if (!c || !cam) return fail(VT_ERR_ARG, "null argument");
+    if (cam->lens_model < 0 || cam->lens_model > 2) return fail(VT_ERR_ARG, "lens_model must be 0, 1 or 2");
+    c->cam = *cam;
+    return VT_OK;
+}
+
+int vt_set_environment(vt_context* c, const vt_environment* env)
+{
+    if (!c || !env) return fail(VT_ERR_ARG, "null argument");
+    VT_HIP(hipSetDevice(c->device));
+    vt_environment e = *env;
+    dfree(c->d_img); dfree(c->d_cdfu); dfree(c->d_cdfv);
+    if (e.use_image) {
+        if (e.image_w <= 0 || e.image_h <= 0 || !e.image_rgb || e.cdf_u_w < 2 || e.cdf_u_h < 1 || !e.cdf_u ||
+            e.cdf_v_n < 2 || !e.cdf_v)
+            return fail(VT_ERR_ARG, "image environment needs an image and its CDF tables");
+        if (e.cdf_u_h != e.cdf_v_n - 1) return fail(VT_ERR_ARG, "CDF U height must equal CDF V size - 1");
+        if (e.cdf_v_n > 8193) return fail(VT_ERR_LIMIT, "CDF V larger than 8193 entries");
+        const size_t ni = (size_t)e.image_w * e.image_h * 3, nu = (size_t)e.cdf_u_w * e.cdf_u_h;
+        VT_HIP(hipMalloc(&c->d_img, ni * 4));
+        VT_HIP(hipMalloc(&c->d_cdfu, nu * 4));
+        VT_HIP(hipMalloc(&c->d_cdfv, (size_t)e.cdf_v_n * 4));
+        VT_HIP(hipMemcpyAsync(c->d_img, e.image_rgb, ni * 4, hipMemcpyHostToDevice, c->stream));
+        VT_HIP(hipMemcpyAsync(c->d_cdfu, e.cdf_u, nu * 4, hipMemcpyHostToDevice, c->stream));
+        VT_HIP(hipMemcpyAsync(c->d_cdfv, e.cdf_v, (size_t)e.cdf_v_n * 4, hipMemcpyHostToDevice, c->stream));
+        VT_HIP(hipStreamSynchronize(c->stream));
+    }
+    e.image_rgb = nullptr; e.cdf_u = nullptr; e.cdf_v = nullptr;
+    c->env = e;
+    return VT_OK;
+}
+
+int vt_set_settings(vt_context* c, const vt_settings* st)
+{
+    if (!c || !st) return fail(VT_ERR_ARG, "null argument");
+    if (st->max_bounces < 0) return fail(VT_ERR_ARG, "max_bounces must be >= 0");
+    c->st = *st;
+    return VT_OK;
+}
+
+int vt_set_frame(vt_context* c, int32_t w, int32_t h, int32_t fp64)
+{
+    if (!c) return fail(VT_ERR_ARG, "null context");
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 28)) return fail(VT_ERR_ARG, "bad frame size");
+    VT_HIP(hipSetDevice(c->device));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    dfree(c->d_accum); dfree(c->d_sum64);
+    c->W = w; c->H = h;
+    VT_HIP(hipMalloc(&c->d_accum, (size_t)w * h * sizeof(float4)));
+    if (fp64) VT_HIP(hipMalloc(&c->d_sum64, (size_t)w * h * 3 * sizeof(double)));
+    return vt_reset_accum(c);
+}
+
+int vt_reset_accum(vt_context* c)
+{
+    if (!c || !c->d_accum) return fail(VT_ERR_STATE, "no frame");
+    VT_HIP(hipSetDevice(c->device));
+    VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_accum, (uint64_t)c->W * c->H * 4, 0u, c->stream));
+    if (c->d_sum64) VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_sum64, (uint64_t)c->W * c->H * 6, 0u, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
+int vt_set_stats(vt_context* c, int32_t enabled)
+{
+    if (!c) return fail(VT_ERR_ARG, "null context");
+    c->stats_enabled = enabled != 0;
+    return VT_OK;
+}
+
+int vt_render(vt_context* c, uint64_t seed, uint32_t n0, uint32_t count, uint32_t rng_frame_max)
+{
+    if (!c) return fail(VT_ERR_ARG, "null context");
+    if (!c->has_volume) return fail(VT_ERR_STATE, "no volume");
+    if (!c->d_accum) return fail(VT_ERR_STATE, "no frame");
+    if (count == 0) return VT_OK;
+    VT_HIP(hipSetDevice(c->device));
+    vt::TraceArgs a = trace_args(c);
+    a.seed = seed; a.n0 = n0; a.count = count; a.rng_frame_max = rng_frame_max;
+    a.stats = c->stats_enabled ? c->d_stats : nullptr;
+    const int per_cu = vt::render_blocks_per_cu(a, c->stats_enabled);
+    const int grid = std::max(1, c->cus * per_cu);
+    VT_HIP(vt::launch_reset_counter(c->d_tile, a.stats, c->stream));
+    VT_HIP(hipEventRecord(c->ev0, c->stream));
+    VT_HIP(vt::launch_render(a, grid, c->stream, c->stats_enabled));
+    VT_HIP(hipEventRecord(c->ev1, c->stream));
+    c->timed = true;
+    c->stats_valid = c->stats_enabled;
+    return VT_OK;
+}
+
+int vt_read_accum(vt_context* c, float* rgba)
+{
+    if (!c || !rgba) return fail(VT_ERR_ARG, "null argument");
+    if (!c->d_accum) return fail(VT_ERR_STATE, "no frame");
+    VT_HIP(hipSetDevice(c->device));
+    VT_HIP(hipMemcpyAsync(rgba, c->d_accum, (size_t)c->W * c->H * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
+int vt_read_sum64(vt_context* c, double* rgb)
+{
+    if (!c || !rgb) return fail(VT_ERR_ARG, "null argument");
+    if (!c->d_sum64) return fail(VT_ERR_STATE, "frame has no FP64 sums");
+    VT_HIP(hipSetDevice(c->device));
+    VT_HIP(hipMemcpyAsync(rgb, c->d_sum64, (size_t)c->W * c->H * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
+int vt_get_stats(vt_context* c, vt_stats* out)
+{
+    if (!c || !out) return fail(VT_ERR_ARG, "null argument");
+    VT_HIP(hipSetDevice(c->device));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    std::memset(out, 0, sizeof(*out));
+    if (c->stats_valid) {
+        unsigned long long s[6];
+        VT_HIP(hipMemcpy(s, c->d_stats, sizeof(s), hipMemcpyDeviceToHost));
+        out->rays = s[0]; out->visits = s[1]; out->interactions = s[2];
+        out->samples = s[3]; out->env_samples = s[4]; out->env_evals = s[5];
+    }
+    if (c->timed) {
+        float ms = 0.0f;
+        VT_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        out->kernel_ms = ms;
+    }
+    return VT_OK;
+}
+
+int vt_trace_first_hit(vt_context* c, uint64_t seed, uint32_t frame, int32_t* rec)
+{
+    if (!c || !rec) return fail(VT_ERR_ARG, "null argument");
+    if (!c->has_volume || c->W <= 0) return fail(VT_ERR_STATE, "volume and frame required");
+    VT_HIP(hipSetDevice(c->device));
+    vt::TraceArgs a = trace_args(c);
+    a.seed = seed; a.n0 = frame;
+    const size_t bytes = (size_t)c->W * c->H * 8 * 4;
+    int32_t* d = nullptr;
+    VT_HIP(hipMalloc(&d, bytes));
+    VT_HIP(vt::launch_first_hit(a, d, c->stream));
+    VT_HIP(hipMemcpyAsync(rec, d, bytes, hipMemcpyDeviceToHost, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    dfree(d);
+    return VT_OK;
+}
+
+int vt_trace_rays(vt_context* c, int32_t n, const float* o, const float* dir, int32_t* rec)
+{
+    if (!c || n < 0 || (n > 0 && (!o || !dir || !rec))) return fail(VT_ERR_ARG, "bad argument");
+    if (!c->has_volume) return fail(VT_ERR_STATE, "no volume");
+    if (n == 0) return VT_OK;
+    VT_HIP(hipSetDevice(c->device));
+    vt::TraceArgs a = trace_args(c);
+    float* d_o = nullptr;
+    float* d_d = nullptr;
+    int32_t* d_r = nullptr;
+    VT_HIP(hipMalloc(&d_o, (size_t)n * 12));
+    VT_HIP(hipMalloc(&d_d, (size_t)n * 12));
+    VT_HIP(hipMalloc(&d_r, (size_t)n * 32));
+    VT_HIP(hipMemcpyAsync(d_o, o, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipMemcpyAsync(d_d, dir, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_trace_rays(a, n, d_o, d_d, d_r, c->stream));
+    VT_HIP(hipMemcpyAsync(rec, d_r, (size_t)n * 32, hipMemcpyDeviceToHost, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    dfree(d_o); dfree(d_d); dfree(d_r);
+    return VT_OK;
+}
+
+int vt_voxelize(vt_context* c, const float* verts, int64_t nv, const uint32_t* idx, int64_t ntri, const float* M,
+                int32_t mode, int32_t material_offset, uint8_t* occupancy)
+{
+    if (!c || nv < 0 || ntri < 0 || !M || (nv > 0 && !verts) || (ntri > 0 && !idx))
+        return fail(VT_ERR_ARG, "bad argument");
+    if (mode != 0 && mode != 1) return fail(VT_ERR_ARG, "mode must be 0 (THIN) or 1 (FAT)");
+    if (!c->has_volume) return fail(VT_ERR_STATE, "no volume: call vt_set_volume first");
+    for (int64_t i = 0; i < 3 * ntri; ++i)
+        if ((int64_t)idx[i] >= nv) return fail(VT_ERR_ARG, "triangle index out of range");
+    if (material_offset < 0 || material_offset >= (int32_t)c->mat.size())
+        return fail(VT_ERR_ARG, "material offset outside material data");
+    int slot = 0;
+    for (int s = 1; s <= c->n_slots; ++s)
+        if (c->slot_off[s] == material_offset) slot = s;
+    if (!slot) {
+        if (c->n_slots == 255) return fail(VT_ERR_LIMIT, "no free material slot");
+        slot = ++c->n_slots;
+        c->slot_off[slot] = material_offset;
+        int r = upload_records(c);
+        if (r != VT_OK) return r;
+    }
+    VT_HIP(hipSetDevice(c->device));
+    vt::VoxArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.d = c->dims;
+    a.ntri = ntri;
+    std::memcpy(a.M, M, sizeof(a.M));
+    a.mode = mode;
+    a.slot = (uint8_t)slot;
+    a.bricks = c->d_bricks;
+    float* d_v = nullptr;
+    uint32_t* d_i = nullptr;
+    uint8_t* d_occ = nullptr;
+    const int64_t nvox = (int64_t)c->dims.nx * c->dims.ny * c->dims.nz;
+    if (ntri > 0) {
+        VT_HIP(hipMalloc(&d_v, (size_t)nv * 12));
+        VT_HIP(hipMalloc(&d_i, (size_t)ntri * 12));
+        VT_HIP(hipMemcpyAsync(d_v, verts, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream));
+        VT_HIP(hipMemcpyAsync(d_i, idx, (size_t)ntri * 12, hipMemcpyHostToDevice, c->stream));
+    }
+    if (occupancy) {
+        VT_HIP(hipMalloc(&d_occ, (size_t)nvox));
+        VT_HIP(vt::launch_fill_u8(d_occ, (uint64_t)nvox, 0, c->stream));
+    }
+    a.verts = d_v; a.idx = d_i; a.occ = d_occ;
+    VT_HIP(vt::launch_voxelize(a, c->stream));
+    VT_HIP(vt::launch_fill_u32(c->d_bits, c->bit_words, 0u, c->stream));
+    VT_HIP(vt::launch_brick_bits(c->dims, c->d_bricks, c->d_bits, c->stream));
+    if (occupancy) VT_HIP(hipMemcpyAsync(occupancy, d_occ, (size_t)nvox, hipMemcpyDeviceToHost, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    dfree(d_v); dfree(d_i); dfree(d_occ);
+    return VT_OK;
+}
+
+int vt_camera_matrices(const float* eye, const float* target, float fov_y, float near_d, float far_d, int32_t lens_model,
+                       int32_t width, int32_t height, float* mv, float* inv_mv, float* proj, float* inv_proj)
+{
+    if (!eye || !target || !mv || !inv_mv || !proj || !inv_proj || width <= 0 || height <= 0)
+        return fail(VT_ERR_ARG, "bad argument");
+    vt::host::camera_matrices(eye, target, fov_y, near_d, far_d, lens_model, width, height, mv, inv_mv, proj, inv_proj);
+    return VT_OK;
+}
+
+int vt_env_build_cdf(const float* rgb, int32_t w, int32_t h, int32_t* cu_w, int32_t* cu_h, float* cdf_u, int32_t* cv_n,
+                     float* cdf_v, float* integral)
+{
+    if (!rgb || w <= 0 || h <= 0 || !cu_w || !cu_h || !cv_n) return fail(VT_ERR_ARG, "bad argument");
+    std::vector<float> f;
+    int fw = 0, fh = 0;
+    vt::host::image_function(rgb, w, h, 512, f, fw, fh);
+    *cu_w = fw + 1;
+    *cu_h = fh;
+    *cv_n = fh + 1;
+    if (!cdf_u || !cdf_v) return VT_OK;
+    float integ = 0.0f;
+    vt::host::calculate_cdf(f.data(), fw, fh, cdf_u, cdf_v, &integ);
+    if (integral) *integral = integ;
+    return VT_OK;
+}
+
+int32_t vt_prune_emissive(const int32_t* offsets, int32_t nx, int32_t ny, int32_t nz, int32_t* emissive, int32_t n)
+{
+    if (!offsets || (n > 0 && !emissive)) return fail(VT_ERR_ARG, "bad argument");
+    const int32_t res[3] = { nx, ny, nz };
+    return vt::host::prune_emissive(offsets, res, emissive, n);
+}
+
+int vt_device_math(vt_context* c, int32_t fn, int32_t n, const float* a, const float* b, float* out)
+{
+    if (!c || n < 0 || (n > 0 && (!a || !b || !out))) return fail(VT_ERR_ARG, "bad argument");
+    if (n == 0) return VT_OK;
+    VT_HIP(hipSetDevice(c->device));
+    float *da = nullptr, *db = nullptr, *dout = nullptr;
+    VT_HIP(hipMalloc(&da, (size_t)n * 4));
+    VT_HIP(hipMalloc(&db, (size_t)n * 4));
+    VT_HIP(hipMalloc(&dout, (size_t)n * 4));
+    VT_HIP(hipMemcpyAsync(da, a, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipMemcpyAsync(db, b, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_device_math(fn, n, da, db, dout, c->stream));
+    VT_HIP(hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    dfree(da); dfree(db); dfree(dout);
+    return VT_OK;
+}
+
+}  // extern "C"

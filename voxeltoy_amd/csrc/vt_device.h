@@ -1,0 +1,479 @@
+/*
+ * vt_device.h - device-side data views and per-ray functions of the voxel
+ * path tracer (gfx950).
+ *
+ * Grid layout in HBM: the dense voxel grid is split into 8x8x8 bricks of one
+ * byte per voxel (the material slot, 0 = empty).  Bricks are stored in Morton
+ * order of their brick coordinates (over the power-of-two cube that covers
+ * the grid) and voxels inside a brick in Morton order of their local
+ * coordinates, so a 64-byte line holds a 4x4x4 block.  One occupancy bit per
+ * brick (same Morton index) lets the traversal skip the voxel loads of empty
+ * bricks; the bit array is staged in LDS when it fits.
+ *
+ * The traversal replays the reference 3D-DDA (src/shaders/shared/dda.h:7-100)
+ * step by step: `dis` is a running FP32 sum, so empty bricks skip the loads
+ * but not the compare/add sequence, and hit voxel, face and step count are
+ * identical to the reference arithmetic.  All float expressions keep the
+ * reference operand order and are compiled with -ffp-contract=off.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define VT_FN __device__ __forceinline__
+#include "vtmath.h"
+
+namespace vt {
+
+/* ---- small float3 with explicit, unfused arithmetic ----------------------- */
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ f3 operator-(f3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b)
+{
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ f3 normalize(f3 v) { const float inv = 1.0f / sqrtf(dot(v, v)); return v * inv; }
+__device__ __forceinline__ float length(f3 v) { return sqrtf(dot(v, v)); }
+
+/* mat4 (row-major) x vec4, column products summed left to right. */
+__device__ __forceinline__ void mat4_mul(const float* m, float x, float y, float z, float w, float* o)
+{
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        o[r] = ((m[r * 4 + 0] * x + m[r * 4 + 1] * y) + m[r * 4 + 2] * z) + m[r * 4 + 3] * w;
+}
+
+/* ---- views ------------------------------------------------------------------ */
+struct MatRec {                 /* one decoded material record (32 bytes) */
+    int32_t type;               /* 0 matte, 1 metal, 2 plastic, -1 none */
+    float e[3];                 /* emission */
+    float r[3];                 /* albedo / reflectance */
+    float rough;                /* Blinn exponent (the reference uses roughness unmapped) */
+};
+
+struct GridView {
+    int32_t nx, ny, nz;
+    int32_t max_steps;
+    float rx, ry, rz;           /* resolution as float */
+    float bmin[3], bmax[3], vs[3], ext[3];
+    const uint8_t* bricks;      /* Morton bricks of 512 bytes */
+    const uint32_t* bits;       /* brick occupancy bits (global copy) */
+    uint32_t bit_words;
+    int32_t n_emissive;
+    const int32_t* emissive;    /* linear voxel indices */
+};
+
+struct EnvView {
+    int32_t use_image;
+    float top[3], bottom[3];
+    int32_t w, h;
+    const float* img;           /* RGB */
+    int32_t cu_w, cu_h;
+    const float* cdfu;
+    int32_t cv_n;
+    const float* cdfv;
+    float integral, rotation;
+};
+
+struct CamView {
+    float inv_mv[16];
+    float inv_proj[16];
+    float P32, P22, P23;        /* proj[2][3], proj[2][2], proj[3][2] (row, col) */
+    float viewport[4];
+    float win_z;
+    float lens_radius;
+    float focal_distance;
+    int32_t lens_model;
+};
+
+struct Settings {
+    int32_t max_bounces;
+    float exposure;
+    float inv_gamma;            /* 1.0f / gamma */
+    int32_t sel[3];
+    float wire_opacity, wire_thickness;
+};
+
+/* ---- Morton indices -------------------------------------------------------------- */
+__device__ __forceinline__ uint32_t spread3_10(uint32_t v)
+{
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t brick_morton(uint32_t bx, uint32_t by, uint32_t bz)
+{
+    return spread3_10(bx) | (spread3_10(by) << 1) | (spread3_10(bz) << 2);
+}
+__device__ __forceinline__ uint32_t spread3_3(uint32_t v) { return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4); }
+__device__ __forceinline__ uint32_t local_morton(uint32_t x, uint32_t y, uint32_t z)
+{
+    return spread3_3(x & 7u) | (spread3_3(y & 7u) << 1) | (spread3_3(z & 7u) << 2);
+}
+
+/* ---- rayAABBIntersection (aabb.h:1-32) -------------------------------------------- */
+__device__ __forceinline__ float aabb(f3 o, f3 d, f3 bmin, f3 bmax)
+{
+    const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const f3 t0 = (bmin - o) * inv;
+    const f3 t1 = (bmax - o) * inv;
+    const float tminf = vt_maxf(vt_minf(t0.x, t1.x), vt_maxf(vt_minf(t0.y, t1.y), vt_minf(t0.z, t1.z)));
+    const float tmaxf = vt_minf(vt_maxf(t0.x, t1.x), vt_minf(vt_maxf(t0.y, t1.y), vt_maxf(t0.z, t1.z)));
+    if (tmaxf < 0.0f) return -1.0f;
+    if (tminf > tmaxf) return -1.0f;
+    return vt_maxf(0.0f, tminf);
+}
+
+/* ---- 3D-DDA (dda.h:7-61) -----------------------------------------------------------
+ * Returns 1 on hit, 0 when the march ended without a hit (position written),
+ * -1 when the start point lies outside the grid (position not written).
+ * `slot` receives the hit voxel's material slot. */
+struct Hit { int32_t x, y, z, steps; uint32_t slot; };
+
+template <bool COUNT>
+__device__ __forceinline__ int dda(const GridView& g, const uint32_t* __restrict__ bits, f3 o, f3 d, Hit& h,
+                                   uint32_t& visits)
+{
+    o = o + mk(vt_signf(d.x), vt_signf(d.y), vt_signf(d.z)) * 0.001f;
+    const float vox = ((o.x - g.bmin[0]) * g.ext[0]) * g.rx;
+    const float voy = ((o.y - g.bmin[1]) * g.ext[1]) * g.ry;
+    const float voz = ((o.z - g.bmin[2]) * g.ext[2]) * g.rz;
+    const float fx = floorf(vox), fy = floorf(voy), fz = floorf(voz);
+    if (fx < 0.0f || fy < 0.0f || fz < 0.0f || fx >= g.rx || fy >= g.ry || fz >= g.rz) {
+        h.steps = -1;
+        return -1;
+    }
+    /* mix(d, 1e-5, step(|d|, 1e-5)) as an exact select */
+    const float dx = vt_stepf(vt_absf(d.x), 1e-5f) != 0.0f ? 1e-5f : d.x;
+    const float dy = vt_stepf(vt_absf(d.y), 1e-5f) != 0.0f ? 1e-5f : d.y;
+    const float dz = vt_stepf(vt_absf(d.z), 1e-5f) != 0.0f ? 1e-5f : d.z;
+    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const float sx = vt_signf(dx), sy = vt_signf(dy), sz = vt_signf(dz);
+    float disx = (((fx - vox) + 0.5f) + sx * 0.5f) * ix;
+    float disy = (((fy - voy) + 0.5f) + sy * 0.5f) * iy;
+    float disz = (((fz - voz) + 0.5f) + sz * 0.5f) * iz;
+    /* (mask*sign)*inc with mask in {0,1} is |inc| when the axis steps */
+    const float ax = vt_absf(ix), ay = vt_absf(iy), az = vt_absf(iz);
+    const int32_t stx = sx > 0.0f ? 1 : (sx < 0.0f ? -1 : 0);
+    const int32_t sty = sy > 0.0f ? 1 : (sy < 0.0f ? -1 : 0);
+    const int32_t stz = sz > 0.0f ? 1 : (sz < 0.0f ? -1 : 0);
+    int32_t x = (int32_t)fx, y = (int32_t)fy, z = (int32_t)fz;
+    int32_t steps = 0;
+    int hit = 0;
+    uint32_t cb = 0xffffffffu, bm = 0;
+    int32_t cbx = -1, cby = -1, cbz = -1;
+    uint32_t occ = 0;
+    uint32_t slot = 0;
+    while (steps < g.max_steps) {
+        if ((uint32_t)x >= (uint32_t)g.nx || (uint32_t)y >= (uint32_t)g.ny || (uint32_t)z >= (uint32_t)g.nz) break;
+        if (COUNT) visits++;
+        const int32_t bx = x >> 3, by = y >> 3, bz = z >> 3;
+        if (bx != cbx || by != cby || bz != cbz) {
+            cbx = bx; cby = by; cbz = bz;
+            bm = brick_morton((uint32_t)bx, (uint32_t)by, (uint32_t)bz);
+            occ = (bits[bm >> 5] >> (bm & 31u)) & 1u;
+            cb = bm;
+        }
+        if (occ) {
+            const uint32_t v = g.bricks[(size_t)cb * 512u + local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z)];
+            if (v) { hit = 1; slot = v; break; }
+        }
+        const bool mx = !(disy < disx) && !(disz < disx);
+        const bool my = !(disx < disy) && !(disz < disy);
+        const bool mz = !(disy < disz) && !(disx < disz);
+        if (mx) { disx = disx + ax; x += stx; }
+        if (my) { disy = disy + ay; y += sty; }
+        if (mz) { disz = disz + az; z += stz; }
+        steps++;
+    }
+    h.x = x; h.y = y; h.z = z; h.steps = steps; h.slot = slot;
+    return hit;
+}
+
+/* traverse (dda.h:63-79): 2 voxel hit, 3 ground hit (final y < 0), 0 miss. */
+enum { KIND_MISS = 0, KIND_VOXEL = 2, KIND_GROUND = 3 };
+
+template <bool COUNT>
+__device__ __forceinline__ int traverse(const GridView& g, const uint32_t* bits, f3 o, f3 d, Hit& h, uint32_t& visits)
+{
+    const int r = dda<COUNT>(g, bits, o, d, h, visits);
+    if (r > 0) return KIND_VOXEL;
+    if (r < 0) return KIND_MISS;
+    return h.y < 0 ? KIND_GROUND : KIND_MISS;
+}
+
+/* Slot of a voxel position: texelFetch semantics, outside the grid reads
+ * offset 0, which is record index 0 of the decoded table. */
+__device__ __forceinline__ uint32_t fetch_slot(const GridView& g, const uint32_t* bits, int32_t x, int32_t y, int32_t z)
+{
+    if ((uint32_t)x >= (uint32_t)g.nx || (uint32_t)y >= (uint32_t)g.ny || (uint32_t)z >= (uint32_t)g.nz) return 0u;
+    const uint32_t bm = brick_morton((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3);
+    return g.bricks[(size_t)bm * 512u + local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z)];
+}
+
+/* ---- voxelSpaceToWorldSpace (coordinates.h:59-82) ---------------------------------- */
+struct Basis { f3 p, t, n, b; };
+
+__device__ __forceinline__ Basis voxel_basis(const GridView& g, float vx, float vy, float vz, f3 o, f3 d)
+{
+    Basis B;
+    const f3 vs = mk(g.vs[0], g.vs[1], g.vs[2]);
+    const f3 vmin = mk(vx, vy, vz) * vs + mk(g.bmin[0], g.bmin[1], g.bmin[2]);
+    const f3 vmax = vmin + vs;
+    const float t = aabb(o, d, vmin, vmax);
+    B.p = o + d * t;
+    const f3 c = vmin + vs * 0.5f;
+    const f3 c2h = B.p - c;
+    const float ax = vt_absf(c2h.x), ay = vt_absf(c2h.y), az = vt_absf(c2h.z);
+    const f3 m = mk(vt_stepf(ay, ax) * vt_stepf(az, ax), vt_stepf(ax, ay) * vt_stepf(az, ay),
+                    vt_stepf(ax, az) * vt_stepf(ay, az));
+    B.n = m * mk(vt_signf(c2h.x), vt_signf(c2h.y), vt_signf(c2h.z));
+    const float k = 0.57735026919f;
+    const f3 t0 = cross(B.n, mk(k, k, k));
+    B.b = normalize(cross(t0, B.n));
+    B.t = normalize(cross(B.b, B.n));
+    return B;
+}
+
+__device__ __forceinline__ f3 local_to_world(f3 v, const Basis& B) { return (B.t * v.x + B.n * v.y) + B.b * v.z; }
+__device__ __forceinline__ f3 world_to_local(f3 v, const Basis& B) { return mk(dot(v, B.t), dot(v, B.n), dot(v, B.b)); }
+
+/* ---- camera rays (generateRay.h, coordinates.h:1-27) ------------------------------- */
+__device__ __forceinline__ void screen_to_eye(const CamView& c, float wx, float wy, float wz, float* eye)
+{
+    const float nx = ((2.0f * wx) - (2.0f * c.viewport[0])) / c.viewport[2] - 1.0f;
+    const float ny = ((2.0f * wy) - (2.0f * c.viewport[1])) / c.viewport[3] - 1.0f;
+    const float nz = ((2.0f * wz) - 0.0f - 1.0f) / (1.0f - 0.0f);
+    const float w = c.P32 / (nz - (c.P22 / c.P23));
+    mat4_mul(c.inv_proj, nx * w, ny * w, nz * w, w, eye);
+}
+
+__device__ __forceinline__ void gen_ray(const CamView& c, float fx, float fy, const float* u, f3& o, f3& d)
+{
+    float tmp[4];
+    if (c.lens_model == 0) {
+        float es[4];
+        screen_to_eye(c, fx + (u[0] - 0.5f), fy + (u[1] - 0.5f), c.win_z + 0.0f, es);
+        mat4_mul(c.inv_mv, 0.0f, 0.0f, 0.0f, 1.0f, tmp);
+        o = mk(tmp[0], tmp[1], tmp[2]);
+        const f3 ed = normalize(mk(es[0], es[1], es[2]));
+        mat4_mul(c.inv_mv, ed.x, ed.y, ed.z, 0.0f, tmp);
+        const float l2 = ((tmp[0] * tmp[0] + tmp[1] * tmp[1]) + tmp[2] * tmp[2]) + tmp[3] * tmp[3];
+        const float inv = 1.0f / sqrtf(l2);
+        d = mk(tmp[0] * inv, tmp[1] * inv, tmp[2] * inv);
+    } else if (c.lens_model == 1) {
+        const float r = sqrtf(u[0]);
+        float sn, cs;
+        vt_sincosf((2.0f * VT_PI_F) * u[1], &sn, &cs);
+        const float lx = (r * cs) * c.lens_radius, ly = (r * sn) * c.lens_radius;
+        float eo[4];
+        screen_to_eye(c, fx, fy, c.win_z, eo);
+        const f3 esO = mk(eo[0], eo[1], eo[2]);
+        const f3 esD = normalize(esO);
+        const float t = c.focal_distance / -(esD.z);
+        const f3 esF = esO + esD * t;
+        mat4_mul(c.inv_mv, esF.x, esF.y, esF.z, 1.0f, tmp);
+        const f3 wsF = mk(tmp[0], tmp[1], tmp[2]);
+        mat4_mul(c.inv_mv, lx, ly, 0.0f, 1.0f, tmp);
+        o = mk(tmp[0], tmp[1], tmp[2]);
+        d = normalize(wsF - o);
+    } else {
+        float eye[4];
+        const float nx = (fx / c.viewport[2]) * 2.0f - 1.0f;
+        const float ny = (fy / c.viewport[3]) * 2.0f - 1.0f;
+        const float nz = ((2.0f * c.win_z) - 0.0f - 1.0f) / (1.0f - 0.0f);
+        mat4_mul(c.inv_proj, nx, ny, nz, 1.0f, eye);
+        mat4_mul(c.inv_mv, eye[0], eye[1], eye[2], eye[3], tmp);
+        o = mk(tmp[0], tmp[1], tmp[2]);
+        mat4_mul(c.inv_mv, 0.0f, 0.0f, -1.0f, 0.0f, tmp);
+        d = mk(tmp[0], tmp[1], tmp[2]);
+    }
+}
+
+/* ---- environment (lights.h, coordinates.h:97-116, envMapSample.h) ------------------- */
+__device__ __forceinline__ f3 tex_bilinear(const EnvView& e, float u, float v)
+{
+    if (vt_isnan(u) || vt_isnan(v)) return mk(vt_nanf(), vt_nanf(), vt_nanf());
+    u = vt_minf(vt_maxf(u, 0.0f), 1.0f);
+    v = vt_minf(vt_maxf(v, 0.0f), 1.0f);
+    const float s = u * (float)e.w - 0.5f;
+    const float t = v * (float)e.h - 0.5f;
+    const float fs = floorf(s), ft = floorf(t);
+    const int32_t i0 = (int32_t)fs, j0 = (int32_t)ft;
+    const float a = s - fs, b = t - ft;
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    const bool i0ok = i0 >= 0 && i0 < e.w, i1ok = i0 + 1 >= 0 && i0 + 1 < e.w;
+    const bool j0ok = j0 >= 0 && j0 < e.h, j1ok = j0 + 1 >= 0 && j0 + 1 < e.h;
+    float out[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const float t00 = (i0ok && j0ok) ? e.img[((size_t)j0 * e.w + i0) * 3 + ch] : 0.0f;
+        const float t10 = (i1ok && j0ok) ? e.img[((size_t)j0 * e.w + i0 + 1) * 3 + ch] : 0.0f;
+        const float t01 = (i0ok && j1ok) ? e.img[((size_t)(j0 + 1) * e.w + i0) * 3 + ch] : 0.0f;
+        const float t11 = (i1ok && j1ok) ? e.img[((size_t)(j0 + 1) * e.w + i0 + 1) * 3 + ch] : 0.0f;
+        out[ch] = ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11;
+    }
+    return mk(out[0], out[1], out[2]);
+}
+
+__device__ __forceinline__ void uv_from_dir(f3 d, float rot, float& u, float& v)
+{
+    const float theta = vt_acosf(d.y);
+    const float phi = (vt_atan2f(d.z, d.x) + VT_PI_F) + rot;
+    const float a = phi / VT_TWO_PI_F;
+    u = a - floorf(a);
+    v = theta / VT_PI_F;
+}
+
+__device__ __forceinline__ f3 dir_from_uv(float u, float v, float rot)
+{
+    const float phi = ((u * VT_TWO_PI_F) - VT_PI_F) - rot;
+    const float theta = v * VT_PI_F;
+    float st, ct, sp, cp;
+    vt_sincosf(theta, &st, &ct);
+    vt_sincosf(phi, &sp, &cp);
+    return mk(st * cp, ct, st * sp);
+}
+
+__device__ __forceinline__ float luminance(f3 c) { return (c.x * 0.2126f + c.y * 0.7152f) + c.z * 0.0722f; }
+
+__device__ __forceinline__ f3 background(const EnvView& e, f3 v)
+{
+    if (e.use_image) {
+        float u, w;
+        uv_from_dir(v, e.rotation, u, w);
+        return tex_bilinear(e, u, w);
+    }
+    const float bias = vt_maxf(0.0f, v.y);
+    return mk(e.bottom[0], e.bottom[1], e.bottom[2]) * (1.0f - bias) + mk(e.top[0], e.top[1], e.top[2]) * bias;
+}
+
+__device__ __forceinline__ int bsearch_cdf(const float* cdf, int hi, float x)
+{
+    int lo = 0;
+    if (hi < 1) return 0;
+    while (lo != hi - 1) {
+        const int mid = (lo + hi) / 2;
+        if (x < cdf[mid]) hi = mid; else lo = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ float cdf_at(const float* cdf, int n, int i) { return (i < 0 || i >= n) ? 0.0f : cdf[i]; }
+
+/* sampleEnvironmentTexture (envMapSample.h:23-65); cdfv may point to LDS. */
+__device__ __forceinline__ void env_sample_uv(const EnvView& e, const float* cdfv, float su, float sv, float& u, float& v)
+{
+    const int sizeU = e.cu_w, sizeV = e.cv_n;
+    const int row = bsearch_cdf(cdfv, sizeV - 1, sv);
+    const float* rowp = e.cdfu + (size_t)row * (size_t)sizeU;
+    const int col = bsearch_cdf(rowp, sizeU - 1, su);
+    const float loU = cdf_at(rowp, sizeU, col), upU = cdf_at(rowp, sizeU, col + 1);
+    const float du = (su - loU) / (upU - loU);
+    const float loV = cdf_at(cdfv, sizeV, row), upV = cdf_at(cdfv, sizeV, row + 1);
+    const float dv = (sv - loV) / (upV - loV);
+    u = ((float)col + du) / (float)(sizeU - 1);
+    v = ((float)row + dv) / (float)(sizeV - 1);
+}
+
+/* ---- BSDFs (lambertian.h, microfacet.h) -------------------------------------------- */
+__device__ __forceinline__ float power_heuristic(float f, float g) { return (f * f) / ((f * f) + (g * g)); }
+
+__device__ __forceinline__ float G_term(f3 wo, f3 wi, f3 wh)
+{
+    const float WoDotWh = vt_absf(dot(wo, wh));
+    return vt_minf(1.0f, vt_minf(((2.0f * wh.y) * wo.y) / WoDotWh, ((2.0f * wh.y) * wi.y) / WoDotWh));
+}
+
+__device__ __forceinline__ void D_term(f3 refl, float e, f3 wo, f3 wh, f3& f, float& pdf)
+{
+    const float pc = vt_powf(wh.y, e);
+    const float fv = ((e + 2.0f) * VT_INV_TWOPI_F) * pc;
+    const float woDotWh = dot(wo, wh);
+    pdf = (woDotWh <= 0.0f) ? 0.0f : ((e + 1.0f) * pc) / ((VT_TWO_PI_F * 4.0f) * woDotWh);
+    f = refl * fv;
+}
+
+__device__ __forceinline__ float F_term(float c)
+{
+    const float ior = 7.3f;
+    const float sq = (1.0f - ior) / (1.0f + ior);
+    const float r0 = sq * sq;
+    return r0 + (1.0f - r0) * vt_powf(1.0f - c, 5.0f);
+}
+
+__device__ __forceinline__ void eval_microfacet(f3 refl, float e, f3 wo, f3 wi, f3& f, float& pdf)
+{
+    const float cWi = wi.y, cWo = wo.y;
+    const f3 wh = normalize(wi + wo);
+    const float cH = dot(wi, wh);
+    D_term(refl, e, wo, wh, f, pdf);
+    const float k = (G_term(wo, wi, wh) * F_term(cH)) / ((4.0f * cWo) * cWi);
+    f = f * k;
+}
+
+__device__ __forceinline__ f3 rec_r(const MatRec& m) { return mk(m.r[0], m.r[1], m.r[2]); }
+
+__device__ __forceinline__ void eval_material(const MatRec& m, f3 wo, f3 wi, f3& f, float& pdf)
+{
+    if (m.type == 0) {
+        pdf = wi.y / VT_PI_F;
+        f = rec_r(m) / VT_PI_F;
+    } else if (m.type == 1) {
+        eval_microfacet(rec_r(m), m.rough, wo, wi, f, pdf);
+    } else if (m.type == 2) {
+        eval_microfacet(mk(1.0f, 1.0f, 1.0f), m.rough, wo, wi, f, pdf);
+    } else {
+        f = mk(0.0f, 0.0f, 0.0f);
+        pdf = 0.0f;
+    }
+}
+
+/* sampleMaterialBSDF (materials.h:30-43).  Draws one rand() except for an
+ * unknown material type. */
+__device__ __forceinline__ f3 sample_material(const MatRec& m, f3 wo, vt_pcg32& rng, f3& f, float& pdf)
+{
+    if (m.type < 0) { f = mk(0.0f, 0.0f, 0.0f); pdf = 0.0f; return mk(0.0f, 0.0f, 0.0f); }
+    float u[4];
+    vt_rand4(&rng, u);
+    if (m.type == 0) {
+        const float r = sqrtf(u[0]);
+        float sn, cs;
+        vt_sincosf((2.0f * VT_PI_F) * u[1], &sn, &cs);
+        const float px = r * cs, py = r * sn;
+        const float y = sqrtf(vt_maxf(0.0f, (1.0f - px * px) - py * py));
+        f = rec_r(m) / VT_PI_F;
+        pdf = y / VT_PI_F;
+        return mk(px, y, py);
+    }
+    const f3 refl = m.type == 1 ? rec_r(m) : mk(1.0f, 1.0f, 1.0f);
+    const float e = m.rough;
+    const float cosT = vt_powf(u[0], 1.0f / (e + 1.0f));
+    const float sinT = sqrtf(vt_maxf(0.0f, 1.0f - cosT * cosT));
+    const float phi = (u[1] * 2.0f) * VT_PI_F;
+    float sp, cp;
+    vt_sincosf(phi, &sp, &cp);
+    const f3 wh = mk(sinT * cp, cosT, sinT * sp);
+    const f3 wi = (-wo) + wh * (2.0f * dot(wo, wh));
+    D_term(refl, e, wo, wh, f, pdf);
+    const float cWi = wi.y, cWo = wo.y;
+    const f3 wh2 = normalize(wi + wo);
+    const float cH = dot(wi, wh2);
+    const f3 t1 = refl * f;
+    const f3 t2 = t1 * G_term(wo, wi, wh2);
+    const f3 t3 = t2 * F_term(cH);
+    f = t3 / ((4.0f * cWo) * cWi);
+    return wi;
+}
+
+}  // namespace vt

@@ -1,0 +1,77 @@
+/*
+ * vt_kernels.h - kernel argument blocks and launchers shared by the HIP
+ * translation units of the voxel path tracer.
+ */
+#pragma once
+
+#include "vt_device.h"
+
+namespace vt {
+
+struct TraceArgs {
+    GridView g;
+    EnvView e;
+    CamView c;
+    Settings s;
+    const MatRec* recs;          /* 256 records: [0] = offset 0 (ground), [slot] */
+    int32_t W, H;
+    uint64_t seed;
+    uint32_t n0, count, rng_frame_max;
+    float4* accum;
+    double* sum64;
+    unsigned long long* stats;   /* 6 counters */
+    uint32_t* tile_counter;
+    int32_t tiles_x, n_tiles;
+    int32_t lds_bits;            /* stage occupancy bits in LDS */
+    int32_t cdfv_lds_bytes;
+};
+
+size_t render_lds_bytes(const TraceArgs& a);
+hipError_t launch_render(const TraceArgs& a, int grid, hipStream_t s, bool stats);
+int render_blocks_per_cu(const TraceArgs& a, bool stats);
+hipError_t launch_first_hit(const TraceArgs& a, int32_t* out, hipStream_t s);
+hipError_t launch_trace_rays(const TraceArgs& a, int32_t n, const float* o, const float* d, int32_t* out,
+                             hipStream_t s);
+hipError_t launch_reset_counter(uint32_t* c, unsigned long long* stats, hipStream_t s);
+hipError_t launch_device_math(int32_t fn, int32_t n, const float* a, const float* b, float* out, hipStream_t s);
+
+/* ---- volume layout --------------------------------------------------------------------- */
+struct BrickDims {
+    int32_t nx, ny, nz;          /* voxels */
+    int32_t bx, by, bz;          /* bricks per axis */
+    int32_t bdim;                /* power-of-two cube edge in bricks (Morton range) */
+    uint64_t n_bricks;           /* bdim^3 */
+};
+
+/* Dense int32 offsets -> bricked slots via lut[offset] (offset in
+ * [0, lut_n)), -1 (or any negative) -> slot 0.  Out-of-range offsets set
+ * *bad. */
+hipError_t launch_brick_from_offsets(const BrickDims& d, const int32_t* dense, const uint8_t* lut, int32_t lut_n,
+                                     uint8_t* bricks, uint32_t* bad, hipStream_t s);
+/* Dense uint8 ids -> bricked slots via remap[256]. */
+hipError_t launch_brick_from_ids(const BrickDims& d, const uint8_t* dense, const uint8_t* remap, uint8_t* bricks,
+                                 hipStream_t s);
+/* Recompute brick occupancy bits (bits must hold n_bricks/32 words). */
+hipError_t launch_brick_bits(const BrickDims& d, const uint8_t* bricks, uint32_t* bits, hipStream_t s);
+/* Bricked slots -> dense int32 offsets via slot_off[256]. */
+hipError_t launch_dense_from_bricks(const BrickDims& d, const uint8_t* bricks, const int32_t* slot_off, int32_t* dense,
+                                    hipStream_t s);
+/* Fill a byte buffer (graph-safe replacement for hipMemsetAsync). */
+hipError_t launch_fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+/* ---- voxelizer ------------------------------------------------------------------------- */
+struct VoxArgs {
+    BrickDims d;
+    const float* verts;
+    const uint32_t* idx;
+    int64_t ntri;
+    float M[16];
+    int32_t mode;                /* 0 THIN, 1 FAT */
+    uint8_t slot;
+    uint8_t* bricks;             /* may be null */
+    uint8_t* occ;                /* dense occupancy, may be null */
+};
+hipError_t launch_voxelize(const VoxArgs& a, hipStream_t s);
+
+}  // namespace vt

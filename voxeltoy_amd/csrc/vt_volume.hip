@@ -1,0 +1,283 @@
+/*
+ * vt_volume.hip - grid layout conversion and the triangle voxelizer.
+ *
+ * The voxelizer kernel restates voxelize.vs (:20-28) and voxelize.gs
+ * (swizzleTri :83-137, voxelizeTriPostSwizzle :146-260, main :262-279) with
+ * one thread per triangle; covered voxels are written straight into the
+ * Morton-bricked grid (and optionally into a dense byte image).  Duplicate
+ * writes of the same value by neighbouring triangles are the same benign
+ * race the geometry shader has (voxelize.gs:142).
+ */
+#include "vt_kernels.h"
+
+namespace vt {
+
+__device__ __forceinline__ uint64_t brick_byte(int32_t x, int32_t y, int32_t z)
+{
+    return (uint64_t)brick_morton((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3) * 512u +
+           local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z);
+}
+
+__global__ __launch_bounds__(256) void brick_from_offsets_kernel(BrickDims d, const int32_t* __restrict__ dense,
+                                                                 const uint8_t* __restrict__ lut, int32_t lut_n,
+                                                                 uint8_t* __restrict__ bricks, uint32_t* bad)
+{
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t x = (int32_t)(i % d.nx);
+        const int64_t r = i / d.nx;
+        const int32_t y = (int32_t)(r % d.ny), z = (int32_t)(r / d.ny);
+        const int32_t off = dense[i];
+        uint8_t slot = 0;
+        if (off >= 0) {
+            if (off < lut_n && lut[off] != 0) slot = lut[off];
+            else atomicOr(bad, 1u);
+        }
+        bricks[brick_byte(x, y, z)] = slot;
+    }
+}
+
+__global__ __launch_bounds__(256) void brick_from_ids_kernel(BrickDims d, const uint8_t* __restrict__ dense,
+                                                             const uint8_t* __restrict__ remap,
+                                                             uint8_t* __restrict__ bricks)
+{
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t x = (int32_t)(i % d.nx);
+        const int64_t r = i / d.nx;
+        const int32_t y = (int32_t)(r % d.ny), z = (int32_t)(r / d.ny);
+        bricks[brick_byte(x, y, z)] = remap[dense[i]];
+    }
+}
+
+__global__ __launch_bounds__(256) void brick_bits_kernel(BrickDims d, const uint8_t* __restrict__ bricks,
+                                                         uint32_t* __restrict__ bits)
+{
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < d.n_bricks;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4* p = (const uint4*)(bricks + b * 512u);
+        uint32_t acc = 0;
+#pragma unroll 8
+        for (int k = 0; k < 32; ++k) {
+            const uint4 v = p[k];
+            acc |= v.x | v.y | v.z | v.w;
+        }
+        if (acc) atomicOr(&bits[b >> 5], 1u << (b & 31u));
+    }
+}
+
+__global__ __launch_bounds__(256) void dense_from_bricks_kernel(BrickDims d, const uint8_t* __restrict__ bricks,
+                                                                const int32_t* __restrict__ slot_off,
+                                                                int32_t* __restrict__ dense)
+{
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t x = (int32_t)(i % d.nx);
+        const int64_t r = i / d.nx;
+        const int32_t y = (int32_t)(r % d.ny), z = (int32_t)(r / d.ny);
+        dense[i] = slot_off[bricks[brick_byte(x, y, z)]];
+    }
+}
+
+__global__ void fill_u8_kernel(uint8_t* p, uint64_t n, uint8_t v)
+{
+    const uint64_t n16 = n / 16;
+    uint4 vv;
+    const uint32_t w = (uint32_t)v * 0x01010101u;
+    vv.x = vv.y = vv.z = vv.w = w;
+    uint4* p16 = (uint4*)p;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        p16[i] = vv;
+    for (uint64_t i = n16 * 16 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static unsigned grid_for(uint64_t n)
+{
+    uint64_t b = (n + 255) / 256;
+    if (b > 65536u * 8u) b = 65536u * 8u;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+hipError_t launch_brick_from_offsets(const BrickDims& d, const int32_t* dense, const uint8_t* lut, int32_t lut_n,
+                                     uint8_t* bricks, uint32_t* bad, hipStream_t s)
+{
+    brick_from_offsets_kernel<<<grid_for((uint64_t)d.nx * d.ny * d.nz), 256, 0, s>>>(d, dense, lut, lut_n, bricks, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_from_ids(const BrickDims& d, const uint8_t* dense, const uint8_t* remap, uint8_t* bricks,
+                                 hipStream_t s)
+{
+    brick_from_ids_kernel<<<grid_for((uint64_t)d.nx * d.ny * d.nz), 256, 0, s>>>(d, dense, remap, bricks);
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_bits(const BrickDims& d, const uint8_t* bricks, uint32_t* bits, hipStream_t s)
+{
+    brick_bits_kernel<<<grid_for(d.n_bricks), 256, 0, s>>>(d, bricks, bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_from_bricks(const BrickDims& d, const uint8_t* bricks, const int32_t* slot_off, int32_t* dense,
+                                    hipStream_t s)
+{
+    dense_from_bricks_kernel<<<grid_for((uint64_t)d.nx * d.ny * d.nz), 256, 0, s>>>(d, bricks, slot_off, dense);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s)
+{
+    fill_u8_kernel<<<grid_for(n / 16 + 1), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s)
+{
+    fill_u32_kernel<<<grid_for(n), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+/* ---- voxelizer ---------------------------------------------------------------------------- */
+
+/* int(float) with x86 cvttss2si semantics (non-finite or out of range ->
+ * INT_MIN), the value the reference's llvmpipe path produces for the
+ * z-range of degenerate triangles; sums wrap as 32-bit integers. */
+__device__ __forceinline__ int32_t cvt_i32(float f)
+{
+    if (!(f >= -2147483648.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)f;
+}
+__device__ __forceinline__ int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
+struct V2 { float x, y; };
+__device__ __forceinline__ float dot2(V2 a, float bx, float by) { return a.x * bx + a.y * by; }
+__device__ __forceinline__ bool finite3(f3 v)
+{
+    return vt_absf(v.x) < vt_inff() && vt_absf(v.y) < vt_inff() && vt_absf(v.z) < vt_inff();
+}
+
+__global__ __launch_bounds__(256) void voxelize_kernel(VoxArgs a)
+{
+    const int64_t tri = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tri >= a.ntri) return;
+    const float rf[3] = { (float)a.d.nx, (float)a.d.ny, (float)a.d.nz };
+    const int32_t res[3] = { a.d.nx, a.d.ny, a.d.nz };
+    f3 v[3];
+    bool finite = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float* p = a.verts + 3 * (size_t)a.idx[3 * tri + k];
+        float ls[4];
+        mat4_mul(a.M, p[0], p[1], p[2], 1.0f, ls);
+        v[k] = mk(ls[0] * rf[0], ls[1] * rf[1], ls[2] * rf[2]);
+        finite = finite && finite3(v[k]);
+    }
+    if (!finite) return;
+    f3 n = cross(v[1] - v[0], v[2] - v[1]);
+    const float anx = vt_absf(n.x), any = vt_absf(n.y), anz = vt_absf(n.z);
+    int lut;
+    if (anx >= any && anx >= anz) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[k] = mk(v[k].y, v[k].z, v[k].x);
+        n = mk(n.y, n.z, n.x);
+        lut = 0;
+    } else if (any >= anx && any >= anz) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[k] = mk(v[k].z, v[k].x, v[k].y);
+        n = mk(n.z, n.x, n.y);
+        lut = 1;
+    } else {
+        lut = 2;
+    }
+    const float amn[3] = { vt_minf(vt_minf(v[0].x, v[1].x), v[2].x), vt_minf(vt_minf(v[0].y, v[1].y), v[2].y),
+                           vt_minf(vt_minf(v[0].z, v[1].z), v[2].z) };
+    const float amx[3] = { vt_maxf(vt_maxf(v[0].x, v[1].x), v[2].x), vt_maxf(vt_maxf(v[0].y, v[1].y), v[2].y),
+                           vt_maxf(vt_maxf(v[0].z, v[1].z), v[2].z) };
+    int32_t mn[3], mx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = (int32_t)vt_minf(vt_maxf(floorf(amn[k]), 0.0f), rf[k]);
+        mx[k] = (int32_t)vt_minf(vt_maxf(ceilf(amx[k]), 0.0f), rf[k]);
+    }
+    const f3 ee[3] = { v[1] - v[0], v[2] - v[1], v[0] - v[2] };
+    V2 nxy[3], nyz[3], nzx[3];
+    float dxy[3], dyz[3], dzx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const f3 ek = ee[k];
+        nxy[k] = (n.z >= 0.0f) ? V2{ -ek.y, ek.x } : V2{ ek.y, -ek.x };
+        nyz[k] = (n.x >= 0.0f) ? V2{ -ek.z, ek.y } : V2{ ek.z, -ek.y };
+        nzx[k] = (n.y >= 0.0f) ? V2{ -ek.x, ek.z } : V2{ ek.x, -ek.z };
+        const f3 vk = v[k];
+        if (a.mode == 0) {
+            dxy[k] = dot2(nxy[k], 0.5f - vk.x, 0.5f - vk.y) + 0.5f * vt_maxf(vt_absf(nxy[k].x), vt_absf(nxy[k].y));
+            dyz[k] = dot2(nyz[k], 0.5f - vk.y, 0.5f - vk.z) + 0.5f * vt_maxf(vt_absf(nyz[k].x), vt_absf(nyz[k].y));
+            dzx[k] = dot2(nzx[k], 0.5f - vk.z, 0.5f - vk.x) + 0.5f * vt_maxf(vt_absf(nzx[k].x), vt_absf(nzx[k].y));
+        } else {
+            dxy[k] = (-dot2(nxy[k], vk.x, vk.y) + vt_maxf(0.0f, nxy[k].x)) + vt_maxf(0.0f, nxy[k].y);
+            dyz[k] = (-dot2(nyz[k], vk.y, vk.z) + vt_maxf(0.0f, nyz[k].x)) + vt_maxf(0.0f, nyz[k].y);
+            dzx[k] = (-dot2(nzx[k], vk.z, vk.x) + vt_maxf(0.0f, nzx[k].x)) + vt_maxf(0.0f, nzx[k].y);
+        }
+    }
+    const f3 np = (n.z < 0.0f) ? -n : n;
+    const float dTri = dot(np, v[0]);
+    const float dMin = (a.mode == 0) ? dTri - (np.x * 0.5f + np.y * 0.5f)
+                                     : (dTri - vt_maxf(np.x, 0.0f)) - vt_maxf(np.y, 0.0f);
+    const float dMax = (a.mode == 0) ? dMin : (dTri - vt_minf(np.x, 0.0f)) - vt_minf(np.y, 0.0f);
+    const float nzInv = 1.0f / np.z;
+    for (int32_t x = mn[0]; x < mx[0]; ++x) {
+        const float fx = (float)x;
+        for (int32_t y = mn[1]; y < mx[1]; ++y) {
+            const float fy = (float)y;
+            const float d0 = dxy[0] + dot2(nxy[0], fx, fy);
+            const float d1 = dxy[1] + dot2(nxy[1], fx, fy);
+            const float d2 = dxy[2] + dot2(nxy[2], fx, fy);
+            if (!(d0 >= 0.0f && d1 >= 0.0f && d2 >= 0.0f)) continue;
+            const float dnp = np.x * fx + np.y * fy;
+            const float zMinInt = (-dnp + dMin) * nzInv;
+            const float zMaxInt = (a.mode == 0) ? zMinInt : (-dnp + dMax) * nzInv;
+            const float zMinFloor = floorf(zMinInt), zMaxCeil = ceilf(zMaxInt);
+            int32_t zMin = wrap_add(cvt_i32(zMinFloor), -(int32_t)(zMinFloor == zMinInt));
+            int32_t zMax = wrap_add(cvt_i32(zMaxCeil), (int32_t)(zMaxCeil == zMaxInt));
+            if (zMin < mn[2]) zMin = mn[2];
+            if (zMax > mx[2]) zMax = mx[2];
+            for (int32_t z = zMin; z < zMax; ++z) {
+                const float fz = (float)z;
+                bool ok = true;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    ok = ok && (dyz[k] + dot2(nyz[k], fy, fz) >= 0.0f);
+                    ok = ok && (dzx[k] + dot2(nzx[k], fz, fx) >= 0.0f);
+                }
+                if (!ok) continue;
+                int32_t c0, c1, c2;
+                if (lut == 0) { c0 = z; c1 = x; c2 = y; }
+                else if (lut == 1) { c0 = y; c1 = z; c2 = x; }
+                else { c0 = x; c1 = y; c2 = z; }
+                if ((uint32_t)c0 >= (uint32_t)res[0] || (uint32_t)c1 >= (uint32_t)res[1] ||
+                    (uint32_t)c2 >= (uint32_t)res[2])
+                    continue;
+                if (a.bricks) a.bricks[brick_byte(c0, c1, c2)] = a.slot;
+                if (a.occ) a.occ[(size_t)c0 + (size_t)c1 * res[0] + (size_t)c2 * (size_t)res[0] * res[1]] = 1u;
+            }
+        }
+    }
+}
+
+hipError_t launch_voxelize(const VoxArgs& a, hipStream_t s)
+{
+    if (a.ntri <= 0) return hipSuccess;
+    voxelize_kernel<<<(unsigned)((a.ntri + 255) / 256), 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace vt
