@@ -12,8 +12,8 @@ The roofline object prices the render kernel at its algorithmic HBM bytes
 (1 B per DDA voxel visit, 32 B per surface interaction, 32 B per sample
 accumulator read-modify-write, binary-search and texel bytes per image
 environment lookup) over the kernel's HIP-event time.  The CPU baseline is
-the C oracle (OpenMP over the host cores) on a bounded row band of the same
-workload, one pass.
+the C oracle (OpenMP over the host cores) on the same frame for as many of
+the step's passes as fit a ~12 s budget.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -47,8 +47,9 @@ def algorithmic_bytes(st: dict, cu_w: int, cv_n: int) -> int:
             + st["env_evals"] * eval_b)
 
 
-def cpu_baseline(wl, budget_s: float = 15.0):
-    """Oracle on a row band of the same workload (1 pass), all host cores."""
+def cpu_baseline(wl, budget_s: float = 12.0):
+    """Oracle on the same workload, all host cores: the whole frame for as
+    many of the step's passes as fit the time budget (at least one)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     oracle.build()
@@ -62,19 +63,17 @@ def cpu_baseline(wl, budget_s: float = 15.0):
     cu, cv, integ = wl.cdf
     o.set_environment(wl.env_top, wl.env_bottom, wl.env_image, cu, cv, integ, wl.env_rotation)
     o.set_settings(max_bounces=wl.bounces)
-    # calibrate on 8 rows through the image centre, then size the band to the budget
-    y0 = wl.height // 2 - 4
     t = time.perf_counter()
-    _, cnt = o.render(wl.width, wl.height, wl.seed, 0, 1, y0=y0, y1=y0 + 8, threads=cores)
-    dt = time.perf_counter() - t
-    rows = int(max(8, min(wl.height, 8 * budget_s / max(dt, 1e-3))))
-    y0 = max(0, wl.height // 2 - rows // 2)
+    acc, cnt = o.render(wl.width, wl.height, wl.seed, 0, 1, threads=cores)
+    dt1 = time.perf_counter() - t
+    passes = int(max(1, min(wl.spp - 1, budget_s / max(dt1, 1e-3))))
     t = time.perf_counter()
-    _, cnt = o.render(wl.width, wl.height, wl.seed, 0, 1, y0=y0, y1=y0 + rows, threads=cores)
-    dt = time.perf_counter() - t
-    return {"value": cnt["rays"] / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"rows {y0}-{y0 + rows} of {wl.width}x{wl.height}, 1 pass of 16, {cnt['rays']} rays "
-                      f"in {dt:.2f} s (C oracle, OpenMP)"}
+    _, cnt2 = o.render(wl.width, wl.height, wl.seed, 1, passes, accum=acc, threads=cores)
+    dt = time.perf_counter() - t + dt1
+    rays = cnt["rays"] + cnt2["rays"]
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"full {wl.width}x{wl.height} frame, passes 0-{passes} of the step's {wl.spp}, {rays} rays in "
+                      f"{dt:.2f} s (C oracle, OpenMP)"}
 
 
 def main():
@@ -83,7 +82,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--mode", type=int, default=1, help="0 megakernel, 1 wavefront (default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the path shards by "
@@ -95,6 +95,7 @@ def main():
     wl = workloads.full()
     ctx = api.Context(0)
     workloads.load(ctx, wl)
+    ctx.set_render_mode(args.mode)
     cu_w, cv_n = wl.cdf[0].shape[1], len(wl.cdf[1])
 
     # statistics launch (untimed): exact work of one step
@@ -145,6 +146,7 @@ def main():
                                "16 spp, 8 bounces (plain tracer)",
                    "grid": 256, "width": wl.width, "height": wl.height, "spp": wl.spp, "bounces": wl.bounces,
                    "emissive_voxels": int(len(wl.scene.emissive)), "rays_per_step": int(rays),
+                   "render_mode": ["megakernel", "wavefront"][args.mode],
                    "kernel_ms": round(k_ms, 3), "counters": {k: int(st[k]) for k in
                                                              ("visits", "interactions", "samples", "env_samples",
                                                               "env_evals")}},

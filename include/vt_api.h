@@ -134,6 +134,12 @@ int vt_set_settings(vt_context* ctx, const vt_settings* st);
 int vt_set_frame(vt_context* ctx, int32_t width, int32_t height, int32_t fp64);
 int vt_reset_accum(vt_context* ctx);
 int vt_set_stats(vt_context* ctx, int32_t enabled);
+/* Kernel organisation of vt_render (results are identical):
+ *   0 = persistent megakernel (one lane renders a pixel's passes start to end)
+ *   1 = wavefront (generate / traverse / shade kernels over ray queues; at
+ *       most max_paths paths in flight, 0 = default 4M).
+ * Default 1. */
+int vt_set_render_mode(vt_context* ctx, int32_t mode, uint64_t max_paths);
 
 /* ---- rendering ----------------------------------------------------------
  * Render `count` progressive passes: pass k uses accumulation index
@@ -189,6 +195,11 @@ int32_t vt_prune_emissive(const int32_t* material_offsets, int32_t nx, int32_t n
  * checking that device and host arithmetic agree bit for bit.  fn: 0 sqrt,
  * 1 a/b, 2 sin, 3 cos, 4 acos, 5 atan2(a,b), 6 log2, 7 exp2, 8 pow(a,b). */
 int vt_device_math(vt_context* ctx, int32_t fn, int32_t n, const float* a, const float* b, float* out);
+
+/* Streaming copy of `bytes` bytes between two device buffers, `iters`
+ * times, with a plain vectorized copy kernel; returns the mean kernel time.
+ * A known-size reference for hardware byte counters and the HBM roofline. */
+int vt_copy_probe(vt_context* ctx, uint64_t bytes, int32_t iters, double* kernel_ms);
 
 #ifdef __cplusplus
 }

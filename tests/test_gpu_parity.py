@@ -106,11 +106,13 @@ def test_trace_rays_injected_bit_exact(ctx, orc):
     assert not bad.any(), f"{bad.sum()} rays differ: {got[bad][:3]} vs {ref[bad][:3]}"
 
 
-@pytest.mark.parametrize("angle,int32_grid", [(0, False), (3, True), (6, False)])
-def test_render_small_scene_bit_exact(ctx, orc, angle, int32_grid):
+@pytest.mark.parametrize("angle,int32_grid,mode,max_paths", [(0, False, 1, 0), (3, True, 0, 0), (6, False, 1, 70001),
+                                                              (5, False, 0, 0)])
+def test_render_small_scene_bit_exact(ctx, orc, angle, int32_grid, mode, max_paths):
     from voxeltoy_amd import workloads
     wl = workloads.small(angle)
     workloads.load(ctx, wl, fp64=True, int32_grid=int32_grid)
+    ctx.set_render_mode(mode, max_paths)
     ctx.set_stats(True)
     ctx.render(wl.seed, 0, wl.spp)
     got = ctx.read_accum()
@@ -125,11 +127,14 @@ def test_render_small_scene_bit_exact(ctx, orc, angle, int32_grid):
     assert np.array_equal(s64, ref_sum)
     for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
         assert stats[k] == cnt[k], k
+    ctx.set_render_mode(1, 0)
 
 
-def test_render_progressive_passes_equal_single_launch(ctx, orc):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_render_progressive_passes_equal_single_launch(ctx, orc, mode):
     """count passes in one launch == count launches of one pass (Renderer::render)."""
     from voxeltoy_amd import workloads
+    ctx.set_render_mode(mode, 0)
     wl = workloads.small(1, width=96, height=80)
     workloads.load(ctx, wl)
     ctx.render(wl.seed, 0, 3)
@@ -139,6 +144,7 @@ def test_render_progressive_passes_equal_single_launch(ctx, orc):
         ctx.render(wl.seed, n, 1)
     b = ctx.read_accum()
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    ctx.set_render_mode(1, 0)
 
 
 def small_full_workload(lens_model=1, w=96, h=72):
@@ -157,12 +163,13 @@ def small_full_workload(lens_model=1, w=96, h=72):
                               bmax=bmax, env_rotation=0.3)
 
 
-@pytest.mark.parametrize("lens_model", [0, 1, 2])
-def test_render_full_shading_bit_exact(ctx, orc, lens_model):
+@pytest.mark.parametrize("lens_model,mode", [(0, 1), (1, 1), (2, 1), (1, 0)])
+def test_render_full_shading_bit_exact(ctx, orc, lens_model, mode):
     from voxeltoy_amd import workloads
     wl = small_full_workload(lens_model)
     assert len(wl.scene.emissive) > 0
     workloads.load(ctx, wl)
+    ctx.set_render_mode(mode, 0)
     ctx.set_stats(True)
     ctx.render(wl.seed, 0, wl.spp)
     got = ctx.read_accum()
@@ -175,6 +182,7 @@ def test_render_full_shading_bit_exact(ctx, orc, lens_model):
     assert cnt["env_samples"] > 0 and cnt["interactions"] > 0
     for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
         assert stats[k] == cnt[k], k
+    ctx.set_render_mode(1, 0)
 
 
 def test_render_settings_variants_bit_exact(ctx, orc):

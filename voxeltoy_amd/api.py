@@ -71,6 +71,7 @@ SIGNATURES = {
     "vt_set_frame": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "vt_reset_accum": (C.c_int, [C.c_void_p]),
     "vt_set_stats": (C.c_int, [C.c_void_p, C.c_int32]),
+    "vt_set_render_mode": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64]),
     "vt_render": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
     "vt_read_accum": (C.c_int, [C.c_void_p, f32p]),
     "vt_read_sum64": (C.c_int, [C.c_void_p, f64p]),
@@ -85,6 +86,7 @@ SIGNATURES = {
                                    C.c_void_p, C.POINTER(C.c_int32), C.c_void_p, C.POINTER(C.c_float)]),
     "vt_prune_emissive": (C.c_int32, [i32p, C.c_int32, C.c_int32, C.c_int32, i32p, C.c_int32]),
     "vt_device_math": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f32p, f32p, f32p]),
+    "vt_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
 }
 
 
@@ -299,6 +301,10 @@ class Context:
     def set_stats(self, enabled: bool):
         _check(lib().vt_set_stats(self._c, 1 if enabled else 0))
 
+    def set_render_mode(self, mode: int, max_paths: int = 0):
+        """0 = megakernel, 1 = wavefront (default); both give identical images."""
+        _check(lib().vt_set_render_mode(self._c, mode, max_paths))
+
     # -- rendering ----------------------------------------------------------
     def render(self, seed: int, n0: int, count: int, rng_frame_max: int = 0xFFFFFFFF):
         _check(lib().vt_render(self._c, seed, n0, count, rng_frame_max))
@@ -339,6 +345,12 @@ class Context:
         _check(lib().vt_voxelize(self._c, v, len(v) // 3, t, len(t) // 3,
                                  np.ascontiguousarray(M, np.float32).reshape(-1), mode, material_offset, _ptr(occ)))
         return occ
+
+    def copy_probe(self, nbytes: int, iters: int = 10) -> float:
+        """Mean kernel ms of a streaming device copy of nbytes."""
+        ms = C.c_double()
+        _check(lib().vt_copy_probe(self._c, nbytes, iters, C.byref(ms)))
+        return ms.value
 
     def device_math(self, fn: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         a = np.ascontiguousarray(a, np.float32)

@@ -57,6 +57,7 @@ struct vt_context {
     vt::BrickDims dims{};
     float bmin[3] = { 0, 0, 0 }, bmax[3] = { 0, 0, 0 }, vs[3] = { 0, 0, 0 };
     uint8_t* d_bricks = nullptr;
+    uint64_t* d_occ64 = nullptr;
     uint32_t* d_bits = nullptr;
     uint32_t bit_words = 0;
     int32_t* d_emissive = nullptr;
@@ -79,6 +80,11 @@ struct vt_context {
     int32_t W = 0, H = 0;
     float4* d_accum = nullptr;
     double* d_sum64 = nullptr;
+
+    int render_mode = 1;
+    uint64_t max_paths = 4u << 20;
+    vt::WfPaths wf{};
+    uint64_t wf_alloc = 0;           /* slots allocated in wf */
 
     uint32_t* d_tile = nullptr;
     unsigned long long* d_stats = nullptr;
@@ -119,6 +125,15 @@ int upload_records(vt_context* c)
     return VT_OK;
 }
 
+/* Rebuild both occupancy levels from the slot bytes. */
+int refresh_occupancy(vt_context* c)
+{
+    VT_HIP(vt::launch_fill_u32(c->d_bits, c->bit_words, 0u, c->stream));
+    VT_HIP(vt::launch_brick_bits(c->dims, c->d_bricks, c->d_bits, c->stream));
+    VT_HIP(vt::launch_block_masks(c->dims, c->d_bricks, c->d_occ64, c->stream));
+    return VT_OK;
+}
+
 vt::BrickDims make_dims(int32_t nx, int32_t ny, int32_t nz)
 {
     vt::BrickDims d;
@@ -147,13 +162,13 @@ void reference_bounds(int32_t nx, int32_t ny, int32_t nz, float* bmin, float* bm
 int common_volume_setup(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const float* mat_data, int32_t mat_size,
                         const float* bmin, const float* bmax)
 {
-    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > 8192 || ny > 8192 || nz > 8192)
-        return fail(VT_ERR_ARG, "volume resolution must be in [1, 8192] per axis");
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > 4096 || ny > 4096 || nz > 4096)
+        return fail(VT_ERR_ARG, "volume resolution must be in [1, 4096] per axis");
     if (mat_size < 0 || (mat_size > 0 && !mat_data)) return fail(VT_ERR_ARG, "bad material data");
     const vt::BrickDims d = make_dims(nx, ny, nz);
-    if (d.bdim > 1024) return fail(VT_ERR_LIMIT, "grid too large for 10-bit brick Morton codes");
     c->has_volume = false;
     dfree(c->d_bricks);
+    dfree(c->d_occ64);
     dfree(c->d_bits);
     dfree(c->d_emissive);
     c->dims = d;
@@ -168,6 +183,7 @@ int common_volume_setup(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const
     for (int i = 0; i < 256; ++i) c->slot_off[i] = -1;
     c->n_slots = 0;
     VT_HIP(hipMalloc(&c->d_bricks, d.n_bricks * 512u));
+    VT_HIP(hipMalloc(&c->d_occ64, d.n_bricks * 64u));
     c->bit_words = (uint32_t)((d.n_bricks + 31) / 32);
     VT_HIP(hipMalloc(&c->d_bits, (size_t)c->bit_words * 4u));
     VT_HIP(vt::launch_fill_u8(c->d_bricks, d.n_bricks * 512u, 0, c->stream));
@@ -184,9 +200,9 @@ int finish_volume(vt_context* c, const int32_t* emissive, int32_t n_emissive, co
         VT_HIP(hipMalloc(&c->d_emissive, (size_t)n_emissive * 4u));
         VT_HIP(hipMemcpyAsync(c->d_emissive, emissive, (size_t)n_emissive * 4u, hipMemcpyHostToDevice, c->stream));
     }
-    VT_HIP(vt::launch_fill_u32(c->d_bits, c->bit_words, 0u, c->stream));
-    VT_HIP(vt::launch_brick_bits(c->dims, c->d_bricks, c->d_bits, c->stream));
-    int r = upload_records(c);
+    int r = refresh_occupancy(c);
+    if (r != VT_OK) return r;
+    r = upload_records(c);
     if (r != VT_OK) return r;
     VT_HIP(hipStreamSynchronize(c->stream));
     c->has_volume = true;
@@ -223,6 +239,7 @@ vt::GridView grid_view(const vt_context* c)
         g.ext[k] = 1.0f / (c->bmax[k] - c->bmin[k]);
     }
     g.bricks = c->d_bricks;
+    g.occ64 = c->d_occ64;
     g.bits = c->d_bits;
     g.bit_words = c->bit_words;
     g.n_emissive = c->n_emissive;
@@ -270,6 +287,36 @@ vt::TraceArgs trace_args(const vt_context* c)
     return a;
 }
 
+void free_wavefront(vt_context* c)
+{
+    vt::WfPaths& P = c->wf;
+    dfree(P.o); dfree(P.d); dfree(P.bo); dfree(P.thr); dfree(P.rad); dfree(P.dir); dfree(P.fac); dfree(P.nd);
+    dfree(P.tgt); dfree(P.meta); dfree(P.rng); dfree(P.hit); dfree(P.smp);
+    dfree(P.qa0); dfree(P.qb0); dfree(P.qa1); dfree(P.qb1); dfree(P.cnt);
+    P.cap = 0;
+    c->wf_alloc = 0;
+}
+
+/* Path storage for `slots` paths (reallocated only when it grows). */
+int ensure_wavefront(vt_context* c, uint64_t slots)
+{
+    if (c->wf_alloc >= slots) { c->wf.cap = slots; return VT_OK; }
+    free_wavefront(c);
+    vt::WfPaths& P = c->wf;
+    const size_t n = (size_t)slots;
+    VT_HIP(hipMalloc(&P.o, n * 16)); VT_HIP(hipMalloc(&P.d, n * 16)); VT_HIP(hipMalloc(&P.bo, n * 16));
+    VT_HIP(hipMalloc(&P.thr, n * 16)); VT_HIP(hipMalloc(&P.rad, n * 16)); VT_HIP(hipMalloc(&P.dir, n * 16));
+    VT_HIP(hipMalloc(&P.fac, n * 16)); VT_HIP(hipMalloc(&P.nd, n * 16)); VT_HIP(hipMalloc(&P.tgt, n * 16));
+    VT_HIP(hipMalloc(&P.meta, n * 16)); VT_HIP(hipMalloc(&P.rng, n * 8)); VT_HIP(hipMalloc(&P.hit, n * 16));
+    VT_HIP(hipMalloc(&P.smp, n * 16));
+    VT_HIP(hipMalloc(&P.qa0, n * 16)); VT_HIP(hipMalloc(&P.qb0, n * 16));
+    VT_HIP(hipMalloc(&P.qa1, n * 16)); VT_HIP(hipMalloc(&P.qb1, n * 16));
+    VT_HIP(hipMalloc(&P.cnt, 16));
+    P.cap = slots;
+    c->wf_alloc = slots;
+    return VT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -315,9 +362,10 @@ void vt_destroy(vt_context* c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    dfree(c->d_bricks); dfree(c->d_bits); dfree(c->d_emissive); dfree(c->d_recs);
+    dfree(c->d_bricks); dfree(c->d_occ64); dfree(c->d_bits); dfree(c->d_emissive); dfree(c->d_recs);
     dfree(c->d_img); dfree(c->d_cdfu); dfree(c->d_cdfv);
     dfree(c->d_accum); dfree(c->d_sum64); dfree(c->d_tile); dfree(c->d_stats);
+    free_wavefront(c);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -543,6 +591,16 @@ int vt_set_stats(vt_context* c, int32_t enabled)
     return VT_OK;
 }
 
+int vt_set_render_mode(vt_context* c, int32_t mode, uint64_t max_paths)
+{
+    if (!c) return fail(VT_ERR_ARG, "null context");
+    if (mode != 0 && mode != 1) return fail(VT_ERR_ARG, "mode must be 0 (megakernel) or 1 (wavefront)");
+    if (max_paths > (1ull << 31)) return fail(VT_ERR_LIMIT, "max_paths above 2^31");
+    c->render_mode = mode;
+    c->max_paths = max_paths ? max_paths : (4ull << 20);
+    return VT_OK;
+}
+
 int vt_render(vt_context* c, uint64_t seed, uint32_t n0, uint32_t count, uint32_t rng_frame_max)
 {
     if (!c) return fail(VT_ERR_ARG, "null context");
@@ -553,12 +611,21 @@ int vt_render(vt_context* c, uint64_t seed, uint32_t n0, uint32_t count, uint32_
     vt::TraceArgs a = trace_args(c);
     a.seed = seed; a.n0 = n0; a.count = count; a.rng_frame_max = rng_frame_max;
     a.stats = c->stats_enabled ? c->d_stats : nullptr;
-    const int per_cu = vt::render_blocks_per_cu(a, c->stats_enabled);
-    const int grid = std::max(1, c->cus * per_cu);
     VT_HIP(vt::launch_reset_counter(c->d_tile, a.stats, c->stream));
-    VT_HIP(hipEventRecord(c->ev0, c->stream));
-    VT_HIP(vt::launch_render(a, grid, c->stream, c->stats_enabled));
-    VT_HIP(hipEventRecord(c->ev1, c->stream));
+    if (c->render_mode == 1) {
+        const uint64_t total = (uint64_t)c->W * c->H * count;
+        int r = ensure_wavefront(c, std::min<uint64_t>(total, c->max_paths));
+        if (r != VT_OK) return r;
+        VT_HIP(hipEventRecord(c->ev0, c->stream));
+        VT_HIP(vt::launch_render_wavefront(a, c->wf, c->cus, c->stream, c->stats_enabled));
+        VT_HIP(hipEventRecord(c->ev1, c->stream));
+    } else {
+        const int per_cu = vt::render_blocks_per_cu(a, c->stats_enabled);
+        const int grid = std::max(1, c->cus * per_cu);
+        VT_HIP(hipEventRecord(c->ev0, c->stream));
+        VT_HIP(vt::launch_render(a, grid, c->stream, c->stats_enabled));
+        VT_HIP(hipEventRecord(c->ev1, c->stream));
+    }
     c->timed = true;
     c->stats_valid = c->stats_enabled;
     return VT_OK;
@@ -689,8 +756,10 @@ int vt_voxelize(vt_context* c, const float* verts, int64_t nv, const uint32_t* i
     }
     a.verts = d_v; a.idx = d_i; a.occ = d_occ;
     VT_HIP(vt::launch_voxelize(a, c->stream));
-    VT_HIP(vt::launch_fill_u32(c->d_bits, c->bit_words, 0u, c->stream));
-    VT_HIP(vt::launch_brick_bits(c->dims, c->d_bricks, c->d_bits, c->stream));
+    {
+        int r = refresh_occupancy(c);
+        if (r != VT_OK) return r;
+    }
     if (occupancy) VT_HIP(hipMemcpyAsync(occupancy, d_occ, (size_t)nvox, hipMemcpyDeviceToHost, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
     dfree(d_v); dfree(d_i); dfree(d_occ);
@@ -745,6 +814,27 @@ int vt_device_math(vt_context* c, int32_t fn, int32_t n, const float* a, const f
     VT_HIP(hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
     dfree(da); dfree(db); dfree(dout);
+    return VT_OK;
+}
+
+int vt_copy_probe(vt_context* c, uint64_t bytes, int32_t iters, double* kernel_ms)
+{
+    if (!c || bytes < 16 || bytes % 16 || iters < 1 || !kernel_ms) return fail(VT_ERR_ARG, "bad argument");
+    VT_HIP(hipSetDevice(c->device));
+    uint8_t *a = nullptr, *b = nullptr;
+    VT_HIP(hipMalloc(&a, bytes));
+    VT_HIP(hipMalloc(&b, bytes));
+    VT_HIP(vt::launch_fill_u8(a, bytes, 1, c->stream));
+    VT_HIP(vt::launch_copy(a, b, bytes, c->stream));
+    VT_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int32_t i = 0; i < iters; ++i) VT_HIP(vt::launch_copy(a, b, bytes, c->stream));
+    VT_HIP(hipEventRecord(c->ev1, c->stream));
+    VT_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    VT_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->timed = false;
+    *kernel_ms = ms / iters;
+    dfree(a); dfree(b);
     return VT_OK;
 }
 

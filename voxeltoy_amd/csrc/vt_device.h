@@ -10,8 +10,14 @@
  * brick (same Morton index) lets the traversal skip the voxel loads of empty
  * bricks; the bit array is staged in LDS when it fits.
  *
+ * A second occupancy level holds one 64-bit mask per 4x4x4 block (in-brick
+ * Morton order makes every 4x4x4 block one contiguous 64-byte run; the mask
+ * bit of local voxel (x, y, z) is x + 4y + 16z).  The traversal tests voxels
+ * against the mask it loaded on entering the block (one 8-byte load per
+ * block, none in empty bricks) and reads the slot byte only at the hit.
+ *
  * The traversal replays the reference 3D-DDA (src/shaders/shared/dda.h:7-100)
- * step by step: `dis` is a running FP32 sum, so empty bricks skip the loads
+ * step by step: `dis` is a running FP32 sum, so empty space skips the loads
  * but not the compare/add sequence, and hit voxel, face and step count are
  * identical to the reference arithmetic.  All float expressions keep the
  * reference operand order and are compiled with -ffp-contract=off.
@@ -65,6 +71,7 @@ struct GridView {
     float rx, ry, rz;           /* resolution as float */
     float bmin[3], bmax[3], vs[3], ext[3];
     const uint8_t* bricks;      /* Morton bricks of 512 bytes */
+    const uint64_t* occ64;      /* per 4x4x4 block voxel-occupancy masks */
     const uint32_t* bits;       /* brick occupancy bits (global copy) */
     uint32_t bit_words;
     int32_t n_emissive;
@@ -120,6 +127,11 @@ __device__ __forceinline__ uint32_t local_morton(uint32_t x, uint32_t y, uint32_
 {
     return spread3_3(x & 7u) | (spread3_3(y & 7u) << 1) | (spread3_3(z & 7u) << 2);
 }
+/* Bit of voxel (x, y, z) in its 4x4x4 block mask. */
+__device__ __forceinline__ uint32_t block_bit(uint32_t x, uint32_t y, uint32_t z)
+{
+    return (x & 3u) | ((y & 3u) << 2) | ((z & 3u) << 4);
+}
 
 /* ---- rayAABBIntersection (aabb.h:1-32) -------------------------------------------- */
 __device__ __forceinline__ float aabb(f3 o, f3 d, f3 bmin, f3 bmax)
@@ -170,35 +182,132 @@ __device__ __forceinline__ int dda(const GridView& g, const uint32_t* __restrict
     int32_t x = (int32_t)fx, y = (int32_t)fy, z = (int32_t)fz;
     int32_t steps = 0;
     int hit = 0;
-    uint32_t cb = 0xffffffffu, bm = 0;
-    int32_t cbx = -1, cby = -1, cbz = -1;
-    uint32_t occ = 0;
-    uint32_t slot = 0;
+    uint32_t block_key = 0xffffffffu, brick_key = 0xffffffffu;
+    uint32_t bm = 0, brick_occ = 0;
+    uint64_t mask = 0;
     while (steps < g.max_steps) {
         if ((uint32_t)x >= (uint32_t)g.nx || (uint32_t)y >= (uint32_t)g.ny || (uint32_t)z >= (uint32_t)g.nz) break;
         if (COUNT) visits++;
-        const int32_t bx = x >> 3, by = y >> 3, bz = z >> 3;
-        if (bx != cbx || by != cby || bz != cbz) {
-            cbx = bx; cby = by; cbz = bz;
-            bm = brick_morton((uint32_t)bx, (uint32_t)by, (uint32_t)bz);
-            occ = (bits[bm >> 5] >> (bm & 31u)) & 1u;
-            cb = bm;
+        const uint32_t bk = ((uint32_t)x >> 2) | (((uint32_t)y >> 2) << 11) | (((uint32_t)z >> 2) << 22);
+        if (bk != block_key) {
+            block_key = bk;
+            const uint32_t rk = (bk >> 1) & ((0x3ffu) | (0x3ffu << 11) | (0x3ffu << 22));
+            if (rk != brick_key) {
+                brick_key = rk;
+                bm = brick_morton((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3);
+                brick_occ = (bits[bm >> 5] >> (bm & 31u)) & 1u;
+            }
+            const uint32_t sb = (((uint32_t)x >> 2) & 1u) | ((((uint32_t)y >> 2) & 1u) << 1) |
+                                ((((uint32_t)z >> 2) & 1u) << 2);
+            mask = brick_occ ? g.occ64[(size_t)bm * 8u + sb] : 0ull;
         }
-        if (occ) {
-            const uint32_t v = g.bricks[(size_t)cb * 512u + local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z)];
-            if (v) { hit = 1; slot = v; break; }
-        }
+        if ((mask >> block_bit((uint32_t)x, (uint32_t)y, (uint32_t)z)) & 1ull) { hit = 1; break; }
         const bool mx = !(disy < disx) && !(disz < disx);
         const bool my = !(disx < disy) && !(disz < disy);
         const bool mz = !(disy < disz) && !(disx < disz);
-        if (mx) { disx = disx + ax; x += stx; }
-        if (my) { disy = disy + ay; y += sty; }
-        if (mz) { disz = disz + az; z += stz; }
+        /* select form of the masked add: adding +0 instead of skipping can
+         * only change the sign of a zero dis, which nothing observes */
+        disx = disx + (mx ? ax : 0.0f);
+        disy = disy + (my ? ay : 0.0f);
+        disz = disz + (mz ? az : 0.0f);
+        x += mx ? stx : 0;
+        y += my ? sty : 0;
+        z += mz ? stz : 0;
         steps++;
     }
-    h.x = x; h.y = y; h.z = z; h.steps = steps; h.slot = slot;
+    h.x = x; h.y = y; h.z = z; h.steps = steps;
+    h.slot = hit ? g.bricks[(size_t)bm * 512u + local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z)] : 0u;
     return hit;
 }
+
+/* The same march as resumable state, for traversal kernels whose lanes
+ * take a new ray as soon as theirs finishes.  init() returns false when the
+ * start point lies outside the grid (steps = -1, nothing else written);
+ * advance() runs at most `budget` steps and returns 0 while the march goes
+ * on, 1 on a hit, 2 when it left the grid or ran out of steps. */
+struct DdaState {
+    int32_t x, y, z, steps;
+    float disx, disy, disz, ax, ay, az;
+    int32_t stx, sty, stz;
+    uint32_t block_key, brick_key, bm, brick_occ;
+    uint64_t mask;
+
+    __device__ __forceinline__ bool init(const GridView& g, f3 o, f3 d)
+    {
+        o = o + mk(vt_signf(d.x), vt_signf(d.y), vt_signf(d.z)) * 0.001f;
+        const float vox = ((o.x - g.bmin[0]) * g.ext[0]) * g.rx;
+        const float voy = ((o.y - g.bmin[1]) * g.ext[1]) * g.ry;
+        const float voz = ((o.z - g.bmin[2]) * g.ext[2]) * g.rz;
+        const float fx = floorf(vox), fy = floorf(voy), fz = floorf(voz);
+        steps = -1;
+        if (fx < 0.0f || fy < 0.0f || fz < 0.0f || fx >= g.rx || fy >= g.ry || fz >= g.rz) return false;
+        const float dx = vt_stepf(vt_absf(d.x), 1e-5f) != 0.0f ? 1e-5f : d.x;
+        const float dy = vt_stepf(vt_absf(d.y), 1e-5f) != 0.0f ? 1e-5f : d.y;
+        const float dz = vt_stepf(vt_absf(d.z), 1e-5f) != 0.0f ? 1e-5f : d.z;
+        const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+        const float sx = vt_signf(dx), sy = vt_signf(dy), sz = vt_signf(dz);
+        disx = (((fx - vox) + 0.5f) + sx * 0.5f) * ix;
+        disy = (((fy - voy) + 0.5f) + sy * 0.5f) * iy;
+        disz = (((fz - voz) + 0.5f) + sz * 0.5f) * iz;
+        ax = vt_absf(ix); ay = vt_absf(iy); az = vt_absf(iz);
+        stx = sx > 0.0f ? 1 : (sx < 0.0f ? -1 : 0);
+        sty = sy > 0.0f ? 1 : (sy < 0.0f ? -1 : 0);
+        stz = sz > 0.0f ? 1 : (sz < 0.0f ? -1 : 0);
+        x = (int32_t)fx; y = (int32_t)fy; z = (int32_t)fz;
+        steps = 0;
+        block_key = brick_key = 0xffffffffu;
+        bm = brick_occ = 0u;
+        mask = 0ull;
+        return true;
+    }
+
+    template <bool COUNT>
+    __device__ __forceinline__ int advance(const GridView& g, const uint32_t* __restrict__ bits, int budget,
+                                           uint32_t& visits)
+    {
+        /* one limit for both the step budget and MAX_STEPS */
+        const int32_t limit = steps + budget < g.max_steps ? steps + budget : g.max_steps;
+        while (steps < limit) {
+            if ((uint32_t)x >= (uint32_t)g.nx || (uint32_t)y >= (uint32_t)g.ny || (uint32_t)z >= (uint32_t)g.nz) return 2;
+            if (COUNT) visits++;
+            const uint32_t bk = ((uint32_t)x >> 2) | (((uint32_t)y >> 2) << 11) | (((uint32_t)z >> 2) << 22);
+            if (bk != block_key) {
+                block_key = bk;
+                const uint32_t rk = (bk >> 1) & ((0x3ffu) | (0x3ffu << 11) | (0x3ffu << 22));
+                if (rk != brick_key) {
+                    brick_key = rk;
+                    bm = brick_morton((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3);
+                    brick_occ = (bits[bm >> 5] >> (bm & 31u)) & 1u;
+                }
+                const uint32_t sb = (((uint32_t)x >> 2) & 1u) | ((((uint32_t)y >> 2) & 1u) << 1) |
+                                    ((((uint32_t)z >> 2) & 1u) << 2);
+                mask = brick_occ ? g.occ64[(size_t)bm * 8u + sb] : 0ull;
+            }
+            if ((mask >> block_bit((uint32_t)x, (uint32_t)y, (uint32_t)z)) & 1ull) return 1;
+            const bool mx = !(disy < disx) && !(disz < disx);
+            const bool my = !(disx < disy) && !(disz < disy);
+            const bool mz = !(disy < disz) && !(disx < disz);
+            /* select form of the masked add, as in dda() */
+            disx = disx + (mx ? ax : 0.0f);
+            disy = disy + (my ? ay : 0.0f);
+            disz = disz + (mz ? az : 0.0f);
+            x += mx ? stx : 0;
+            y += my ? sty : 0;
+            z += mz ? stz : 0;
+            steps++;
+        }
+        return steps >= g.max_steps ? 2 : 0;
+    }
+
+    /* traverse() result of a finished march: kind and hit record. */
+    __device__ __forceinline__ int finish(const GridView& g, int status, Hit& h) const
+    {
+        h.x = x; h.y = y; h.z = z; h.steps = steps;
+        h.slot = status == 1 ? g.bricks[(size_t)bm * 512u + local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z)] : 0u;
+        if (status == 1) return 2;
+        return y < 0 ? 3 : 0;
+    }
+};
 
 /* traverse (dda.h:63-79): 2 voxel hit, 3 ground hit (final y < 0), 0 miss. */
 enum { KIND_MISS = 0, KIND_VOXEL = 2, KIND_GROUND = 3 };

@@ -26,7 +26,55 @@ struct TraceArgs {
     int32_t cdfv_lds_bytes;
 };
 
+struct Counters {
+    uint32_t rays, visits, interactions, samples, env_samples, env_evals;
+};
+
+enum { PH_PRIMARY = 0, PH_SHADOW = 1, PH_CONT = 2 };
+
+/* Registers of one path between traversals. */
+struct PathState {
+    f3 ray_o, ray_d;             /* the ray to trace next */
+    f3 basis_o;                  /* origin voxelSpaceToWorldSpace re-intersects from */
+    f3 throughput, radiance;
+    f3 direct;                   /* light-sample contribution pending its shadow ray */
+    f3 factor;                   /* f * |cos| / pdf of the sampled continuation */
+    f3 next_d;                   /* continuation direction */
+    float bsdf_pdf;
+    int32_t ex, ey, ez;          /* emissive voxel the shadow ray must reach */
+    int32_t light_is_voxel;
+    int32_t phase, bounces;
+    vt_pcg32 rng;
+};
+
+/* Wavefront path storage: one slot per path of a chunk of the launch's
+ * (pass, pixel) space, structure-of-arrays in HBM. */
+struct WfPaths {
+    float4* o;        /* ray_o.xyz, bsdf_pdf */
+    float4* d;        /* ray_d.xyz */
+    float4* bo;       /* basis_o.xyz */
+    float4* thr;      /* throughput */
+    float4* rad;      /* radiance */
+    float4* dir;      /* pending light-sample contribution */
+    float4* fac;      /* continuation factor */
+    float4* nd;       /* continuation direction */
+    int4* tgt;        /* emissive target voxel, light_is_voxel */
+    uint4* meta;      /* pixel, pass, phase | bounces << 8 */
+    uint64_t* rng;    /* PCG32 state (increment = 2*pixel+1) */
+    int4* hit;        /* traversal result: x, y, z, kind << 8 | slot */
+    float4* smp;      /* finished tonemapped sample of each path */
+    float4* qa0;      /* ray queue 0: origin.xyz, path slot id (as bits) */
+    float4* qb0;      /*              direction.xyz */
+    float4* qa1;      /* ray queue 1 */
+    float4* qb1;
+    uint32_t* cnt;    /* queue lengths: cnt[0] for q0, cnt[1] for q1 */
+    uint64_t cap;     /* slots */
+};
+
 size_t render_lds_bytes(const TraceArgs& a);
+/* Wavefront render of the launch in a: path chunks of at most P.cap slots,
+ * 1 + 2*max_bounces traverse/shade rounds per chunk. */
+hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats);
 hipError_t launch_render(const TraceArgs& a, int grid, hipStream_t s, bool stats);
 int render_blocks_per_cu(const TraceArgs& a, bool stats);
 hipError_t launch_first_hit(const TraceArgs& a, int32_t* out, hipStream_t s);
@@ -53,12 +101,16 @@ hipError_t launch_brick_from_ids(const BrickDims& d, const uint8_t* dense, const
                                  hipStream_t s);
 /* Recompute brick occupancy bits (bits must hold n_bricks/32 words). */
 hipError_t launch_brick_bits(const BrickDims& d, const uint8_t* bricks, uint32_t* bits, hipStream_t s);
+/* Per 4x4x4 block occupancy masks (occ64 holds n_bricks*8 words). */
+hipError_t launch_block_masks(const BrickDims& d, const uint8_t* bricks, uint64_t* occ64, hipStream_t s);
 /* Bricked slots -> dense int32 offsets via slot_off[256]. */
 hipError_t launch_dense_from_bricks(const BrickDims& d, const uint8_t* bricks, const int32_t* slot_off, int32_t* dense,
                                     hipStream_t s);
 /* Fill a byte buffer (graph-safe replacement for hipMemsetAsync). */
 hipError_t launch_fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+/* Streaming 16-byte-vector copy (bytes must be a multiple of 16). */
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 
 /* ---- voxelizer ------------------------------------------------------------------------- */
 struct VoxArgs {

@@ -15,64 +15,128 @@
 
 namespace vt {
 
-struct Counters {
-    uint32_t rays, visits, interactions, samples, env_samples, env_evals;
-};
 
-template <bool STATS>
-__device__ __forceinline__ f3 eval_env(const EnvView& e, f3 wi, float& pdf, Counters& cnt)
-{
-    const f3 L = background(e, wi);
-    if (e.use_image) {
-        if (STATS) cnt.env_evals++;
-        pdf = luminance(L) / e.integral;
-    } else {
-        pdf = 1.0f / (4.0f * VT_PI_F);
-    }
-    return L;
-}
+/* pathTracer.fs main (:172-296) with directLighting (:64-170), written as a
+ * per-path state machine with a single traversal site.  Each path vertex is
+ * shaded once: the light sample (first rand(), :77) and the BSDF sample
+ * (second rand(), :255) are both drawn before any ray is traced, the shadow
+ * ray's contribution is kept aside and added after its traversal (:133-164
+ * only needs visibility), then the continuation ray is traced (:282).  Draw
+ * order, operand order and accumulation order equal the reference's,
+ * including throughput * vec3(0) for occluded light samples. */
 
-/* directLighting (pathTracer.fs:64-170) */
+/* Camera ray of one pass of one pixel; returns false when the ray misses the
+ * volume bounds (S.radiance then holds the background). */
 template <bool STATS>
-__device__ __forceinline__ f3 direct_lighting(const TraceArgs& a, const MatRec* recs, const uint32_t* bits,
-                                              const float* cdfv, const MatRec& m, const Basis& B, f3 wo,
-                                              vt_pcg32& rng, Counters& cnt)
+__device__ __forceinline__ bool start_path(const TraceArgs& a, int32_t px, int32_t py, uint32_t rng_frame, PathState& S,
+                                           Counters& cnt)
 {
     const GridView& g = a.g;
+    vt_rng_init(&S.rng, a.seed, (uint32_t)(py * a.W + px), rng_frame);
+    float u[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+    if (a.c.lens_model == 0 || a.c.lens_model == 1) vt_rand4(&S.rng, u);
+    gen_ray(a.c, (float)px + 0.5f, (float)py + 0.5f, u, S.basis_o, S.ray_d);
+    if (STATS) { cnt.samples++; cnt.rays++; }
+    const float t = aabb(S.basis_o, S.ray_d, mk(g.bmin[0], g.bmin[1], g.bmin[2]), mk(g.bmax[0], g.bmax[1], g.bmax[2]));
+    if (t < 0.0f) {
+        S.radiance = background(a.e, S.ray_d);
+        if (STATS && a.e.use_image) cnt.env_evals++;
+        return false;
+    }
+    S.ray_o = S.basis_o + S.ray_d * t;
+    S.throughput = mk(1.0f, 1.0f, 1.0f);
+    S.radiance = mk(0.0f, 0.0f, 0.0f);
+    S.direct = S.factor = S.next_d = mk(0.0f, 0.0f, 0.0f);
+    S.bsdf_pdf = 0.0f;
+    S.ex = S.ey = S.ez = 0;
+    S.light_is_voxel = 0;
+    S.phase = PH_PRIMARY;
+    S.bounces = 0;
+    return true;
+}
+
+/* Consume the traversal result of the ray in S; returns true when S holds a
+ * new ray to trace, false when the path is finished. */
+template <bool STATS>
+__device__ __forceinline__ bool shade_step(const TraceArgs& a, const MatRec* recs, const uint32_t* bits,
+                                           const float* cdfv, PathState& S, const Hit& h, int kind, Counters& cnt)
+{
+    const GridView& g = a.g;
+    const Settings& st = a.s;
+    if (S.phase == PH_SHADOW) {
+        const bool visible = S.light_is_voxel ? (kind != KIND_MISS && h.x == S.ex && h.y == S.ey && h.z == S.ez)
+                                              : (kind == KIND_MISS);
+        S.radiance = S.radiance + S.throughput * (visible ? S.direct : mk(0.0f, 0.0f, 0.0f));
+        S.throughput = S.throughput * S.factor;
+        S.basis_o = S.ray_o;
+        S.ray_d = S.next_d;
+        S.phase = PH_CONT;
+        if (STATS) cnt.rays++;
+        return true;
+    }
+    if (kind == KIND_MISS) {
+        if (S.phase == PH_PRIMARY) {
+            S.radiance = background(a.e, S.ray_d);
+            if (STATS && a.e.use_image) cnt.env_evals++;
+        } else {
+            float lpdf;
+            const f3 L = background(a.e, S.ray_d);
+            if (a.e.use_image) {
+                if (STATS) cnt.env_evals++;
+                lpdf = luminance(L) / a.e.integral;
+            } else {
+                lpdf = 1.0f / (4.0f * VT_PI_F);
+            }
+            S.radiance = S.radiance + (S.throughput * L) * power_heuristic(S.bsdf_pdf, lpdf);
+        }
+        return false;
+    }
+    if (S.phase == PH_CONT) S.bounces++;
+    if (S.bounces >= st.max_bounces) return false;
+    /* ---- shade the vertex (pathTracer.fs:214-280) ---- */
+    if (STATS) cnt.interactions++;
+    const Basis B = voxel_basis(g, (float)h.x, (float)h.y, (float)h.z, S.basis_o, S.ray_d);
+    const MatRec& m = recs[h.slot];   /* ground hits carry slot 0 = record of offset 0 */
+    if (h.x == st.sel[0] && h.y == st.sel[1] && h.z == st.sel[2]) {
+        S.radiance = S.radiance + mk(1.0f, 0.0f, 0.0f);
+        return false;
+    }
+    const f3 wo = -S.ray_d;
+    const f3 lsWo = world_to_local(wo, B);
+    if (S.bounces == 0) S.radiance = S.radiance + S.throughput * mk(m.e[0], m.e[1], m.e[2]);
+    /* light sample (directLighting :77-124) */
     float u[4];
-    vt_rand4(&rng, u);
+    vt_rand4(&S.rng, u);
     const int numLights = g.n_emissive + 1;
     const int li = (int)(u[0] * (float)numLights);
-    const bool emissive = li < numLights - 1;
-    f3 L, wi;
-    float pdf;
-    int32_t ex = 0, ey = 0, ez = 0;
-    if (emissive) {
+    S.light_is_voxel = li < numLights - 1;
+    f3 L, wl;
+    float lpdf;
+    if (S.light_is_voxel) {
         int32_t idx = g.emissive[li];
         const int32_t dz = g.nx * g.ny, dy = g.nx;
-        ez = idx / dz; idx -= ez * dz;
-        ey = idx / dy; idx -= ey * dy;
-        ex = idx;
-        const MatRec& me = recs[fetch_slot(g, bits, ex, ey, ez)];
+        S.ez = idx / dz; idx -= S.ez * dz;
+        S.ey = idx / dy; idx -= S.ey * dy;
+        S.ex = idx;
+        const MatRec& me = recs[fetch_slot(g, bits, S.ex, S.ey, S.ez)];
         L = mk(me.e[0], me.e[1], me.e[2]) * 10.0f;
-        const f3 res = mk(g.rx, g.ry, g.rz);
-        const f3 vsE = mk((float)ex, (float)ey, (float)ez);
+        const f3 vsE = mk((float)S.ex, (float)S.ey, (float)S.ez);
         const f3 bmin = mk(g.bmin[0], g.bmin[1], g.bmin[2]), bmax = mk(g.bmax[0], g.bmax[1], g.bmax[2]);
-        f3 wsE = mk(vsE.x / res.x, vsE.y / res.y, vsE.z / res.z) * (bmax - bmin) + bmin;
+        f3 wsE = mk(vsE.x / g.rx, vsE.y / g.ry, vsE.z / g.rz) * (bmax - bmin) + bmin;
         wsE = wsE + mk(u[1], u[2], u[3]) * mk(g.vs[0], g.vs[1], g.vs[2]);
         const f3 toLight = wsE - B.p;
         const float r = length(toLight);
-        wi = toLight / r;
-        const Basis LB = voxel_basis(g, vsE.x, vsE.y, vsE.z, B.p, wi);
+        wl = toLight / r;
+        const Basis LB = voxel_basis(g, vsE.x, vsE.y, vsE.z, B.p, wl);
         const float area = (6.0f * g.vs[0]) * g.vs[1];
-        const float jac = (r * r) / vt_absf(dot(-wi, LB.n));
-        pdf = jac / area;
+        const float jac = (r * r) / vt_absf(dot(-wl, LB.n));
+        lpdf = jac / area;
     } else if (a.e.use_image) {
         float uu, vv;
         env_sample_uv(a.e, cdfv, u[1], u[2], uu, vv);
-        wi = dir_from_uv(uu, vv, a.e.rotation);
+        wl = dir_from_uv(uu, vv, a.e.rotation);
         L = tex_bilinear(a.e, uu, vv);
-        pdf = luminance(L) / a.e.integral;
+        lpdf = luminance(L) / a.e.integral;
         if (STATS) cnt.env_samples++;
     } else {
         const float y = u[1];
@@ -80,109 +144,69 @@ __device__ __forceinline__ f3 direct_lighting(const TraceArgs& a, const MatRec* 
         const float phi = (u[2] * 2.0f) * VT_PI_F;
         float sn, cs;
         vt_sincosf(phi, &sn, &cs);
-        pdf = 1.0f / (2.0f * VT_PI_F);
-        wi = local_to_world(mk(r * cs, y, r * sn), B);
-        L = background(a.e, wi);
+        lpdf = 1.0f / (2.0f * VT_PI_F);
+        wl = local_to_world(mk(r * cs, y, r * sn), B);
+        L = background(a.e, wl);
     }
-    pdf = pdf / (float)numLights;
-    Hit h;
-    if (STATS) cnt.rays++;
-    const int kind = traverse<STATS>(g, bits, B.p, wi, h, cnt.visits);
-    if (emissive) {
-        if (kind == KIND_MISS || h.x != ex || h.y != ey || h.z != ez) return mk(0.0f, 0.0f, 0.0f);
-    } else {
-        if (kind != KIND_MISS) return mk(0.0f, 0.0f, 0.0f);
+    lpdf = lpdf / (float)numLights;
+    {
+        f3 f;
+        float bpdf;
+        eval_material(m, lsWo, world_to_local(wl, B), f, bpdf);
+        const float mis = power_heuristic(lpdf, bpdf);
+        S.direct = ((f * L) * vt_absf(dot(wl, B.n))) * mis / lpdf;
     }
-    const f3 lsWo = world_to_local(wo, B);
-    const f3 lsWi = world_to_local(wi, B);
+    /* BSDF sample (:255-276) */
     f3 f;
-    float bpdf;
-    eval_material(m, lsWo, lsWi, f, bpdf);
-    const float mis = power_heuristic(pdf, bpdf);
-    const float cosv = vt_absf(dot(wi, B.n));
-    return ((f * L) * cosv) * mis / pdf;
+    const f3 lsWi = sample_material(m, lsWo, S.rng, f, S.bsdf_pdf);
+    if (st.wire_opacity > 0.0f) {
+        const f3 q = B.p - mk(g.bmin[0], g.bmin[1], g.bmin[2]);
+        const f3 ext = mk(g.bmax[0] - g.bmin[0], g.bmax[1] - g.bmin[1], g.bmax[2] - g.bmin[2]);
+        const f3 vc = mk(q.x / ext.x, q.y / ext.y, q.z / ext.z) * mk(g.rx, g.ry, g.rz);
+        const f3 uvw = mk((float)h.x, (float)h.y, (float)h.z) - vc;
+        const float ux = vt_absf(dot(mk(B.n.y, B.n.z, B.n.x), uvw));
+        const float uy = vt_absf(dot(mk(B.n.z, B.n.x, B.n.y), uvw));
+        const float th = st.wire_thickness;
+        float w = ((vt_stepf(th, ux) * vt_stepf(ux, 1.0f - th)) * vt_stepf(th, uy)) * vt_stepf(uy, 1.0f - th);
+        w = (1.0f - st.wire_opacity) + st.wire_opacity * w;
+        f = f * w;
+    }
+    S.next_d = local_to_world(lsWi, B);
+    S.factor = (f * vt_absf(dot(S.next_d, B.n))) / S.bsdf_pdf;
+    S.ray_o = B.p;
+    S.ray_d = wl;
+    S.phase = PH_SHADOW;
+    if (STATS) cnt.rays++;
+    return true;
 }
 
-/* pathTracer.fs main (:172-296) for one pixel and one pass. */
+__device__ __forceinline__ f3 tonemap(const Settings& st, f3 r)
+{
+    const float ig = st.inv_gamma;
+    return mk(vt_powf(r.x * st.exposure, ig), vt_powf(r.y * st.exposure, ig), vt_powf(r.z * st.exposure, ig));
+}
+
+/* One pass of one pixel, start to finish (megakernel form). */
 template <bool STATS>
 __device__ __forceinline__ f3 trace_sample(const TraceArgs& a, const MatRec* recs, const uint32_t* bits,
                                            const float* cdfv, int32_t px, int32_t py, uint32_t rng_frame,
                                            Counters& cnt)
 {
-    const GridView& g = a.g;
-    const Settings& st = a.s;
-    vt_pcg32 rng;
-    vt_rng_init(&rng, a.seed, (uint32_t)(py * a.W + px), rng_frame);
-    float u[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
-    if (a.c.lens_model == 0 || a.c.lens_model == 1) vt_rand4(&rng, u);
-    f3 o, d;
-    gen_ray(a.c, (float)px + 0.5f, (float)py + 0.5f, u, o, d);
-    if (STATS) { cnt.samples++; cnt.rays++; }
-    const float t = aabb(o, d, mk(g.bmin[0], g.bmin[1], g.bmin[2]), mk(g.bmax[0], g.bmax[1], g.bmax[2]));
-    f3 radiance = mk(0.0f, 0.0f, 0.0f);
-    bool escaped = false;
-    Hit h;
-    if (t < 0.0f) {
-        escaped = true;
-    } else {
-        const f3 entry = o + d * t;
-        if (traverse<STATS>(g, bits, entry, d, h, cnt.visits) == KIND_MISS) escaped = true;
-    }
-    if (escaped) {
-        radiance = background(a.e, d);
-        if (STATS && a.e.use_image) cnt.env_evals++;
-    } else {
-        f3 throughput = mk(1.0f, 1.0f, 1.0f);
-        int bounces = 0;
-        while (bounces < st.max_bounces) {
-            if (STATS) cnt.interactions++;
-            const Basis B = voxel_basis(g, (float)h.x, (float)h.y, (float)h.z, o, d);
-            const MatRec& m = recs[h.slot];   /* ground hits carry slot 0 = record of offset 0 */
-            if (h.x == st.sel[0] && h.y == st.sel[1] && h.z == st.sel[2]) {
-                radiance = radiance + mk(1.0f, 0.0f, 0.0f);
-                break;
-            }
-            const f3 wo = -d;
-            const f3 lsWo = world_to_local(wo, B);
-            if (bounces == 0) radiance = radiance + throughput * mk(m.e[0], m.e[1], m.e[2]);
-            radiance = radiance + throughput * direct_lighting<STATS>(a, recs, bits, cdfv, m, B, wo, rng, cnt);
-            f3 f;
-            float pdf;
-            const f3 lsWi = sample_material(m, lsWo, rng, f, pdf);
-            if (st.wire_opacity > 0.0f) {
-                const f3 q = B.p - mk(g.bmin[0], g.bmin[1], g.bmin[2]);
-                const f3 ext = mk(g.bmax[0] - g.bmin[0], g.bmax[1] - g.bmin[1], g.bmax[2] - g.bmin[2]);
-                const f3 vc = mk(q.x / ext.x, q.y / ext.y, q.z / ext.z) * mk(g.rx, g.ry, g.rz);
-                const f3 uvw = mk((float)h.x, (float)h.y, (float)h.z) - vc;
-                const float ux = vt_absf(dot(mk(B.n.y, B.n.z, B.n.x), uvw));
-                const float uy = vt_absf(dot(mk(B.n.z, B.n.x, B.n.y), uvw));
-                const float th = st.wire_thickness;
-                float w = ((vt_stepf(th, ux) * vt_stepf(ux, 1.0f - th)) * vt_stepf(th, uy)) * vt_stepf(uy, 1.0f - th);
-                w = (1.0f - st.wire_opacity) + st.wire_opacity * w;
-                f = f * w;
-            }
-            const f3 wi = local_to_world(lsWi, B);
-            const float cosv = vt_absf(dot(wi, B.n));
-            throughput = throughput * ((f * cosv) / pdf);
-            o = B.p;
-            d = wi;
-            if (STATS) cnt.rays++;
-            if (traverse<STATS>(g, bits, o, d, h, cnt.visits) == KIND_MISS) {
-                float lpdf;
-                const f3 L = eval_env<STATS>(a.e, d, lpdf, cnt);
-                const float mis = power_heuristic(pdf, lpdf);
-                radiance = radiance + (throughput * L) * mis;
-                break;
-            }
-            bounces++;
+    PathState S;
+    if (start_path<STATS>(a, px, py, rng_frame, S, cnt)) {
+        for (;;) {
+            Hit h;
+            const int kind = traverse<STATS>(a.g, bits, S.ray_o, S.ray_d, h, cnt.visits);
+            if (!shade_step<STATS>(a, recs, bits, cdfv, S, h, kind, cnt)) break;
         }
     }
-    const float ig = st.inv_gamma;
-    return mk(vt_powf(radiance.x * st.exposure, ig), vt_powf(radiance.y * st.exposure, ig),
-              vt_powf(radiance.z * st.exposure, ig));
+    return tonemap(a.s, S.radiance);
 }
 
-/* Stage records, CDF marginal and (optionally) occupancy bits in LDS. */
+/* Stage records, CDF marginal and (when LDSB) the brick occupancy bits in
+ * LDS.  LDSB is a template parameter so the traversal's bit reads compile to
+ * LDS instructions rather than generic (flat) loads. */
+template <bool LDSB>
 __device__ __forceinline__ void stage_lds(const TraceArgs& a, uint8_t* smem, MatRec*& recs, float*& cdfv,
                                           const uint32_t*& bits)
 {
@@ -195,7 +219,7 @@ __device__ __forceinline__ void stage_lds(const TraceArgs& a, uint8_t* smem, Mat
         for (int i = threadIdx.x; i < a.e.cv_n; i += blockDim.x) cv[i] = a.e.cdfv[i];
     }
     cdfv = cv;
-    if (a.lds_bits) {
+    if (LDSB) {
         uint32_t* sb = (uint32_t*)(smem + 256 * sizeof(MatRec) + a.cdfv_lds_bytes);
         for (uint32_t i = threadIdx.x; i < a.g.bit_words; i += blockDim.x) sb[i] = a.g.bits[i];
         bits = sb;
@@ -205,14 +229,14 @@ __device__ __forceinline__ void stage_lds(const TraceArgs& a, uint8_t* smem, Mat
     __syncthreads();
 }
 
-template <bool STATS>
+template <bool STATS, bool LDSB>
 __global__ __launch_bounds__(256) void render_kernel(TraceArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     MatRec* recs;
     float* cdfv;
     const uint32_t* bits;
-    stage_lds(a, smem, recs, cdfv, bits);
+    stage_lds<LDSB>(a, smem, recs, cdfv, bits);
     const int lane = threadIdx.x & 63;
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
     uint64_t tot[6] = { 0, 0, 0, 0, 0, 0 };
@@ -225,22 +249,25 @@ __global__ __launch_bounds__(256) void render_kernel(TraceArgs a)
         const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
         if (px < a.W && py < a.H) {
             const size_t p = (size_t)py * (size_t)a.W + (size_t)px;
-            float4 acc = a.accum[p];
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-            if (a.sum64) { s0 = a.sum64[3 * p]; s1 = a.sum64[3 * p + 1]; s2 = a.sum64[3 * p + 2]; }
             for (uint32_t k = 0; k < a.count; ++k) {
                 const uint32_t n = a.n0 + k;
                 const uint32_t rf = n < a.rng_frame_max ? n : a.rng_frame_max;
                 const f3 c = trace_sample<STATS>(a, recs, bits, cdfv, px, py, rf, cnt);
+                /* accumulation.fs:17, one pass at a time (the running mean
+                 * stays in memory between passes, as the ping-pong textures do) */
                 const float fn = (float)n, fn1 = (float)(n + 1u);
+                float4 acc = a.accum[p];
                 acc.x = (c.x + acc.x * fn) / fn1;
                 acc.y = (c.y + acc.y * fn) / fn1;
                 acc.z = (c.z + acc.z * fn) / fn1;
                 acc.w = (1.0f + acc.w * fn) / fn1;
-                if (a.sum64) { s0 += (double)c.x; s1 += (double)c.y; s2 += (double)c.z; }
+                a.accum[p] = acc;
+                if (a.sum64) {
+                    a.sum64[3 * p] += (double)c.x;
+                    a.sum64[3 * p + 1] += (double)c.y;
+                    a.sum64[3 * p + 2] += (double)c.z;
+                }
             }
-            a.accum[p] = acc;
-            if (a.sum64) { a.sum64[3 * p] = s0; a.sum64[3 * p + 1] = s1; a.sum64[3 * p + 2] = s2; }
         }
         if (STATS) {
             tot[0] += cnt.rays; tot[1] += cnt.visits; tot[2] += cnt.interactions;
@@ -256,6 +283,300 @@ __global__ __launch_bounds__(256) void render_kernel(TraceArgs a)
         __syncthreads();
         if (threadIdx.x < 6) atomicAdd(&a.stats[threadIdx.x], red[threadIdx.x]);
     }
+}
+
+/* ---- wavefront form ---------------------------------------------------------------
+ * generate -> (traverse -> shade) x (1 + 2*max_bounces) -> accumulate, per
+ * chunk of paths.  The traversal kernel only runs the DDA, so it needs few
+ * registers and keeps many waves resident; shading consumes the hits and
+ * appends surviving paths to the next queue with one atomic per wave. */
+
+__device__ __forceinline__ void store_path(const WfPaths& P, uint32_t id, const PathState& S, uint32_t pix,
+                                           uint32_t pass)
+{
+    P.o[id] = make_float4(S.ray_o.x, S.ray_o.y, S.ray_o.z, S.bsdf_pdf);
+    P.d[id] = make_float4(S.ray_d.x, S.ray_d.y, S.ray_d.z, 0.0f);
+    P.bo[id] = make_float4(S.basis_o.x, S.basis_o.y, S.basis_o.z, 0.0f);
+    P.thr[id] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
+    P.rad[id] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
+    P.dir[id] = make_float4(S.direct.x, S.direct.y, S.direct.z, 0.0f);
+    P.fac[id] = make_float4(S.factor.x, S.factor.y, S.factor.z, 0.0f);
+    P.nd[id] = make_float4(S.next_d.x, S.next_d.y, S.next_d.z, 0.0f);
+    P.tgt[id] = make_int4(S.ex, S.ey, S.ez, S.light_is_voxel);
+    P.meta[id] = make_uint4(pix, pass, (uint32_t)S.phase | ((uint32_t)S.bounces << 8), 0u);
+    P.rng[id] = S.rng.state;
+}
+
+__device__ __forceinline__ void load_path(const WfPaths& P, uint32_t id, PathState& S, uint32_t& pix, uint32_t& pass)
+{
+    const float4 o = P.o[id], d = P.d[id], bo = P.bo[id], th = P.thr[id], rd = P.rad[id], di = P.dir[id],
+                 fa = P.fac[id], nd = P.nd[id];
+    const int4 tg = P.tgt[id];
+    const uint4 me = P.meta[id];
+    S.ray_o = mk(o.x, o.y, o.z);
+    S.bsdf_pdf = o.w;
+    S.ray_d = mk(d.x, d.y, d.z);
+    S.basis_o = mk(bo.x, bo.y, bo.z);
+    S.throughput = mk(th.x, th.y, th.z);
+    S.radiance = mk(rd.x, rd.y, rd.z);
+    S.direct = mk(di.x, di.y, di.z);
+    S.factor = mk(fa.x, fa.y, fa.z);
+    S.next_d = mk(nd.x, nd.y, nd.z);
+    S.ex = tg.x; S.ey = tg.y; S.ez = tg.z; S.light_is_voxel = tg.w;
+    pix = me.x;
+    pass = me.y;
+    S.phase = (int32_t)(me.z & 0xffu);
+    S.bounces = (int32_t)(me.z >> 8);
+    S.rng.state = P.rng[id];
+    S.rng.inc = ((uint64_t)pix << 1u) | 1u;
+}
+
+template <bool STATS>
+__device__ __forceinline__ void flush_counters(const TraceArgs& a, const Counters& cnt)
+{
+    if (!STATS) return;
+    __shared__ unsigned long long red[6];
+    if (threadIdx.x < 6) red[threadIdx.x] = 0ull;
+    __syncthreads();
+    atomicAdd(&red[0], (unsigned long long)cnt.rays);
+    atomicAdd(&red[1], (unsigned long long)cnt.visits);
+    atomicAdd(&red[2], (unsigned long long)cnt.interactions);
+    atomicAdd(&red[3], (unsigned long long)cnt.samples);
+    atomicAdd(&red[4], (unsigned long long)cnt.env_samples);
+    atomicAdd(&red[5], (unsigned long long)cnt.env_evals);
+    __syncthreads();
+    if (threadIdx.x < 6 && red[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], red[threadIdx.x]);
+}
+
+/* Append the ray (o, d) of path `id` for the lanes with `keep` to a ray
+ * queue (length counter *n), one atomic per wave; the queue stores the rays
+ * themselves so the traversal reads them in queue order.  Every lane of the
+ * wave must call it. */
+__device__ __forceinline__ void wave_append(bool keep, uint32_t id, f3 o, f3 d, float4* qa, float4* qb, uint32_t* n)
+{
+    const unsigned long long mask = __ballot(keep);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0;
+    if (lane == 0 && mask) base = atomicAdd(n, (uint32_t)__popcll(mask));
+    base = __shfl(base, 0, 64);
+    if (keep) {
+        const uint32_t j = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        qa[j] = make_float4(o.x, o.y, o.z, __uint_as_float(id));
+        qb[j] = make_float4(d.x, d.y, d.z, 0.0f);
+    }
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(256) void wf_generate_kernel(TraceArgs a, WfPaths P, uint64_t g0, uint32_t n)
+{
+    Counters cnt = { 0, 0, 0, 0, 0, 0 };
+    const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
+    const uint32_t wave_base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (uint32_t base = wave_base; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + (threadIdx.x & 63u);
+        bool keep = false;
+        f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
+        if (i < n) {
+            const uint64_t gi = g0 + i;
+            const uint32_t k = (uint32_t)(gi / WH), pix = (uint32_t)(gi % WH);
+            const int32_t px = (int32_t)(pix % (uint32_t)a.W), py = (int32_t)(pix / (uint32_t)a.W);
+            const uint32_t nn = a.n0 + k;
+            const uint32_t rf = nn < a.rng_frame_max ? nn : a.rng_frame_max;
+            PathState S;
+            if (start_path<STATS>(a, px, py, rf, S, cnt)) {
+                store_path(P, i, S, pix, k);
+                keep = true;
+                ro = S.ray_o;
+                rd = S.ray_d;
+            } else {
+                const f3 c = tonemap(a.s, S.radiance);
+                P.smp[i] = make_float4(c.x, c.y, c.z, 1.0f);
+            }
+        }
+        wave_append(keep, i, ro, rd, P.qa0, P.qb0, &P.cnt[0]);
+    }
+    flush_counters<STATS>(a, cnt);
+}
+
+/* Persistent traversal: each lane marches its ray up to DDA_BUDGET steps at
+ * a time; when at least REFILL_MIN lanes of the wave are idle they take the
+ * next queue entries together (one atomic per wave on the queue head, loads
+ * in queue order), so lanes stay busy until the queue drains instead of
+ * idling behind the longest ray of their wave. */
+#define DDA_BUDGET 16
+#define REFILL_MIN 16
+
+template <bool STATS, bool LDSB>
+__global__ __launch_bounds__(256) void wf_traverse_kernel(TraceArgs a, WfPaths P, int sel)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    MatRec* recs;
+    float* cdfv;
+    const uint32_t* bits;
+    stage_lds<LDSB>(a, smem, recs, cdfv, bits);
+    const float4* qa = sel ? P.qa1 : P.qa0;
+    const float4* qb = sel ? P.qb1 : P.qb0;
+    const uint32_t n = P.cnt[sel];
+    uint32_t* head = &P.cnt[2 + sel];
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.cnt[sel ^ 1] = 0u;
+    Counters cnt = { 0, 0, 0, 0, 0, 0 };
+    const uint32_t lane = threadIdx.x & 63u;
+    DdaState D;
+    uint32_t id = 0;
+    bool has = false;
+    bool exhausted = false;
+    for (;;) {
+        const unsigned long long idle = __ballot(!has);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (!exhausted && (n_idle >= REFILL_MIN || n_idle == 64u)) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(head, n_idle);
+            base = __shfl(base, 0, 64);
+            if (base + n_idle >= n) exhausted = true;
+            if (!has) {
+                const uint32_t j = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (j < n) {
+                    const float4 o = qa[j], d = qb[j];
+                    id = __float_as_uint(o.w);
+                    if (D.init(a.g, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z))) {
+                        has = true;
+                    } else {
+                        P.hit[id] = make_int4(0, 0, 0, KIND_MISS << 8);
+                    }
+                }
+            }
+        }
+        if (!__ballot(has)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (has) {
+            const int status = D.advance<STATS>(a.g, bits, DDA_BUDGET, cnt.visits);
+            if (status) {
+                Hit h;
+                const int kind = D.finish(a.g, status, h);
+                P.hit[id] = make_int4(h.x, h.y, h.z, (kind << 8) | (int32_t)h.slot);
+                has = false;
+            }
+        }
+    }
+    flush_counters<STATS>(a, cnt);
+}
+
+template <bool STATS, bool LDSB>
+__global__ __launch_bounds__(256) void wf_shade_kernel(TraceArgs a, WfPaths P, int sel)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    MatRec* recs;
+    float* cdfv;
+    const uint32_t* bits;
+    stage_lds<LDSB>(a, smem, recs, cdfv, bits);
+    const float4* qa = sel ? P.qa1 : P.qa0;
+    float4* qao = sel ? P.qa0 : P.qa1;
+    float4* qbo = sel ? P.qb0 : P.qb1;
+    const uint32_t n = P.cnt[sel];
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.cnt[2 + (sel ^ 1)] = 0u;
+    Counters cnt = { 0, 0, 0, 0, 0, 0 };
+    const uint32_t wave_base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (uint32_t base = wave_base; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t j = base + (threadIdx.x & 63u);
+        bool keep = false;
+        uint32_t id = 0;
+        f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
+        if (j < n) {
+            id = __float_as_uint(qa[j].w);
+            PathState S;
+            uint32_t pix, pass;
+            load_path(P, id, S, pix, pass);
+            const int4 hv = P.hit[id];
+            Hit h;
+            h.x = hv.x; h.y = hv.y; h.z = hv.z; h.slot = (uint32_t)hv.w & 0xffu;
+            h.steps = 0;
+            const int kind = hv.w >> 8;
+            keep = shade_step<STATS>(a, recs, bits, cdfv, S, h, kind, cnt);
+            if (keep) {
+                store_path(P, id, S, pix, pass);
+                ro = S.ray_o;
+                rd = S.ray_d;
+            } else {
+                const f3 c = tonemap(a.s, S.radiance);
+                P.smp[id] = make_float4(c.x, c.y, c.z, 1.0f);
+            }
+        }
+        wave_append(keep, id, ro, rd, qao, qbo, &P.cnt[sel ^ 1]);
+    }
+    flush_counters<STATS>(a, cnt);
+}
+
+/* accumulation.fs:17 over the chunk's samples, pass by pass per pixel. */
+__global__ __launch_bounds__(256) void wf_accumulate_kernel(TraceArgs a, WfPaths P, uint64_t g0, uint64_t g1)
+{
+    const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
+    const uint32_t k_lo = (uint32_t)(g0 / WH), k_hi = (uint32_t)((g1 - 1) / WH);
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < WH; p += (uint64_t)gridDim.x * blockDim.x) {
+        float4 acc = a.accum[p];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        if (a.sum64) { s0 = a.sum64[3 * p]; s1 = a.sum64[3 * p + 1]; s2 = a.sum64[3 * p + 2]; }
+        bool any = false;
+        for (uint32_t k = k_lo; k <= k_hi; ++k) {
+            const uint64_t gi = (uint64_t)k * WH + p;
+            if (gi < g0 || gi >= g1) continue;
+            any = true;
+            const float4 c = P.smp[gi - g0];
+            const uint32_t n = a.n0 + k;
+            const float fn = (float)n, fn1 = (float)(n + 1u);
+            acc.x = (c.x + acc.x * fn) / fn1;
+            acc.y = (c.y + acc.y * fn) / fn1;
+            acc.z = (c.z + acc.z * fn) / fn1;
+            acc.w = (1.0f + acc.w * fn) / fn1;
+            if (a.sum64) { s0 += (double)c.x; s1 += (double)c.y; s2 += (double)c.z; }
+        }
+        if (any) {
+            a.accum[p] = acc;
+            if (a.sum64) { a.sum64[3 * p] = s0; a.sum64[3 * p + 1] = s1; a.sum64[3 * p + 2] = s2; }
+        }
+    }
+}
+
+__global__ void wf_reset_kernel(uint32_t* cnt)
+{
+    if (threadIdx.x < 4) cnt[threadIdx.x] = 0u;
+}
+
+template <bool STATS, bool LDSB>
+static hipError_t wf_run(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s)
+{
+    const size_t lds = render_lds_bytes(a);
+    int bt = 0, bs = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bt, wf_traverse_kernel<STATS, LDSB>, 256, lds) != hipSuccess || bt < 1)
+        bt = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, wf_shade_kernel<STATS, LDSB>, 256, lds) != hipSuccess || bs < 1)
+        bs = 1;
+    const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
+    const uint64_t total = WH * a.count;
+    const int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
+    for (uint64_t g0 = 0; g0 < total; g0 += P.cap) {
+        const uint64_t g1 = g0 + P.cap < total ? g0 + P.cap : total;
+        const uint32_t n = (uint32_t)(g1 - g0);
+        wf_reset_kernel<<<1, 64, 0, s>>>(P.cnt);
+        const unsigned gg = (unsigned)((n + 255) / 256 < (uint32_t)cus * 8u ? (n + 255) / 256 : (uint32_t)cus * 8u);
+        wf_generate_kernel<STATS><<<gg, 256, 0, s>>>(a, P, g0, n);
+        int sel = 0;
+        for (int r = 0; r < rounds; ++r) {
+            wf_traverse_kernel<STATS, LDSB><<<cus * bt, 256, lds, s>>>(a, P, sel);
+            wf_shade_kernel<STATS, LDSB><<<cus * bs, 256, lds, s>>>(a, P, sel);
+            sel ^= 1;
+        }
+        const unsigned ga = (unsigned)((WH + 255) / 256 < (uint64_t)cus * 8u ? (WH + 255) / 256 : (uint64_t)cus * 8u);
+        wf_accumulate_kernel<<<ga, 256, 0, s>>>(a, P, g0, g1);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats)
+{
+    if (stats) return a.lds_bits ? wf_run<true, true>(a, P, cus, s) : wf_run<true, false>(a, P, cus, s);
+    return a.lds_bits ? wf_run<false, true>(a, P, cus, s) : wf_run<false, false>(a, P, cus, s);
 }
 
 /* First hit of the primary ray of every pixel (the parity probe). */
@@ -339,11 +660,16 @@ size_t render_lds_bytes(const TraceArgs& a)
     return 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes + (a.lds_bits ? (size_t)a.g.bit_words * 4u : 0u);
 }
 
+static void (*render_ptr(bool stats, bool ldsb))(TraceArgs)
+{
+    if (stats) return ldsb ? render_kernel<true, true> : render_kernel<true, false>;
+    return ldsb ? render_kernel<false, true> : render_kernel<false, false>;
+}
+
 hipError_t launch_render(const TraceArgs& a, int grid, hipStream_t s, bool stats)
 {
     const size_t lds = render_lds_bytes(a);
-    if (stats) render_kernel<true><<<grid, 256, lds, s>>>(a);
-    else render_kernel<false><<<grid, 256, lds, s>>>(a);
+    render_ptr(stats, a.lds_bits != 0)<<<grid, 256, lds, s>>>(a);
     return hipGetLastError();
 }
 
@@ -351,9 +677,8 @@ int render_blocks_per_cu(const TraceArgs& a, bool stats)
 {
     int blocks = 0;
     const size_t lds = render_lds_bytes(a);
-    hipError_t e;
-    if (stats) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_kernel<true>, 256, lds);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_kernel<false>, 256, lds);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_ptr(stats, a.lds_bits != 0),
+                                                                       256, lds);
     if (e != hipSuccess || blocks < 1) blocks = 1;
     return blocks;
 }

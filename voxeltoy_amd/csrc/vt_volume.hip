@@ -66,6 +66,35 @@ __global__ __launch_bounds__(256) void brick_bits_kernel(BrickDims d, const uint
     }
 }
 
+/* One mask per 4x4x4 block: occ64[w] bit (x + 4y + 16z) is set when the
+ * voxel at local (x, y, z) of the block is occupied; the block's 64 bytes
+ * are bricks[w*64 ...] in Morton order. */
+__global__ __launch_bounds__(256) void block_masks_kernel(uint64_t n_words, const uint8_t* __restrict__ bricks,
+                                                          uint64_t* __restrict__ occ64)
+{
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_words;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4* p = (const uint4*)(bricks + w * 64u);
+        uint64_t m = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 v = p[q];
+            const uint32_t wd[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const uint32_t mo = (uint32_t)(q * 16 + k * 4 + b);   /* Morton index in the block */
+                    const uint32_t bit = (mo & 1u) | ((mo >> 2) & 2u) |            /* x bits 0, 3 */
+                                         (((mo >> 1) & 1u) << 2) | (((mo >> 4) & 1u) << 3) |   /* y */
+                                         (((mo >> 2) & 1u) << 4) | (((mo >> 5) & 1u) << 5);    /* z */
+                    if ((wd[k] >> (8 * b)) & 0xffu) m |= 1ull << bit;
+                }
+        }
+        occ64[w] = m;
+    }
+}
+
 __global__ __launch_bounds__(256) void dense_from_bricks_kernel(BrickDims d, const uint8_t* __restrict__ bricks,
                                                                 const int32_t* __restrict__ slot_off,
                                                                 int32_t* __restrict__ dense)
@@ -99,6 +128,12 @@ __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v)
         p[i] = v;
 }
 
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 static unsigned grid_for(uint64_t n)
 {
     uint64_t b = (n + 255) / 256;
@@ -127,6 +162,13 @@ hipError_t launch_brick_bits(const BrickDims& d, const uint8_t* bricks, uint32_t
     return hipGetLastError();
 }
 
+hipError_t launch_block_masks(const BrickDims& d, const uint8_t* bricks, uint64_t* occ64, hipStream_t s)
+{
+    const uint64_t n_words = d.n_bricks * 8u;
+    block_masks_kernel<<<grid_for(n_words), 256, 0, s>>>(n_words, bricks, occ64);
+    return hipGetLastError();
+}
+
 hipError_t launch_dense_from_bricks(const BrickDims& d, const uint8_t* bricks, const int32_t* slot_off, int32_t* dense,
                                     hipStream_t s)
 {
@@ -137,6 +179,13 @@ hipError_t launch_dense_from_bricks(const BrickDims& d, const uint8_t* bricks, c
 hipError_t launch_fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s)
 {
     fill_u8_kernel<<<grid_for(n / 16 + 1), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s)
+{
+    const uint64_t n16 = bytes / 16;
+    copy_kernel<<<grid_for(n16), 256, 0, s>>>((const uint4*)src, (uint4*)dst, n16);
     return hipGetLastError();
 }
 

@@ -4,8 +4,8 @@
            orbit angles, constant-white environment, 4 spp, 4 bounces.
   full   : 256^3 terrain + boxes, 70/20/10 Lambert/metal/emissive, 1024x512
            FP32 sky+sun environment (importance sampled), 1024x1024 thin-lens
-           camera (lens radius 0.02 x grid extent, random focus depth),
-           16 spp, 8 bounces.
+           camera looking down onto the terrain (lens radius 0.02 x grid
+           extent, random focus depth), 16 spp, 8 bounces.
 """
 from __future__ import annotations
 
@@ -72,10 +72,15 @@ def full(width: int = 1024, height: int = 1024, seed_scene: int = 2, n: int = 25
     bmin, bmax = reference_bounds(sc.shape)
     rng = np.random.default_rng(seed_scene + 100)
     extent = float(np.max(bmax - bmin))
-    cam = scenes.orbit_camera(bmin, bmax, math.radians(115.0), math.radians(30.0), width, height, lens_model=1,
+    # looking down 40 degrees onto the terrain from half the focusOnBounds distance
+    cam = scenes.orbit_camera(bmin, bmax, math.radians(130.0), math.radians(30.0), width, height, lens_model=1,
                               lens_radius=0.02 * extent)
+    target = cam.target.copy()
+    target[1] = bmin[1] + 0.2 * extent
+    cam.eye = (target + (cam.eye - cam.target) * 0.5).astype(np.float32)
+    cam.target = target.astype(np.float32)
     dist = float(np.linalg.norm(cam.target - cam.eye))
-    cam.focal_distance = float(rng.uniform(dist - 0.4 * extent, dist + 0.2 * extent))
+    cam.focal_distance = float(rng.uniform(dist - 0.3 * extent, dist + 0.2 * extent))
     img = scenes.sky_environment(seed_scene + 1, env_w, env_h)
     cdf = api.env_build_cdf(img)
     return Workload("full", sc, cam, width, height, spp=spp, bounces=bounces, env_image=img, cdf=cdf, bmin=bmin,
