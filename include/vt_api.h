@@ -137,9 +137,17 @@ int vt_set_stats(vt_context* ctx, int32_t enabled);
 /* Kernel organisation of vt_render (results are identical):
  *   0 = persistent megakernel (one lane renders a pixel's passes start to end)
  *   1 = wavefront (generate / traverse / shade kernels over ray queues; at
- *       most max_paths paths in flight, 0 = default 4M).
+ *       most max_paths paths in flight, 0 = default 16M).
  * Default 1. */
 int vt_set_render_mode(vt_context* ctx, int32_t mode, uint64_t max_paths);
+/* Wavefront traversal tuning (results are identical for every value):
+ * dda_budget   DDA steps a lane takes between refill checks (1..4096),
+ * refill_min   idle lanes of a 64-lane wave that trigger a refill (1..64),
+ * refill_chunk queue entries a wave reserves per atomic (64..65536, or -1
+ *              for exactly the idle lanes);
+ * 0 selects the default of each.  No counterpart in the reference (its
+ * traversal is one fragment-shader loop). */
+int vt_set_traversal_tuning(vt_context* ctx, int32_t dda_budget, int32_t refill_min, int32_t refill_chunk);
 
 /* ---- rendering ----------------------------------------------------------
  * Render `count` progressive passes: pass k uses accumulation index

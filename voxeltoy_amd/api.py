@@ -72,6 +72,7 @@ SIGNATURES = {
     "vt_reset_accum": (C.c_int, [C.c_void_p]),
     "vt_set_stats": (C.c_int, [C.c_void_p, C.c_int32]),
     "vt_set_render_mode": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64]),
+    "vt_set_traversal_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "vt_render": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
     "vt_read_accum": (C.c_int, [C.c_void_p, f32p]),
     "vt_read_sum64": (C.c_int, [C.c_void_p, f64p]),
@@ -304,6 +305,12 @@ class Context:
     def set_render_mode(self, mode: int, max_paths: int = 0):
         """0 = megakernel, 1 = wavefront (default); both give identical images."""
         _check(lib().vt_set_render_mode(self._c, mode, max_paths))
+
+    def set_traversal_tuning(self, dda_budget: int = 0, refill_min: int = 0, refill_chunk: int = 0):
+        """Wavefront traversal: DDA steps between refill checks, idle lanes
+        per wave that trigger a refill, queue entries reserved per atomic
+        (-1 = exactly the idle lanes); 0 = default.  Images are identical."""
+        _check(lib().vt_set_traversal_tuning(self._c, dda_budget, refill_min, refill_chunk))
 
     # -- rendering ----------------------------------------------------------
     def render(self, seed: int, n0: int, count: int, rng_frame_max: int = 0xFFFFFFFF):

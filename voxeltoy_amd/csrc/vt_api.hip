@@ -28,6 +28,12 @@ int fail(int code, const std::string& msg)
     return code;
 }
 
+/* Wavefront defaults (vt_set_render_mode, vt_set_traversal_tuning). */
+#define VT_DEFAULT_MAX_PATHS (16ull << 20)
+#define VT_DEFAULT_DDA_BUDGET 32
+#define VT_DEFAULT_REFILL_MIN 16
+#define VT_DEFAULT_REFILL_CHUNK 256
+
 #define VT_HIP(expr)                                                                      \
     do {                                                                                  \
         hipError_t e_ = (expr);                                                           \
@@ -82,7 +88,10 @@ struct vt_context {
     double* d_sum64 = nullptr;
 
     int render_mode = 1;
-    uint64_t max_paths = 4u << 20;
+    uint64_t max_paths = VT_DEFAULT_MAX_PATHS;
+    int32_t dda_budget = VT_DEFAULT_DDA_BUDGET;
+    int32_t refill_min = VT_DEFAULT_REFILL_MIN;
+    int32_t refill_chunk = VT_DEFAULT_REFILL_CHUNK;
     vt::WfPaths wf{};
     uint64_t wf_alloc = 0;           /* slots allocated in wf */
 
@@ -128,9 +137,7 @@ int upload_records(vt_context* c)
 /* Rebuild both occupancy levels from the slot bytes. */
 int refresh_occupancy(vt_context* c)
 {
-    VT_HIP(vt::launch_fill_u32(c->d_bits, c->bit_words, 0u, c->stream));
-    VT_HIP(vt::launch_brick_bits(c->dims, c->d_bricks, c->d_bits, c->stream));
-    VT_HIP(vt::launch_block_masks(c->dims, c->d_bricks, c->d_occ64, c->stream));
+    VT_HIP(vt::launch_block_masks(c->dims, c->d_bricks, c->d_occ64, c->d_bits, c->stream));
     return VT_OK;
 }
 
@@ -144,6 +151,8 @@ vt::BrickDims make_dims(int32_t nx, int32_t ny, int32_t nz)
     while (p < m) p <<= 1;
     d.bdim = p;
     d.n_bricks = (uint64_t)p * p * p;
+    d.kx = (nx + 3) / 4; d.ky = (ny + 3) / 4; d.kz = (nz + 3) / 4;
+    d.n_blocks = (uint64_t)d.kx * d.ky * d.kz;
     return d;
 }
 
@@ -183,8 +192,8 @@ int common_volume_setup(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const
     for (int i = 0; i < 256; ++i) c->slot_off[i] = -1;
     c->n_slots = 0;
     VT_HIP(hipMalloc(&c->d_bricks, d.n_bricks * 512u));
-    VT_HIP(hipMalloc(&c->d_occ64, d.n_bricks * 64u));
-    c->bit_words = (uint32_t)((d.n_bricks + 31) / 32);
+    VT_HIP(hipMalloc(&c->d_occ64, d.n_blocks * 8u));
+    c->bit_words = (uint32_t)((d.n_blocks + 31) / 32);
     VT_HIP(hipMalloc(&c->d_bits, (size_t)c->bit_words * 4u));
     VT_HIP(vt::launch_fill_u8(c->d_bricks, d.n_bricks * 512u, 0, c->stream));
     return VT_OK;
@@ -239,6 +248,8 @@ vt::GridView grid_view(const vt_context* c)
         g.ext[k] = 1.0f / (c->bmax[k] - c->bmin[k]);
     }
     g.bricks = c->d_bricks;
+    g.bx4 = (uint32_t)c->dims.kx;
+    g.by4 = (uint32_t)c->dims.ky;
     g.occ64 = c->d_occ64;
     g.bits = c->d_bits;
     g.bit_words = c->bit_words;
@@ -284,6 +295,9 @@ vt::TraceArgs trace_args(const vt_context* c)
     a.cdfv_lds_bytes = c->env.use_image ? ((c->env.cdf_v_n * 4 + 15) / 16) * 16 : 0;
     const size_t base = 256 * sizeof(vt::MatRec) + (size_t)a.cdfv_lds_bytes;
     a.lds_bits = (base + (size_t)c->bit_words * 4u <= c->lds_limit) ? 1 : 0;
+    a.dda_budget = c->dda_budget;
+    a.refill_min = c->refill_min;
+    a.refill_chunk = c->refill_chunk;
     return a;
 }
 
@@ -597,7 +611,20 @@ int vt_set_render_mode(vt_context* c, int32_t mode, uint64_t max_paths)
     if (mode != 0 && mode != 1) return fail(VT_ERR_ARG, "mode must be 0 (megakernel) or 1 (wavefront)");
     if (max_paths > (1ull << 31)) return fail(VT_ERR_LIMIT, "max_paths above 2^31");
     c->render_mode = mode;
-    c->max_paths = max_paths ? max_paths : (4ull << 20);
+    c->max_paths = max_paths ? max_paths : VT_DEFAULT_MAX_PATHS;
+    return VT_OK;
+}
+
+int vt_set_traversal_tuning(vt_context* c, int32_t dda_budget, int32_t refill_min, int32_t refill_chunk)
+{
+    if (!c) return fail(VT_ERR_ARG, "null context");
+    if (dda_budget < 0 || dda_budget > 4096) return fail(VT_ERR_ARG, "dda_budget must be in [0, 4096]");
+    if (refill_min < 0 || refill_min > 64) return fail(VT_ERR_ARG, "refill_min must be in [0, 64]");
+    if (refill_chunk != -1 && refill_chunk != 0 && (refill_chunk < 64 || refill_chunk > 65536))
+        return fail(VT_ERR_ARG, "refill_chunk must be -1, 0 or in [64, 65536]");
+    c->dda_budget = dda_budget ? dda_budget : VT_DEFAULT_DDA_BUDGET;
+    c->refill_min = refill_min ? refill_min : VT_DEFAULT_REFILL_MIN;
+    c->refill_chunk = refill_chunk == 0 ? VT_DEFAULT_REFILL_CHUNK : (refill_chunk < 0 ? 0 : refill_chunk);
     return VT_OK;
 }
 

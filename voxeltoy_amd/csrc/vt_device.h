@@ -6,15 +6,15 @@
  * byte per voxel (the material slot, 0 = empty).  Bricks are stored in Morton
  * order of their brick coordinates (over the power-of-two cube that covers
  * the grid) and voxels inside a brick in Morton order of their local
- * coordinates, so a 64-byte line holds a 4x4x4 block.  One occupancy bit per
- * brick (same Morton index) lets the traversal skip the voxel loads of empty
- * bricks; the bit array is staged in LDS when it fits.
+ * coordinates, so a 64-byte line holds a 4x4x4 block.  The slot bytes are
+ * read only at a hit.
  *
- * A second occupancy level holds one 64-bit mask per 4x4x4 block (in-brick
- * Morton order makes every 4x4x4 block one contiguous 64-byte run; the mask
- * bit of local voxel (x, y, z) is x + 4y + 16z).  The traversal tests voxels
- * against the mask it loaded on entering the block (one 8-byte load per
- * block, none in empty bricks) and reads the slot byte only at the hit.
+ * The traversal reads two occupancy levels over the 4x4x4 blocks, both in
+ * linear block order (bx + BX*(by + BY*bz), BX = ceil(nx/4), ...): one bit
+ * per block (set when any voxel of the block is occupied), staged in LDS when
+ * it fits, and one 64-bit voxel mask per block (bit x + 4y + 16z for local
+ * voxel (x, y, z)).  Entering a block costs one bit test; only non-empty
+ * blocks load their 8-byte mask.
  *
  * The traversal replays the reference 3D-DDA (src/shaders/shared/dda.h:7-100)
  * step by step: `dis` is a running FP32 sum, so empty space skips the loads
@@ -71,8 +71,9 @@ struct GridView {
     float rx, ry, rz;           /* resolution as float */
     float bmin[3], bmax[3], vs[3], ext[3];
     const uint8_t* bricks;      /* Morton bricks of 512 bytes */
-    const uint64_t* occ64;      /* per 4x4x4 block voxel-occupancy masks */
-    const uint32_t* bits;       /* brick occupancy bits (global copy) */
+    uint32_t bx4, by4;          /* 4x4x4 block grid extents in x and y */
+    const uint64_t* occ64;      /* per-block voxel masks, linear block order */
+    const uint32_t* bits;       /* per-block non-empty bits (global copy) */
     uint32_t bit_words;
     int32_t n_emissive;
     const int32_t* emissive;    /* linear voxel indices */
@@ -126,6 +127,12 @@ __device__ __forceinline__ uint32_t spread3_3(uint32_t v) { return (v & 1u) | ((
 __device__ __forceinline__ uint32_t local_morton(uint32_t x, uint32_t y, uint32_t z)
 {
     return spread3_3(x & 7u) | (spread3_3(y & 7u) << 1) | (spread3_3(z & 7u) << 2);
+}
+/* Linear index of the 4x4x4 block holding in-grid voxel (x, y, z); fits
+ * 24-bit multiplies since block extents are at most 1024. */
+__device__ __forceinline__ uint32_t block_index(const GridView& g, uint32_t x, uint32_t y, uint32_t z)
+{
+    return (x >> 2) + __umul24(g.bx4, (y >> 2) + __umul24(g.by4, z >> 2));
 }
 /* Bit of voxel (x, y, z) in its 4x4x4 block mask. */
 __device__ __forceinline__ uint32_t block_bit(uint32_t x, uint32_t y, uint32_t z)
@@ -182,24 +189,15 @@ __device__ __forceinline__ int dda(const GridView& g, const uint32_t* __restrict
     int32_t x = (int32_t)fx, y = (int32_t)fy, z = (int32_t)fz;
     int32_t steps = 0;
     int hit = 0;
-    uint32_t block_key = 0xffffffffu, brick_key = 0xffffffffu;
-    uint32_t bm = 0, brick_occ = 0;
+    uint32_t block = 0xffffffffu;
     uint64_t mask = 0;
     while (steps < g.max_steps) {
         if ((uint32_t)x >= (uint32_t)g.nx || (uint32_t)y >= (uint32_t)g.ny || (uint32_t)z >= (uint32_t)g.nz) break;
         if (COUNT) visits++;
-        const uint32_t bk = ((uint32_t)x >> 2) | (((uint32_t)y >> 2) << 11) | (((uint32_t)z >> 2) << 22);
-        if (bk != block_key) {
-            block_key = bk;
-            const uint32_t rk = (bk >> 1) & ((0x3ffu) | (0x3ffu << 11) | (0x3ffu << 22));
-            if (rk != brick_key) {
-                brick_key = rk;
-                bm = brick_morton((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3);
-                brick_occ = (bits[bm >> 5] >> (bm & 31u)) & 1u;
-            }
-            const uint32_t sb = (((uint32_t)x >> 2) & 1u) | ((((uint32_t)y >> 2) & 1u) << 1) |
-                                ((((uint32_t)z >> 2) & 1u) << 2);
-            mask = brick_occ ? g.occ64[(size_t)bm * 8u + sb] : 0ull;
+        const uint32_t bk = block_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+        if (bk != block) {
+            block = bk;
+            mask = ((bits[bk >> 5] >> (bk & 31u)) & 1u) ? g.occ64[bk] : 0ull;
         }
         if ((mask >> block_bit((uint32_t)x, (uint32_t)y, (uint32_t)z)) & 1ull) { hit = 1; break; }
         const bool mx = !(disy < disx) && !(disz < disx);
@@ -216,7 +214,9 @@ __device__ __forceinline__ int dda(const GridView& g, const uint32_t* __restrict
         steps++;
     }
     h.x = x; h.y = y; h.z = z; h.steps = steps;
-    h.slot = hit ? g.bricks[(size_t)bm * 512u + local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z)] : 0u;
+    h.slot = hit ? g.bricks[(size_t)brick_morton((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3) * 512u +
+                            local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z)]
+                 : 0u;
     return hit;
 }
 
@@ -229,7 +229,7 @@ struct DdaState {
     int32_t x, y, z, steps;
     float disx, disy, disz, ax, ay, az;
     int32_t stx, sty, stz;
-    uint32_t block_key, brick_key, bm, brick_occ;
+    uint32_t block;
     uint64_t mask;
 
     __device__ __forceinline__ bool init(const GridView& g, f3 o, f3 d)
@@ -255,8 +255,7 @@ struct DdaState {
         stz = sz > 0.0f ? 1 : (sz < 0.0f ? -1 : 0);
         x = (int32_t)fx; y = (int32_t)fy; z = (int32_t)fz;
         steps = 0;
-        block_key = brick_key = 0xffffffffu;
-        bm = brick_occ = 0u;
+        block = 0xffffffffu;
         mask = 0ull;
         return true;
     }
@@ -270,18 +269,10 @@ struct DdaState {
         while (steps < limit) {
             if ((uint32_t)x >= (uint32_t)g.nx || (uint32_t)y >= (uint32_t)g.ny || (uint32_t)z >= (uint32_t)g.nz) return 2;
             if (COUNT) visits++;
-            const uint32_t bk = ((uint32_t)x >> 2) | (((uint32_t)y >> 2) << 11) | (((uint32_t)z >> 2) << 22);
-            if (bk != block_key) {
-                block_key = bk;
-                const uint32_t rk = (bk >> 1) & ((0x3ffu) | (0x3ffu << 11) | (0x3ffu << 22));
-                if (rk != brick_key) {
-                    brick_key = rk;
-                    bm = brick_morton((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3);
-                    brick_occ = (bits[bm >> 5] >> (bm & 31u)) & 1u;
-                }
-                const uint32_t sb = (((uint32_t)x >> 2) & 1u) | ((((uint32_t)y >> 2) & 1u) << 1) |
-                                    ((((uint32_t)z >> 2) & 1u) << 2);
-                mask = brick_occ ? g.occ64[(size_t)bm * 8u + sb] : 0ull;
+            const uint32_t bk = block_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+            if (bk != block) {
+                block = bk;
+                mask = ((bits[bk >> 5] >> (bk & 31u)) & 1u) ? g.occ64[bk] : 0ull;
             }
             if ((mask >> block_bit((uint32_t)x, (uint32_t)y, (uint32_t)z)) & 1ull) return 1;
             const bool mx = !(disy < disx) && !(disz < disx);
@@ -303,7 +294,10 @@ struct DdaState {
     __device__ __forceinline__ int finish(const GridView& g, int status, Hit& h) const
     {
         h.x = x; h.y = y; h.z = z; h.steps = steps;
-        h.slot = status == 1 ? g.bricks[(size_t)bm * 512u + local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z)] : 0u;
+        h.slot = status == 1 ? g.bricks[(size_t)brick_morton((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3) *
+                                            512u +
+                                        local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z)]
+                             : 0u;
         if (status == 1) return 2;
         return y < 0 ? 3 : 0;
     }

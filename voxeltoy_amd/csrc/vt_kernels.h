@@ -24,6 +24,9 @@ struct TraceArgs {
     int32_t tiles_x, n_tiles;
     int32_t lds_bits;            /* stage occupancy bits in LDS */
     int32_t cdfv_lds_bytes;
+    int32_t dda_budget;          /* wavefront traversal: DDA steps between refill checks */
+    int32_t refill_min;          /* wavefront traversal: idle lanes that trigger a refill */
+    int32_t refill_chunk;        /* wavefront traversal: queue entries a wave reserves per atomic */
 };
 
 struct Counters {
@@ -89,6 +92,8 @@ struct BrickDims {
     int32_t bx, by, bz;          /* bricks per axis */
     int32_t bdim;                /* power-of-two cube edge in bricks (Morton range) */
     uint64_t n_bricks;           /* bdim^3 */
+    int32_t kx, ky, kz;          /* 4x4x4 blocks per axis (ceil(n/4)) */
+    uint64_t n_blocks;           /* kx*ky*kz */
 };
 
 /* Dense int32 offsets -> bricked slots via lut[offset] (offset in
@@ -99,10 +104,10 @@ hipError_t launch_brick_from_offsets(const BrickDims& d, const int32_t* dense, c
 /* Dense uint8 ids -> bricked slots via remap[256]. */
 hipError_t launch_brick_from_ids(const BrickDims& d, const uint8_t* dense, const uint8_t* remap, uint8_t* bricks,
                                  hipStream_t s);
-/* Recompute brick occupancy bits (bits must hold n_bricks/32 words). */
-hipError_t launch_brick_bits(const BrickDims& d, const uint8_t* bricks, uint32_t* bits, hipStream_t s);
-/* Per 4x4x4 block occupancy masks (occ64 holds n_bricks*8 words). */
-hipError_t launch_block_masks(const BrickDims& d, const uint8_t* bricks, uint64_t* occ64, hipStream_t s);
+/* Per 4x4x4 block voxel masks in linear block order (occ64 holds n_blocks
+ * words) and the per-block non-empty bits ((n_blocks+31)/32 words). */
+hipError_t launch_block_masks(const BrickDims& d, const uint8_t* bricks, uint64_t* occ64, uint32_t* bits,
+                              hipStream_t s);
 /* Bricked slots -> dense int32 offsets via slot_off[256]. */
 hipError_t launch_dense_from_bricks(const BrickDims& d, const uint8_t* bricks, const int32_t* slot_off, int32_t* dense,
                                     hipStream_t s);

@@ -291,44 +291,92 @@ __global__ __launch_bounds__(256) void render_kernel(TraceArgs a)
  * registers and keeps many waves resident; shading consumes the hits and
  * appends surviving paths to the next queue with one atomic per wave. */
 
-__device__ __forceinline__ void store_path(const WfPaths& P, uint32_t id, const PathState& S, uint32_t pix,
-                                           uint32_t pass)
+/* Path state in HBM, moved per phase.  Every path of a traversal round is in
+ * the same phase (round 0 PRIMARY, odd rounds SHADOW, even rounds CONT), so
+ * each shade kernel moves only the fields its phase reads and writes:
+ *   generate -> bo, d, meta, rng        (throughput 1 and radiance 0 implied)
+ *   PRIMARY  <- bo, d, meta, rng        -> o, dir, fac, nd, tgt, rng, rad, thr
+ *   SHADOW   <- o, nd, tgt, rad, thr, dir, fac   -> rad, thr
+ *   CONT     <- o, nd, thr, rad, meta, rng       -> o, dir, fac, nd, tgt, rng, bounces
+ * After SHADOW, basis_o = ray_o and ray_d = next_d, so CONT reads them from
+ * o and nd. */
+__device__ __forceinline__ void store_generated(const WfPaths& P, uint32_t id, const PathState& S, uint32_t pix,
+                                                uint32_t pass)
 {
-    P.o[id] = make_float4(S.ray_o.x, S.ray_o.y, S.ray_o.z, S.bsdf_pdf);
-    P.d[id] = make_float4(S.ray_d.x, S.ray_d.y, S.ray_d.z, 0.0f);
     P.bo[id] = make_float4(S.basis_o.x, S.basis_o.y, S.basis_o.z, 0.0f);
-    P.thr[id] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
-    P.rad[id] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
+    P.d[id] = make_float4(S.ray_d.x, S.ray_d.y, S.ray_d.z, 0.0f);
+    P.meta[id] = make_uint4(pix, pass, (uint32_t)S.bounces, 0u);
+    P.rng[id] = S.rng.state;
+}
+
+template <int PHASE>
+__device__ __forceinline__ void load_phase(const WfPaths& P, uint32_t id, PathState& S)
+{
+    S.phase = PHASE;
+    S.direct = S.factor = S.next_d = mk(0.0f, 0.0f, 0.0f);
+    S.ex = S.ey = S.ez = S.light_is_voxel = 0;
+    S.bsdf_pdf = 0.0f;
+    if (PHASE == PH_SHADOW) {
+        const float4 o = P.o[id], nd = P.nd[id], th = P.thr[id], rd = P.rad[id], di = P.dir[id], fa = P.fac[id];
+        const int4 tg = P.tgt[id];
+        S.ray_o = mk(o.x, o.y, o.z);
+        S.bsdf_pdf = o.w;
+        S.next_d = mk(nd.x, nd.y, nd.z);
+        S.basis_o = S.ray_o;
+        S.ray_d = S.next_d;
+        S.throughput = mk(th.x, th.y, th.z);
+        S.radiance = mk(rd.x, rd.y, rd.z);
+        S.direct = mk(di.x, di.y, di.z);
+        S.factor = mk(fa.x, fa.y, fa.z);
+        S.ex = tg.x; S.ey = tg.y; S.ez = tg.z; S.light_is_voxel = tg.w;
+        S.bounces = 0;
+        S.rng.state = 0; S.rng.inc = 1;
+        return;
+    }
+    const uint4 me = P.meta[id];
+    S.bounces = (int32_t)me.z;
+    S.rng.state = P.rng[id];
+    S.rng.inc = ((uint64_t)me.x << 1u) | 1u;
+    if (PHASE == PH_PRIMARY) {
+        const float4 bo = P.bo[id], d = P.d[id];
+        S.basis_o = mk(bo.x, bo.y, bo.z);
+        S.ray_d = mk(d.x, d.y, d.z);
+        S.ray_o = S.basis_o;
+        S.throughput = mk(1.0f, 1.0f, 1.0f);
+        S.radiance = mk(0.0f, 0.0f, 0.0f);
+    } else {
+        const float4 o = P.o[id], nd = P.nd[id], th = P.thr[id], rd = P.rad[id];
+        S.basis_o = mk(o.x, o.y, o.z);
+        S.ray_o = S.basis_o;
+        S.bsdf_pdf = o.w;
+        S.ray_d = mk(nd.x, nd.y, nd.z);
+        S.throughput = mk(th.x, th.y, th.z);
+        S.radiance = mk(rd.x, rd.y, rd.z);
+    }
+}
+
+/* Store after a shade step that continues the path (the next phase is
+ * SHADOW after PRIMARY/CONT, CONT after SHADOW). */
+template <int PHASE>
+__device__ __forceinline__ void store_phase(const WfPaths& P, uint32_t id, const PathState& S)
+{
+    if (PHASE == PH_SHADOW) {
+        P.rad[id] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
+        P.thr[id] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
+        return;
+    }
+    P.o[id] = make_float4(S.ray_o.x, S.ray_o.y, S.ray_o.z, S.bsdf_pdf);
     P.dir[id] = make_float4(S.direct.x, S.direct.y, S.direct.z, 0.0f);
     P.fac[id] = make_float4(S.factor.x, S.factor.y, S.factor.z, 0.0f);
     P.nd[id] = make_float4(S.next_d.x, S.next_d.y, S.next_d.z, 0.0f);
     P.tgt[id] = make_int4(S.ex, S.ey, S.ez, S.light_is_voxel);
-    P.meta[id] = make_uint4(pix, pass, (uint32_t)S.phase | ((uint32_t)S.bounces << 8), 0u);
     P.rng[id] = S.rng.state;
-}
-
-__device__ __forceinline__ void load_path(const WfPaths& P, uint32_t id, PathState& S, uint32_t& pix, uint32_t& pass)
-{
-    const float4 o = P.o[id], d = P.d[id], bo = P.bo[id], th = P.thr[id], rd = P.rad[id], di = P.dir[id],
-                 fa = P.fac[id], nd = P.nd[id];
-    const int4 tg = P.tgt[id];
-    const uint4 me = P.meta[id];
-    S.ray_o = mk(o.x, o.y, o.z);
-    S.bsdf_pdf = o.w;
-    S.ray_d = mk(d.x, d.y, d.z);
-    S.basis_o = mk(bo.x, bo.y, bo.z);
-    S.throughput = mk(th.x, th.y, th.z);
-    S.radiance = mk(rd.x, rd.y, rd.z);
-    S.direct = mk(di.x, di.y, di.z);
-    S.factor = mk(fa.x, fa.y, fa.z);
-    S.next_d = mk(nd.x, nd.y, nd.z);
-    S.ex = tg.x; S.ey = tg.y; S.ez = tg.z; S.light_is_voxel = tg.w;
-    pix = me.x;
-    pass = me.y;
-    S.phase = (int32_t)(me.z & 0xffu);
-    S.bounces = (int32_t)(me.z >> 8);
-    S.rng.state = P.rng[id];
-    S.rng.inc = ((uint64_t)pix << 1u) | 1u;
+    if (PHASE == PH_PRIMARY) {
+        P.rad[id] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
+        P.thr[id] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
+    } else {
+        P.meta[id].z = (uint32_t)S.bounces;
+    }
 }
 
 template <bool STATS>
@@ -348,32 +396,45 @@ __device__ __forceinline__ void flush_counters(const TraceArgs& a, const Counter
     if (threadIdx.x < 6 && red[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], red[threadIdx.x]);
 }
 
-/* Append the ray (o, d) of path `id` for the lanes with `keep` to a ray
- * queue (length counter *n), one atomic per wave; the queue stores the rays
- * themselves so the traversal reads them in queue order.  Every lane of the
- * wave must call it. */
-__device__ __forceinline__ void wave_append(bool keep, uint32_t id, f3 o, f3 d, float4* qa, float4* qb, uint32_t* n)
+/* Append the ray (o, d) of path `id` for the threads with `keep` to a ray
+ * queue (length counter *n); the queue stores the rays themselves so the
+ * traversal reads them in queue order.  One atomic per workgroup of NT
+ * threads: every queue counter is a single device address that all XCDs
+ * contend for.  Every thread of the workgroup must call it. */
+template <int NT>
+__device__ __forceinline__ void block_append(bool keep, uint32_t id, f3 o, f3 d, float4* qa, float4* qb, uint32_t* n)
 {
+    __shared__ uint32_t wsum[NT / 64];
+    __shared__ uint32_t bbase;
     const unsigned long long mask = __ballot(keep);
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t base = 0;
-    if (lane == 0 && mask) base = atomicAdd(n, (uint32_t)__popcll(mask));
-    base = __shfl(base, 0, 64);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) wsum[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < NT / 64; ++w) { const uint32_t c = wsum[w]; wsum[w] = t; t += c; }
+        bbase = t ? atomicAdd(n, t) : 0u;
+    }
+    __syncthreads();
     if (keep) {
-        const uint32_t j = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        const uint32_t j = bbase + wsum[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
         qa[j] = make_float4(o.x, o.y, o.z, __uint_as_float(id));
         qb[j] = make_float4(d.x, d.y, d.z, 0.0f);
     }
+    __syncthreads();
 }
 
+#define WF_GENERATE_THREADS 1024
+#define WF_SHADE_THREADS 512
+
 template <bool STATS>
-__global__ __launch_bounds__(256) void wf_generate_kernel(TraceArgs a, WfPaths P, uint64_t g0, uint32_t n)
+__global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceArgs a, WfPaths P, uint64_t g0,
+                                                                         uint32_t n)
 {
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
     const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
-    const uint32_t wave_base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (uint32_t base = wave_base; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t i = base + (threadIdx.x & 63u);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
         bool keep = false;
         f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
         if (i < n) {
@@ -384,7 +445,7 @@ __global__ __launch_bounds__(256) void wf_generate_kernel(TraceArgs a, WfPaths P
             const uint32_t rf = nn < a.rng_frame_max ? nn : a.rng_frame_max;
             PathState S;
             if (start_path<STATS>(a, px, py, rf, S, cnt)) {
-                store_path(P, i, S, pix, k);
+                store_generated(P, i, S, pix, k);
                 keep = true;
                 ro = S.ray_o;
                 rd = S.ray_d;
@@ -393,21 +454,23 @@ __global__ __launch_bounds__(256) void wf_generate_kernel(TraceArgs a, WfPaths P
                 P.smp[i] = make_float4(c.x, c.y, c.z, 1.0f);
             }
         }
-        wave_append(keep, i, ro, rd, P.qa0, P.qb0, &P.cnt[0]);
+        block_append<WF_GENERATE_THREADS>(keep, i, ro, rd, P.qa0, P.qb0, &P.cnt[0]);
     }
     flush_counters<STATS>(a, cnt);
 }
 
-/* Persistent traversal: each lane marches its ray up to DDA_BUDGET steps at
- * a time; when at least REFILL_MIN lanes of the wave are idle they take the
- * next queue entries together (one atomic per wave on the queue head, loads
- * in queue order), so lanes stay busy until the queue drains instead of
- * idling behind the longest ray of their wave. */
-#define DDA_BUDGET 16
-#define REFILL_MIN 16
+/* Persistent traversal: each lane marches its ray a.dda_budget steps at a
+ * time; whenever at least a.refill_min lanes of the wave are idle they take
+ * the next entries of the wave's private chunk of the queue, in queue order.
+ * A wave reserves a.refill_chunk entries per atomic on the queue head
+ * (0 = exactly the idle lanes), so few waves contend for the counter, and
+ * lanes stay busy until the queue drains instead of idling behind the
+ * longest ray of their wave.  Workgroups of 1024 threads share one LDS copy
+ * of the block occupancy bits. */
+#define WF_TRAVERSE_THREADS 1024
 
 template <bool STATS, bool LDSB>
-__global__ __launch_bounds__(256) void wf_traverse_kernel(TraceArgs a, WfPaths P, int sel)
+__global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceArgs a, WfPaths P, int sel)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     MatRec* recs;
@@ -421,21 +484,35 @@ __global__ __launch_bounds__(256) void wf_traverse_kernel(TraceArgs a, WfPaths P
     if (blockIdx.x == 0 && threadIdx.x == 0) P.cnt[sel ^ 1] = 0u;
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t refill_min = (uint32_t)a.refill_min;
+    const uint32_t chunk = (uint32_t)a.refill_chunk;
+    const int budget = a.dda_budget;
     DdaState D;
     uint32_t id = 0;
     bool has = false;
-    bool exhausted = false;
+    /* wave-uniform: the wave's reserved queue range [c_pos, c_end) */
+    uint32_t c_pos = 0, c_end = 0;
+    bool head_done = false;       /* the queue head has passed n */
     for (;;) {
         const unsigned long long idle = __ballot(!has);
         const uint32_t n_idle = (uint32_t)__popcll(idle);
-        if (!exhausted && (n_idle >= REFILL_MIN || n_idle == 64u)) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(head, n_idle);
-            base = __shfl(base, 0, 64);
-            if (base + n_idle >= n) exhausted = true;
+        const bool exhausted = head_done && c_pos >= c_end;
+        if (!exhausted && (n_idle >= refill_min || n_idle == 64u)) {
+            const uint32_t avail = c_end - c_pos;
+            const bool grab = avail < n_idle && !head_done;
+            const uint32_t take = chunk ? chunk : n_idle - avail;
+            uint32_t nb = 0;
+            if (grab) {
+                if (lane == 0) nb = atomicAdd(head, take);
+                nb = __shfl(nb, 0, 64);
+                if (nb + take >= n) head_done = true;
+            }
             if (!has) {
-                const uint32_t j = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (j < n) {
+                const uint32_t p = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                uint32_t j = 0xffffffffu;
+                if (p < avail) j = c_pos + p;
+                else if (grab && nb + (p - avail) < n) j = nb + (p - avail);
+                if (j != 0xffffffffu) {
                     const float4 o = qa[j], d = qb[j];
                     id = __float_as_uint(o.w);
                     if (D.init(a.g, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z))) {
@@ -445,13 +522,20 @@ __global__ __launch_bounds__(256) void wf_traverse_kernel(TraceArgs a, WfPaths P
                     }
                 }
             }
+            if (grab) {
+                c_end = nb + take < n ? nb + take : n;
+                c_pos = nb + (n_idle - avail);
+                if (c_pos > c_end) c_pos = c_end;
+            } else {
+                c_pos += n_idle < avail ? n_idle : avail;
+            }
         }
         if (!__ballot(has)) {
-            if (exhausted) break;
+            if (head_done && c_pos >= c_end) break;
             continue;
         }
         if (has) {
-            const int status = D.advance<STATS>(a.g, bits, DDA_BUDGET, cnt.visits);
+            const int status = D.advance<STATS>(a.g, bits, budget, cnt.visits);
             if (status) {
                 Hit h;
                 const int kind = D.finish(a.g, status, h);
@@ -463,31 +547,29 @@ __global__ __launch_bounds__(256) void wf_traverse_kernel(TraceArgs a, WfPaths P
     flush_counters<STATS>(a, cnt);
 }
 
-template <bool STATS, bool LDSB>
-__global__ __launch_bounds__(256) void wf_shade_kernel(TraceArgs a, WfPaths P, int sel)
+template <bool STATS, int PHASE>
+__global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a, WfPaths P, int sel)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     MatRec* recs;
     float* cdfv;
     const uint32_t* bits;
-    stage_lds<LDSB>(a, smem, recs, cdfv, bits);
+    stage_lds<false>(a, smem, recs, cdfv, bits);
     const float4* qa = sel ? P.qa1 : P.qa0;
     float4* qao = sel ? P.qa0 : P.qa1;
     float4* qbo = sel ? P.qb0 : P.qb1;
     const uint32_t n = P.cnt[sel];
     if (blockIdx.x == 0 && threadIdx.x == 0) P.cnt[2 + (sel ^ 1)] = 0u;
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
-    const uint32_t wave_base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (uint32_t base = wave_base; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t j = base + (threadIdx.x & 63u);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t j = base + threadIdx.x;
         bool keep = false;
         uint32_t id = 0;
         f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
         if (j < n) {
             id = __float_as_uint(qa[j].w);
             PathState S;
-            uint32_t pix, pass;
-            load_path(P, id, S, pix, pass);
+            load_phase<PHASE>(P, id, S);
             const int4 hv = P.hit[id];
             Hit h;
             h.x = hv.x; h.y = hv.y; h.z = hv.z; h.slot = (uint32_t)hv.w & 0xffu;
@@ -495,7 +577,7 @@ __global__ __launch_bounds__(256) void wf_shade_kernel(TraceArgs a, WfPaths P, i
             const int kind = hv.w >> 8;
             keep = shade_step<STATS>(a, recs, bits, cdfv, S, h, kind, cnt);
             if (keep) {
-                store_path(P, id, S, pix, pass);
+                store_phase<PHASE>(P, id, S);
                 ro = S.ray_o;
                 rd = S.ray_d;
             } else {
@@ -503,7 +585,7 @@ __global__ __launch_bounds__(256) void wf_shade_kernel(TraceArgs a, WfPaths P, i
                 P.smp[id] = make_float4(c.x, c.y, c.z, 1.0f);
             }
         }
-        wave_append(keep, id, ro, rd, qao, qbo, &P.cnt[sel ^ 1]);
+        block_append<WF_SHADE_THREADS>(keep, id, ro, rd, qao, qbo, &P.cnt[sel ^ 1]);
     }
     flush_counters<STATS>(a, cnt);
 }
@@ -543,15 +625,24 @@ __global__ void wf_reset_kernel(uint32_t* cnt)
     if (threadIdx.x < 4) cnt[threadIdx.x] = 0u;
 }
 
+/* Resident workgroups per CU of a kernel (at least 1). */
+template <typename K>
+static int blocks_per_cu(K kernel, int threads, size_t lds)
+{
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, threads, lds) != hipSuccess || b < 1) b = 1;
+    return b;
+}
+
 template <bool STATS, bool LDSB>
 static hipError_t wf_run(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s)
 {
-    const size_t lds = render_lds_bytes(a);
-    int bt = 0, bs = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bt, wf_traverse_kernel<STATS, LDSB>, 256, lds) != hipSuccess || bt < 1)
-        bt = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, wf_shade_kernel<STATS, LDSB>, 256, lds) != hipSuccess || bs < 1)
-        bs = 1;
+    const size_t lds_t = render_lds_bytes(a);
+    const size_t lds_s = 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
+    const int bt = blocks_per_cu(wf_traverse_kernel<STATS, LDSB>, WF_TRAVERSE_THREADS, lds_t);
+    const int bs0 = blocks_per_cu(wf_shade_kernel<STATS, PH_PRIMARY>, WF_SHADE_THREADS, lds_s);
+    const int bs1 = blocks_per_cu(wf_shade_kernel<STATS, PH_SHADOW>, WF_SHADE_THREADS, lds_s);
+    const int bs2 = blocks_per_cu(wf_shade_kernel<STATS, PH_CONT>, WF_SHADE_THREADS, lds_s);
     const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
     const uint64_t total = WH * a.count;
     const int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
@@ -559,12 +650,18 @@ static hipError_t wf_run(const TraceArgs& a, const WfPaths& P, int cus, hipStrea
         const uint64_t g1 = g0 + P.cap < total ? g0 + P.cap : total;
         const uint32_t n = (uint32_t)(g1 - g0);
         wf_reset_kernel<<<1, 64, 0, s>>>(P.cnt);
-        const unsigned gg = (unsigned)((n + 255) / 256 < (uint32_t)cus * 8u ? (n + 255) / 256 : (uint32_t)cus * 8u);
-        wf_generate_kernel<STATS><<<gg, 256, 0, s>>>(a, P, g0, n);
+        const uint32_t gn = (n + WF_GENERATE_THREADS - 1) / WF_GENERATE_THREADS;
+        const unsigned gg = (unsigned)(gn < (uint32_t)cus * 2u ? gn : (uint32_t)cus * 2u);
+        wf_generate_kernel<STATS><<<gg, WF_GENERATE_THREADS, 0, s>>>(a, P, g0, n);
         int sel = 0;
         for (int r = 0; r < rounds; ++r) {
-            wf_traverse_kernel<STATS, LDSB><<<cus * bt, 256, lds, s>>>(a, P, sel);
-            wf_shade_kernel<STATS, LDSB><<<cus * bs, 256, lds, s>>>(a, P, sel);
+            wf_traverse_kernel<STATS, LDSB><<<cus * bt, WF_TRAVERSE_THREADS, lds_t, s>>>(a, P, sel);
+            if (r == 0)
+                wf_shade_kernel<STATS, PH_PRIMARY><<<cus * bs0, WF_SHADE_THREADS, lds_s, s>>>(a, P, sel);
+            else if (r & 1)
+                wf_shade_kernel<STATS, PH_SHADOW><<<cus * bs1, WF_SHADE_THREADS, lds_s, s>>>(a, P, sel);
+            else
+                wf_shade_kernel<STATS, PH_CONT><<<cus * bs2, WF_SHADE_THREADS, lds_s, s>>>(a, P, sel);
             sel ^= 1;
         }
         const unsigned ga = (unsigned)((WH + 255) / 256 < (uint64_t)cus * 8u ? (WH + 255) / 256 : (uint64_t)cus * 8u);

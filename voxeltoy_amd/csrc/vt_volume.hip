@@ -50,31 +50,19 @@ __global__ __launch_bounds__(256) void brick_from_ids_kernel(BrickDims d, const 
     }
 }
 
-__global__ __launch_bounds__(256) void brick_bits_kernel(BrickDims d, const uint8_t* __restrict__ bricks,
-                                                         uint32_t* __restrict__ bits)
-{
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < d.n_bricks;
-         b += (uint64_t)gridDim.x * blockDim.x) {
-        const uint4* p = (const uint4*)(bricks + b * 512u);
-        uint32_t acc = 0;
-#pragma unroll 8
-        for (int k = 0; k < 32; ++k) {
-            const uint4 v = p[k];
-            acc |= v.x | v.y | v.z | v.w;
-        }
-        if (acc) atomicOr(&bits[b >> 5], 1u << (b & 31u));
-    }
-}
-
-/* One mask per 4x4x4 block: occ64[w] bit (x + 4y + 16z) is set when the
- * voxel at local (x, y, z) of the block is occupied; the block's 64 bytes
- * are bricks[w*64 ...] in Morton order. */
-__global__ __launch_bounds__(256) void block_masks_kernel(uint64_t n_words, const uint8_t* __restrict__ bricks,
+/* occ64[b] for block b = bx + kx*(by + ky*bz): bit (x + 4y + 16z) set when
+ * local voxel (x, y, z) of the block is occupied.  The block's 64 slot bytes
+ * are one Morton-ordered run inside its brick. */
+__global__ __launch_bounds__(256) void block_masks_kernel(BrickDims d, const uint8_t* __restrict__ bricks,
                                                           uint64_t* __restrict__ occ64)
 {
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_words;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < d.n_blocks;
          w += (uint64_t)gridDim.x * blockDim.x) {
-        const uint4* p = (const uint4*)(bricks + w * 64u);
+        const uint32_t bx = (uint32_t)(w % (uint64_t)d.kx);
+        const uint64_t r = w / (uint64_t)d.kx;
+        const uint32_t by = (uint32_t)(r % (uint64_t)d.ky), bz = (uint32_t)(r / (uint64_t)d.ky);
+        const uint32_t sb = (bx & 1u) | ((by & 1u) << 1) | ((bz & 1u) << 2);
+        const uint4* p = (const uint4*)(bricks + (uint64_t)brick_morton(bx >> 1, by >> 1, bz >> 1) * 512u + sb * 64u);
         uint64_t m = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -92,6 +80,20 @@ __global__ __launch_bounds__(256) void block_masks_kernel(uint64_t n_words, cons
                 }
         }
         occ64[w] = m;
+    }
+}
+
+/* bits[i] bit k = (occ64[32i + k] != 0). */
+__global__ __launch_bounds__(256) void block_bits_kernel(uint64_t n_blocks, const uint64_t* __restrict__ occ64,
+                                                         uint32_t* __restrict__ bits)
+{
+    const uint64_t n_words = (n_blocks + 31) / 32;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t v = 0;
+        for (uint32_t k = 0; k < 32u && i * 32u + k < n_blocks; ++k)
+            if (occ64[i * 32u + k]) v |= 1u << k;
+        bits[i] = v;
     }
 }
 
@@ -156,16 +158,11 @@ hipError_t launch_brick_from_ids(const BrickDims& d, const uint8_t* dense, const
     return hipGetLastError();
 }
 
-hipError_t launch_brick_bits(const BrickDims& d, const uint8_t* bricks, uint32_t* bits, hipStream_t s)
+hipError_t launch_block_masks(const BrickDims& d, const uint8_t* bricks, uint64_t* occ64, uint32_t* bits,
+                              hipStream_t s)
 {
-    brick_bits_kernel<<<grid_for(d.n_bricks), 256, 0, s>>>(d, bricks, bits);
-    return hipGetLastError();
-}
-
-hipError_t launch_block_masks(const BrickDims& d, const uint8_t* bricks, uint64_t* occ64, hipStream_t s)
-{
-    const uint64_t n_words = d.n_bricks * 8u;
-    block_masks_kernel<<<grid_for(n_words), 256, 0, s>>>(n_words, bricks, occ64);
+    block_masks_kernel<<<grid_for(d.n_blocks), 256, 0, s>>>(d, bricks, occ64);
+    block_bits_kernel<<<grid_for((d.n_blocks + 31) / 32), 256, 0, s>>>(d.n_blocks, occ64, bits);
     return hipGetLastError();
 }
 
