@@ -194,6 +194,9 @@ int vt_camera_matrices(const float* eye, const float* target, float fov_y, float
  * integral.  cdf_u/cdf_v may be NULL to query sizes. */
 int vt_env_build_cdf(const float* rgb, int32_t w, int32_t h, int32_t* cdf_u_w, int32_t* cdf_u_h,
                      float* cdf_u, int32_t* cdf_v_n, float* cdf_v, float* integral);
+/* computeMeshTransform (src/renderer/import.cpp:46-64): row-major 4x4 that
+ * fits the bounds into the unit cube with a one-voxel margin. */
+int vt_mesh_transform(const float* bmin, const float* bmax, const int32_t* res, float* M);
 /* pruneInteriorEmissiveVoxels (src/renderer/import.cpp:227-250); returns
  * the new count. */
 int32_t vt_prune_emissive(const int32_t* material_offsets, int32_t nx, int32_t ny, int32_t nz,

@@ -1,0 +1,346 @@
+"""Host side of the renderer object (no GPU): .vox and OBJ loading, volume
+bounds, camera parameters, settings and material edits, through the vtr_*
+C ABI with a device-less renderer (device -1).
+
+Pins: synthetic files written here from known grids (the loader must give
+the grid back with the reference's axis conversion), closed forms of the
+camera formulas (src/camera/cameraParameters.cpp), the MagicaVoxel default
+palette's documented structure, and invariants of the reference's own data
+fixtures (resources/scene_fall.vox, resources/bunny.obj, copied as data
+under tests/golden/).
+"""
+from __future__ import annotations
+
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from voxeltoy_amd import api
+from voxeltoy_amd import renderer as vr
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def write_vox(path, size, voxels, palette=None):
+    """Minimal MagicaVoxel 150 file: SIZE, XYZI and optional RGBA chunks.
+    voxels: iterable of (x, y, z, colour index) in file axes (z up)."""
+    def chunk(cid, content, children=b""):
+        return cid + struct.pack("<ii", len(content), len(children)) + content + children
+    sx, sy, sz = size
+    body = chunk(b"SIZE", struct.pack("<iii", sx, sy, sz))
+    vox = list(voxels)
+    body += chunk(b"XYZI", struct.pack("<i", len(vox)) + b"".join(struct.pack("<BBBB", *v) for v in vox))
+    if palette is not None:
+        body += chunk(b"RGBA", bytes(np.asarray(palette, np.uint8).reshape(-1)[:1024]))
+    data = b"VOX " + struct.pack("<i", 150) + chunk(b"MAIN", b"", body)
+    with open(path, "wb") as f:
+        f.write(data)
+
+
+def host_renderer():
+    r = vr.Renderer(device=-1)
+    r.initialize("")
+    return r
+
+
+def test_initial_state_matches_reference_constructor():
+    """Renderer::Renderer (renderer.cpp:41-65) + CameraParameters() defaults."""
+    r = host_renderer()
+    c = r.camera()
+    assert np.allclose(c["target"], 0)
+    # eye at distance 100 along the initial forward (+Z)
+    assert np.allclose(c["eye"], [0, 0, -100])
+    # focal length 50 mm on a 36 mm film: fov = 2 atan(18/50)
+    assert math.isclose(c["fov_y"], 2 * math.atan(18 / 50), rel_tol=1e-6)
+    assert math.isclose(c["focal_length"], 50.0, rel_tol=1e-5)
+    # f/16 -> lens radius = f / N / 2
+    assert math.isclose(c["lens_radius"], 50.0 / 16 / 2, rel_tol=1e-5)
+    assert c["near_d"] == pytest.approx(0.1) and c["far_d"] == 10000
+    s = r.render_settings()
+    assert (s.width, s.height, s.max_bounces, s.max_samples) == (512, 512, 1, 2048 * 2048)
+    off, bmin, bmax = r.volume()
+    assert off.shape == (16, 16, 16) and (off == -1).all()
+    assert np.allclose(bmin, -500) and np.allclose(bmax, 500)
+    # the focal-point service starts at "infinity" (renderer.cpp:713-715)
+    assert r.focal_distance() == pytest.approx(99999999.0)
+
+
+def test_camera_closed_forms():
+    r = host_renderer()
+    r.set_eye_target([0, 0, 0], [3, 0, 4])
+    r.set_focal_length(35.0)
+    c = r.camera()
+    assert math.isclose(c["fov_y"], 2 * math.atan(c["film"][1] / 2 / 35.0), rel_tol=1e-6)
+    r.set_fstop(2.0)
+    assert math.isclose(r.camera()["lens_radius"], 35.0 / 2.0 / 2, rel_tol=1e-5)
+    # film changes keep the fov (the focal length is re-evaluated afterwards)
+    fov = r.camera()["fov_y"]
+    r.set_film_size(36.0, 12.0)
+    assert math.isclose(r.camera()["fov_y"], fov, rel_tol=1e-5)
+    assert math.isclose(r.camera()["focal_length"], 12.0 / (2 * math.tan(fov / 2)), rel_tol=1e-5)
+    # orbit keeps the distance to the target; theta from +Y, phi from +X to +Z
+    r.set_eye_target([0, 0, -10], [1, 2, 3])
+    d0 = np.linalg.norm(np.array([1, 2, 3]) - np.array([0, 0, -10]))
+    r.orbit(math.radians(60), math.radians(30))
+    c = r.camera()
+    v = c["target"] - c["eye"]
+    assert math.isclose(np.linalg.norm(v), d0, rel_tol=1e-5)
+    sph = np.array([math.sin(math.radians(60)) * math.cos(math.radians(30)), math.cos(math.radians(60)),
+                    math.sin(math.radians(60)) * math.sin(math.radians(30))])
+    assert np.allclose(v / np.linalg.norm(v), sph, atol=1e-5)
+    # negative inputs clamp to zero (cameraParameters.cpp:151-175)
+    r.set_lens_radius(-3.0)
+    assert r.camera()["lens_radius"] == 0.0
+
+
+def test_focus_on_bounds_distance():
+    """CameraController::focusOnBounds: distance = |size| / (2 tan(fov/2))."""
+    r = host_renderer()
+    offs = -np.ones((8, 4, 16), np.int32)
+    r.load_volume(offs, np.zeros(0, np.float32))
+    r.focus_on_bounds()
+    c = r.camera()
+    _, bmin, bmax = r.volume()
+    centre = (bmin + bmax) / 2
+    want = np.linalg.norm(bmax - bmin) / (2 * math.tan(c["fov_y"] / 2))
+    assert np.allclose(c["target"], centre, atol=1e-4)
+    assert math.isclose(np.linalg.norm(c["eye"] - c["target"]), want, rel_tol=1e-5)
+
+
+def test_volume_bounds_are_cubic_voxels_longest_axis_1000():
+    r = host_renderer()
+    offs = -np.ones((10, 20, 40), np.int32)   # z, y, x
+    r.load_volume(offs, np.zeros(0, np.float32))
+    _, bmin, bmax = r.volume()
+    size = bmax - bmin
+    assert np.allclose(size, [1000, 500, 250])
+    assert np.allclose(bmin, -size / 2)
+
+
+@pytest.mark.parametrize("custom_palette", [False, True])
+def test_vox_roundtrip_axis_conversion(tmp_path, custom_palette):
+    rng = np.random.default_rng(3)
+    sx, sy, sz = 7, 5, 9            # file axes, z up
+    vox = []
+    grid = -np.ones((sy, sz, sx), np.int64)   # grid[z_grid=file y, y_grid=file z, x]
+    for _ in range(60):
+        x, y, z = int(rng.integers(sx)), int(rng.integers(sy)), int(rng.integers(sz))
+        c = int(rng.choice([1, 2, 7, 100, 216, 254]))
+        vox.append((x, y, z, c))
+    pal = rng.integers(0, 256, size=(256, 4), dtype=np.uint8)
+    write_vox(tmp_path / "t.vox", (sx, sy, sz), vox, pal[:255] if custom_palette else None)
+    # expected: later voxels overwrite earlier ones at the same position; 254 = empty (skipped)
+    colour = {}
+    for x, y, z, c in vox:
+        if c == 254:
+            continue
+        colour[(x, y, z)] = c
+    r = host_renderer()
+    r.load_vox_file(str(tmp_path / "t.vox"))
+    off, bmin, bmax = r.volume()
+    assert off.shape == (sy, sz, sx)       # (z, y, x) of the grid = (file y, file z, file x)
+    mat = r.material_data()
+    used = set(colour.values())
+    assert mat.size == 7 * len(used)
+    ref_pal = np.zeros((256, 4), np.uint8)
+    if custom_palette:
+        ref_pal[1:] = pal[:255]              # file colour k is palette entry k + 1
+    else:
+        ref_pal = default_palette()
+    for (x, y, z), c in colour.items():
+        o = off[y, z, x]
+        assert o >= 0 and o % 7 == 0
+        rec = mat[o:o + 7]
+        assert rec[0] == 0 and (rec[1:4] == 0).all()
+        assert np.array_equal(rec[4:7], ref_pal[c, :3].astype(np.float32) / np.float32(255))
+    assert (off >= 0).sum() == len(colour)
+    assert r.emissive().size == 0
+    # camera distance = half the bounds diagonal (import.cpp:40)
+    c = r.camera()
+    assert math.isclose(np.linalg.norm(c["eye"] - c["target"]), 0.5 * np.linalg.norm(bmax - bmin), rel_tol=1e-5)
+
+
+def default_palette():
+    """MagicaVoxel's default palette by its documented structure."""
+    L = [0xff, 0xcc, 0x99, 0x66, 0x33, 0x00]
+    R = [0xee, 0xdd, 0xbb, 0xaa, 0x88, 0x77, 0x55, 0x44, 0x22, 0x11]
+    p = np.zeros((256, 4), np.uint8)
+    for i in range(1, 256):
+        if i <= 215:
+            k = i - 1
+            p[i] = (L[k // 36], L[(k // 6) % 6], L[k % 6], 255)
+        else:
+            j = i - 216
+            v = R[j % 10]
+            p[i] = [(v, 0, 0, 255), (0, v, 0, 255), (0, 0, v, 255), (v, v, v, 255)][j // 10]
+    return p
+
+
+def test_default_palette_known_entries():
+    p = default_palette()
+    assert tuple(p[1]) == (255, 255, 255, 255)
+    assert tuple(p[2]) == (255, 255, 0xcc, 255)
+    assert tuple(p[215]) == (0, 0, 0x33, 255)
+    assert tuple(p[216]) == (0xee, 0, 0, 255)
+    assert tuple(p[255]) == (0x11, 0x11, 0x11, 255)
+
+
+def test_vox_errors_are_reported(tmp_path):
+    r = host_renderer()
+    with pytest.raises(vr.RendererError):
+        r.load_vox_file(str(tmp_path / "missing.vox"))
+    (tmp_path / "bad.vox").write_bytes(b"NOPE" + b"\0" * 32)
+    with pytest.raises(vr.RendererError, match="magic"):
+        r.load_vox_file(str(tmp_path / "bad.vox"))
+    (tmp_path / "v.vox").write_bytes(b"VOX " + struct.pack("<i", 151) + b"\0" * 12)
+    with pytest.raises(vr.RendererError, match="version"):
+        r.load_vox_file(str(tmp_path / "v.vox"))
+    # a failed load leaves the previous volume in place
+    off, _, _ = r.volume()
+    assert off.shape == (16, 16, 16)
+
+
+def parse_vox(path):
+    """Independent reading of the fixture: size, voxels, palette chunk."""
+    b = open(path, "rb").read()
+    assert b[:4] == b"VOX "
+    pos = 8 + 12
+    size = vox = pal = None
+    while pos < len(b):
+        cid = b[pos:pos + 4]
+        n, m = struct.unpack("<ii", b[pos + 4:pos + 12])
+        c = b[pos + 12:pos + 12 + n]
+        if cid == b"SIZE":
+            size = struct.unpack("<iii", c[:12])
+        elif cid == b"XYZI":
+            cnt = struct.unpack("<i", c[:4])[0]
+            vox = np.frombuffer(c[4:4 + 4 * cnt], np.uint8).reshape(-1, 4)
+        elif cid == b"RGBA":
+            pal = np.zeros((256, 4), np.uint8)
+            pal[1:] = np.frombuffer(c[:1020], np.uint8).reshape(255, 4)
+        pos += 12 + n + m
+    return size, vox, pal
+
+
+def test_vox_reference_fixture_scene_fall():
+    path = os.path.join(GOLDEN, "scene_fall.vox")
+    size, vox, pal = parse_vox(path)
+    r = host_renderer()
+    r.load_vox_file(path)
+    off, _, _ = r.volume()
+    sx, sy, sz = size
+    assert off.shape == (sy, sz, sx)
+    pal = default_palette() if pal is None else pal
+    keep = vox[vox[:, 3] != 254]
+    lin = {}
+    for x, y, z, c in keep:
+        lin[(int(x), int(y), int(z))] = int(c)
+    assert (off >= 0).sum() == len(lin)
+    mat = r.material_data()
+    assert mat.size == 7 * len(set(lin.values()))
+    for (x, y, z), c in list(lin.items())[::97]:
+        o = off[y, z, x]
+        assert np.array_equal(mat[o + 4:o + 7], pal[c, :3].astype(np.float32) / np.float32(255))
+
+
+def write_obj(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def test_obj_fans_negative_indices_groups(tmp_path):
+    write_obj(tmp_path / "m.obj", """# test
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+v 9 9 9
+vt 0 0
+vn 0 0 1
+g quad
+f 1/1/1 2/1/1 3/1/1 4/1/1
+g tri
+f -5//1 -3//1 -2//1
+""")
+    v, t, bmin, bmax = vr.load_obj(tmp_path / "m.obj")
+    # quad -> fan (0,1,2), (0,2,3); the unused vertex 5 is dropped; the second
+    # group re-indexes its own positions in order of first use
+    assert t.shape == (3, 3)
+    tris = [tuple(map(tuple, v[list(tr)])) for tr in t]
+    assert tris[0] == ((0, 0, 0), (1, 0, 0), (1, 1, 0))
+    assert tris[1] == ((0, 0, 0), (1, 1, 0), (0, 1, 0))
+    assert tris[2] == ((0, 0, 0), (1, 1, 0), (0, 1, 0))
+    assert v.shape == (7, 3)
+    assert np.array_equal(bmin, [0, 0, 0]) and np.array_equal(bmax, [1, 1, 0])
+
+
+def test_obj_bad_index_and_missing_file(tmp_path):
+    write_obj(tmp_path / "b.obj", "v 0 0 0\nv 1 0 0\nf 1 2 3\n")
+    with pytest.raises(vr.RendererError):
+        vr.load_obj(tmp_path / "b.obj")
+    with pytest.raises(vr.RendererError):
+        vr.load_obj(tmp_path / "missing.obj")
+
+
+def test_obj_reference_fixture_bunny():
+    path = os.path.join(GOLDEN, "bunny.obj")
+    lines = open(path).read().splitlines()
+    pos = np.array([[float(x) for x in l.split()[1:4]] for l in lines if l.startswith("v ")], np.float32)
+    faces = [l.split()[1:] for l in lines if l.startswith("f ")]
+    v, t, bmin, bmax = vr.load_obj(path)
+    assert t.shape[0] == sum(len(f) - 2 for f in faces)
+    used = sorted({int(c.split("/")[0]) - 1 for f in faces for c in f})
+    assert v.shape[0] == len(used)
+    assert np.array_equal(bmin, pos[used].min(0)) and np.array_equal(bmax, pos[used].max(0))
+    # the triangles are the file's faces (as position triples)
+    first = [int(c.split("/")[0]) - 1 for c in faces[0]]
+    assert np.array_equal(v[t[0]], pos[first[:3]])
+
+
+def test_mesh_transform_closed_form():
+    bmin, bmax, res = np.array([-1, 2, 0], np.float32), np.array([3, 4, 1], np.float32), [64, 64, 64]
+    M = api.mesh_transform(bmin, bmax, res)
+    s = (1 - 2.0 / 64) / 4.0        # major axis x, size 4
+    corner = M @ np.array([*bmin, 1], np.float32)
+    assert np.allclose(np.diag(M)[:3], s) and M[3, 3] == 1
+    assert np.allclose(corner[:3], 1.0 / 64, atol=1e-6)   # one-voxel margin
+    far = M @ np.array([*bmax, 1], np.float32)
+    assert math.isclose(far[0], 1 - 1.0 / 64, rel_tol=1e-6)
+
+
+def test_settings_roundtrip_and_validation():
+    r = host_renderer()
+    s = r.render_settings()
+    s.max_bounces = 5
+    s.exposure = 2.0
+    s.background_rotation_degrees = 30
+    r.update_render_settings(s)
+    t = r.render_settings()
+    assert (t.max_bounces, t.exposure, t.background_rotation_degrees) == (5, 2.0, 30)
+    s.gamma = 0.0
+    with pytest.raises(vr.RendererError):
+        r.update_render_settings(s)
+
+
+def test_material_edits_and_serialization(tmp_path):
+    r = host_renderer()
+    mat = np.array([0, 0, 0, 0, .5, .5, .5, 1, 0, 0, 0, .9, .8, .7, 20], np.float32)
+    offs = -np.ones((4, 4, 4), np.int32)
+    offs[0, 0, 0] = 0
+    offs[1, 1, 1] = 7
+    r.load_volume(offs, mat)
+    r.update_material_color(4, [0.1, 0.2, 0.3])
+    r.update_material_value(14, 55.0)
+    m = r.material_data()
+    assert np.allclose(m[4:7], [0.1, 0.2, 0.3]) and m[14] == 55.0
+    with pytest.raises(vr.RendererError):
+        r.update_material_color(13, [1, 1, 1])   # past the end
+
+
+def test_render_without_device_fails_loudly():
+    r = host_renderer()
+    with pytest.raises(vr.RendererError):
+        r.render()

@@ -86,6 +86,7 @@ SIGNATURES = {
     "vt_env_build_cdf": (C.c_int, [f32p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                    C.c_void_p, C.POINTER(C.c_int32), C.c_void_p, C.POINTER(C.c_float)]),
     "vt_prune_emissive": (C.c_int32, [i32p, C.c_int32, C.c_int32, C.c_int32, i32p, C.c_int32]),
+    "vt_mesh_transform": (C.c_int, [f32p, f32p, i32p, f32p]),
     "vt_device_math": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f32p, f32p, f32p]),
     "vt_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
 }
@@ -150,6 +151,14 @@ def env_build_cdf(rgb: np.ndarray):
     _check(lib().vt_env_build_cdf(rgb.reshape(-1), w, h, C.byref(cuw), C.byref(cuh), _ptr(cu), C.byref(cvn),
                                   _ptr(cv), C.byref(integ)))
     return cu.reshape(cuh.value, cuw.value), cv, integ.value
+
+
+def mesh_transform(bmin, bmax, res) -> np.ndarray:
+    """computeMeshTransform (src/renderer/import.cpp:46-64), row-major 4x4."""
+    M = np.zeros(16, np.float32)
+    _check(lib().vt_mesh_transform(np.asarray(bmin, np.float32), np.asarray(bmax, np.float32),
+                                   np.asarray(res, np.int32), M))
+    return M.reshape(4, 4)
 
 
 def prune_emissive(offsets: np.ndarray, emissive: np.ndarray) -> np.ndarray:

@@ -22,6 +22,7 @@ HIP_SOURCES = ["csrc/vt_trace.hip", "csrc/vt_volume.hip", "csrc/vt_api.hip"]
 CXX_SOURCES = ["host/vt_host_algos.cpp", "host/camera.cpp", "host/scene_io.cpp", "host/renderer.cpp"]
 HEADERS = ["csrc/vtmath.h", "csrc/vt_device.h", "csrc/vt_kernels.h", "host/vt_host_algos.h",
            "host/camera.h", "host/scene_io.h", "host/renderer.h", "host/vt_math_types.h"]
+API_HEADERS = ["vt_api.h", "vt_renderer.h"]
 
 ARCH = os.environ.get("VT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -53,7 +54,7 @@ def build(force: bool = True, jobs: int = 8, verbose: bool = False) -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
     existing_cxx = [s for s in CXX_SOURCES if os.path.exists(os.path.join(PKG, s))]
     hdr_mtime = max(os.path.getmtime(os.path.join(PKG, h)) for h in HEADERS if os.path.exists(os.path.join(PKG, h)))
-    hdr_mtime = max(hdr_mtime, os.path.getmtime(os.path.join(ROOT, "include", "vt_api.h")))
+    hdr_mtime = max([hdr_mtime] + [os.path.getmtime(os.path.join(ROOT, "include", h)) for h in API_HEADERS])
     cmds = []
     for s in HIP_SOURCES:
         src, obj = os.path.join(PKG, s), _obj(s)

@@ -819,6 +819,14 @@ int vt_env_build_cdf(const float* rgb, int32_t w, int32_t h, int32_t* cu_w, int3
     return VT_OK;
 }
 
+int vt_mesh_transform(const float* bmin, const float* bmax, const int32_t* res, float* M)
+{
+    if (!bmin || !bmax || !res || !M) return fail(VT_ERR_ARG, "null argument");
+    if (res[0] <= 0 || res[1] <= 0 || res[2] <= 0) return fail(VT_ERR_ARG, "resolution must be positive");
+    vt::host::mesh_transform(bmin, bmax, res, M);
+    return VT_OK;
+}
+
 int32_t vt_prune_emissive(const int32_t* offsets, int32_t nx, int32_t ny, int32_t nz, int32_t* emissive, int32_t n)
 {
     if (!offsets || (n > 0 && !emissive)) return fail(VT_ERR_ARG, "bad argument");
