@@ -84,6 +84,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--mode", type=int, default=1, help="0 megakernel, 1 wavefront (default)")
+    ap.add_argument("--workload", default="full", choices=["full", "batch"],
+                    help="full: the 256^3 full-shading scene (default); batch: 64 mixed scenes, one batched "
+                         "graph launch against 64 sequential renders")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the path shards by "
@@ -91,6 +94,9 @@ def main():
         sys.exit(2)
 
     from voxeltoy_amd import api, workloads
+
+    if args.workload == "batch":
+        return bench_batch(args, api, workloads)
 
     wl = workloads.full()
     ctx = api.Context(0)
@@ -157,6 +163,76 @@ def main():
     }
     print(json.dumps(out))
     ctx.close()
+
+
+def bench_batch(args, api, workloads):
+    """64 scenes (grid 32^3-512^3, image 128^2-1024^2, 1-64 spp, 5-60%
+    occupancy, 4 bounces): one batched wavefront launch replayed from a HIP
+    graph, against the same 64 renders launched one scene after another."""
+    wls = workloads.batch(64)
+    ctxs = []
+    for wl in wls:
+        c = api.Context(0)
+        workloads.load(c, wl)
+        ctxs.append(c)
+    seeds = [wl.seed for wl in wls]
+    counts = [wl.spp for wl in wls]
+    b = api.Batch(ctxs)
+    b.render(seeds, counts, stats=True)
+    st = b.stats()
+    rays = st["rays"]
+
+    def batched():
+        for c in ctxs:
+            c.reset_accum()
+        b.render(seeds, counts, graph=True)
+
+    def sequential():
+        for c, wl in zip(ctxs, wls):
+            c.reset_accum()
+            c.render(wl.seed, 0, wl.spp)
+        for c in ctxs:
+            c.synchronize()
+
+    for _ in range(args.warmup):
+        batched()
+        sequential()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batched()
+    t_b = (time.perf_counter() - t0) / args.steps
+    k_ms = b.stats()["kernel_ms"]
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sequential()
+    t_s = (time.perf_counter() - t0) / args.steps
+    out = {
+        "metric": METRIC,
+        "value": round(rays / t_b / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_b * 1000.0, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batch of 64 scenes: grid 32^3-512^3, image 128^2-1024^2, 1-64 spp, "
+                               "5-60% occupancy, 70/20/10 lambert/metal/emissive, gradient env, 4 bounces",
+                   "scenes": len(wls), "rays_per_step": int(rays), "samples_per_step": int(st["samples"]),
+                   "batched_graph_ms": round(t_b * 1000.0, 3), "batched_kernel_ms": round(k_ms, 3),
+                   "sequential_ms": round(t_s * 1000.0, 3),
+                   "sequential_mrays_per_s": round(rays / t_s / 1e6, 2),
+                   "speedup_vs_sequential": round(t_s / t_b, 3)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    b.close()
+    for c in ctxs:
+        c.close()
 
 
 if __name__ == "__main__":

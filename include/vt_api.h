@@ -161,6 +161,40 @@ int vt_read_accum(vt_context* ctx, float* rgba);
 int vt_read_sum64(vt_context* ctx, double* rgb);
 int vt_get_stats(vt_context* ctx, vt_stats* out);
 
+/* ---- batches -------------------------------------------------------------
+ * Many scenes (contexts on one device, each with its own volume, camera,
+ * environment, settings and frame) rendered by one wavefront launch whose
+ * queues hold the paths of every scene; optionally recorded once as a HIP
+ * graph and replayed.  Each context's accumulation afterwards equals what
+ * vt_render(ctx, seeds[i], n0[i], counts[i], rng_frame_max) gives.  The
+ * traversal tuning and path capacity (vt_set_render_mode max_paths) of
+ * ctxs[0] apply.  No counterpart in the reference (one scene per GL
+ * context); the contexts must outlive the batch. */
+typedef struct vt_batch vt_batch;
+int vt_batch_create(vt_context* const* ctxs, int32_t n, vt_batch** out);
+/* flags: bit 0 = launch through a HIP graph (recorded on first use and
+ * re-recorded when the batch's shape changes), bit 1 = collect stats. */
+int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, const uint32_t* counts,
+                    uint32_t rng_frame_max, int32_t flags);
+/* Counters of the last batch render (summed over scenes) and its time. */
+int vt_batch_get_stats(vt_batch* b, vt_stats* out);
+void vt_batch_destroy(vt_batch* b);
+
+/* ---- autotuning ------------------------------------------------------------
+ * Times `passes`-pass renders of the context's current scene under each
+ * kernel organisation and traversal tuning (both render modes, step budget,
+ * refill threshold, queue chunk), applies the fastest and reports it.  Every
+ * candidate renders the identical image; the accumulation is restored
+ * afterwards.  No counterpart in the reference. */
+typedef struct vt_tuning {
+    int32_t render_mode;
+    int32_t dda_budget, refill_min, refill_chunk;
+    uint64_t max_paths;
+    double ms;              /* time of the chosen candidate */
+    int32_t candidates;     /* number timed */
+} vt_tuning;
+int vt_autotune(vt_context* ctx, uint32_t passes, vt_tuning* out);
+
 /* ---- probes -------------------------------------------------------------
  * First hit of every pixel's primary ray of frame `frame`: rec[8] per
  * pixel = kind (0 AABB miss, 1 traversal miss, 2 voxel, 3 ground), voxel

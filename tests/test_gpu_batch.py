@@ -1,0 +1,134 @@
+"""GPU: batched multi-scene rendering and the autotuner.  A batch must give
+every scene exactly the image of its own single-scene render under the same
+PCG streams (bit for bit), with and without the recorded HIP graph and on
+graph replays with new seeds; the autotuner may only change speed."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_scenes():
+    """Six small scenes of different sizes, resolutions, spp, bounces,
+    environments and lens models."""
+    from voxeltoy_amd import scenes, workloads
+    out = []
+    specs = [(16, 24, 16, 3, 1, 0), (32, 40, 32, 2, 2, 1), (24, 16, 48, 1, 0, 0), (48, 64, 40, 2, 3, 2),
+             (20, 32, 32, 4, 1, 1), (40, 48, 24, 1, 2, 0)]
+    for i, (n, W, H, spp, bounces, lens) in enumerate(specs):
+        sc = scenes.small_scene(10 + i, n=n)
+        bmin, bmax = workloads.reference_bounds(sc.shape)
+        cam = scenes.orbit_camera(bmin, bmax, math.radians(110 + 5 * i), math.radians(30 * i), W, H,
+                                  lens_model=lens, lens_radius=25.0 if lens == 1 else 0.0,
+                                  focal_distance=1200.0)
+        wl = workloads.Workload(f"s{i}", sc, cam, W, H, spp=spp, bounces=bounces, seed=100 + i, bmin=bmin,
+                                bmax=bmax, env_top=(1.5, 1.4, 1.2), env_bottom=(0.3, 0.3, 0.35))
+        if i == 3:
+            img = scenes.sky_environment(7, 64, 32, sun_peak=50.0, sun_exponent=200.0)
+            from voxeltoy_amd import api
+            wl.env_image, wl.cdf = img, api.env_build_cdf(img)
+        out.append(wl)
+    return out
+
+
+def singles(api, workloads, wls, seeds, n0):
+    imgs = []
+    for wl, sd, z in zip(wls, seeds, n0):
+        c = api.Context(0)
+        workloads.load(c, wl)
+        c.render(sd, z, wl.spp)
+        imgs.append(c.read_accum())
+        c.close()
+    return imgs
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_equals_single_scene_renders(graph):
+    from voxeltoy_amd import api, workloads
+    wls = mixed_scenes()
+    seeds = [wl.seed for wl in wls]
+    n0 = [0, 0, 2, 0, 1, 0]
+    ref = singles(api, workloads, wls, seeds, n0)
+    ctxs = []
+    for wl in wls:
+        c = api.Context(0)
+        workloads.load(c, wl)
+        ctxs.append(c)
+    b = api.Batch(ctxs)
+    b.render(seeds, [wl.spp for wl in wls], n0=n0, graph=graph)
+    for i, c in enumerate(ctxs):
+        got = c.read_accum()
+        assert np.array_equal(got.view(np.uint32), ref[i].view(np.uint32)), f"scene {i} differs"
+    if graph:
+        # replay of the recorded graph with new seeds (same shape) after clearing
+        seeds2 = [s + 7 for s in seeds]
+        for c in ctxs:
+            c.reset_accum()
+        b.render(seeds2, [wl.spp for wl in wls], n0=n0, graph=True)
+        ref2 = singles(api, workloads, wls, seeds2, n0)
+        for i, c in enumerate(ctxs):
+            assert np.array_equal(c.read_accum().view(np.uint32), ref2[i].view(np.uint32)), f"replay scene {i}"
+    b.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_batch_chunked_and_stats():
+    """Path capacity smaller than the batch: chunks cut across scenes."""
+    from voxeltoy_amd import api, workloads
+    wls = mixed_scenes()[:4]
+    seeds = [wl.seed for wl in wls]
+    ref = singles(api, workloads, wls, seeds, [0] * 4)
+    ctxs = []
+    for wl in wls:
+        c = api.Context(0)
+        workloads.load(c, wl)
+        ctxs.append(c)
+    ctxs[0].set_render_mode(1, 3001)
+    b = api.Batch(ctxs)
+    b.render(seeds, [wl.spp for wl in wls], stats=True)
+    for i, c in enumerate(ctxs):
+        assert np.array_equal(c.read_accum().view(np.uint32), ref[i].view(np.uint32)), f"scene {i} differs"
+    st = b.stats()
+    assert st["samples"] == sum(wl.width * wl.height * wl.spp for wl in wls)
+    assert st["rays"] >= st["samples"] and st["kernel_ms"] > 0
+    b.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_batch_rejects_bad_input():
+    from voxeltoy_amd import api
+    c = api.Context(0)
+    b = api.Batch([c])
+    with pytest.raises(api.VTError):
+        b.render([1], [1])          # no volume / frame yet
+    b.close()
+    with pytest.raises(api.VTError):
+        api.Batch([])
+    c.close()
+
+
+def test_autotune_keeps_image_and_accumulation():
+    from voxeltoy_amd import api, workloads
+    wl = mixed_scenes()[1]
+    c = api.Context(0)
+    workloads.load(c, wl)
+    c.render(wl.seed, 0, 1)
+    before = c.read_accum()
+    t = c.autotune(passes=2)
+    assert t["candidates"] >= 10 and t["render_mode"] in (0, 1) and t["ms"] > 0
+    assert np.array_equal(c.read_accum().view(np.uint32), before.view(np.uint32))
+    c.render(wl.seed, 1, 2)
+    tuned = c.read_accum()
+    d = api.Context(0)
+    workloads.load(d, wl)
+    d.render(wl.seed, 0, 1)
+    d.render(wl.seed, 1, 2)
+    assert np.array_equal(tuned.view(np.uint32), d.read_accum().view(np.uint32))
+    c.close()
+    d.close()

@@ -50,6 +50,15 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("render_mode", C.c_int32), ("dda_budget", C.c_int32), ("refill_min", C.c_int32),
+                ("refill_chunk", C.c_int32), ("max_paths", C.c_uint64), ("ms", C.c_double),
+                ("candidates", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # symbol -> (restype, argtypes)
 SIGNATURES = {
     "vt_last_error": (C.c_char_p, []),
@@ -73,6 +82,12 @@ SIGNATURES = {
     "vt_set_stats": (C.c_int, [C.c_void_p, C.c_int32]),
     "vt_set_render_mode": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64]),
     "vt_set_traversal_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "vt_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
+    "vt_batch_render": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                  C.c_uint32, C.c_int32]),
+    "vt_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "vt_batch_destroy": (None, [C.c_void_p]),
+    "vt_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Tuning)]),
     "vt_render": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
     "vt_read_accum": (C.c_int, [C.c_void_p, f32p]),
     "vt_read_sum64": (C.c_int, [C.c_void_p, f64p]),
@@ -321,6 +336,13 @@ class Context:
         (-1 = exactly the idle lanes); 0 = default.  Images are identical."""
         _check(lib().vt_set_traversal_tuning(self._c, dda_budget, refill_min, refill_chunk))
 
+    def autotune(self, passes: int = 1) -> dict:
+        """Time every kernel organisation / traversal tuning on `passes`
+        passes of the current scene, keep the fastest (images are identical)."""
+        t = Tuning()
+        _check(lib().vt_autotune(self._c, passes, C.byref(t)))
+        return t.as_dict()
+
     # -- rendering ----------------------------------------------------------
     def render(self, seed: int, n0: int, count: int, rng_frame_max: int = 0xFFFFFFFF):
         _check(lib().vt_render(self._c, seed, n0, count, rng_frame_max))
@@ -374,3 +396,38 @@ class Context:
         out = np.zeros_like(a)
         _check(lib().vt_device_math(self._c, fn, len(a), a, b, out))
         return out
+
+
+class Batch:
+    """Several contexts (scenes) rendered by one wavefront launch, optionally
+    through a recorded HIP graph.  The contexts must outlive the batch."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c._c.value for c in self.contexts])
+        self._b = C.c_void_p()
+        _check(lib().vt_batch_create(arr, len(self.contexts), C.byref(self._b)))
+
+    def render(self, seeds, counts, n0=None, rng_frame_max: int = 0xFFFFFFFF, graph: bool = False,
+               stats: bool = False):
+        n = len(self.contexts)
+        s = (C.c_uint64 * n)(*[int(x) for x in seeds])
+        k = (C.c_uint32 * n)(*[int(x) for x in counts])
+        z = None if n0 is None else (C.c_uint32 * n)(*[int(x) for x in n0])
+        _check(lib().vt_batch_render(self._b, s, z, k, rng_frame_max, (1 if graph else 0) | (2 if stats else 0)))
+
+    def stats(self) -> dict:
+        st = Stats()
+        _check(lib().vt_batch_get_stats(self._b, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._b:
+            lib().vt_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

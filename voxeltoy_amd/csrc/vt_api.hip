@@ -31,7 +31,7 @@ int fail(int code, const std::string& msg)
 /* Wavefront defaults (vt_set_render_mode, vt_set_traversal_tuning). */
 #define VT_DEFAULT_MAX_PATHS (16ull << 20)
 #define VT_DEFAULT_DDA_BUDGET 32
-#define VT_DEFAULT_REFILL_MIN 16
+#define VT_DEFAULT_REFILL_MIN 8
 #define VT_DEFAULT_REFILL_CHUNK 256
 
 #define VT_HIP(expr)                                                                      \
@@ -301,22 +301,22 @@ vt::TraceArgs trace_args(const vt_context* c)
     return a;
 }
 
-void free_wavefront(vt_context* c)
+void free_paths(vt::WfPaths& P, uint64_t& alloc)
 {
-    vt::WfPaths& P = c->wf;
     dfree(P.o); dfree(P.d); dfree(P.bo); dfree(P.thr); dfree(P.rad); dfree(P.dir); dfree(P.fac); dfree(P.nd);
     dfree(P.tgt); dfree(P.meta); dfree(P.rng); dfree(P.hit); dfree(P.smp);
     dfree(P.qa0); dfree(P.qb0); dfree(P.qa1); dfree(P.qb1); dfree(P.cnt);
     P.cap = 0;
-    c->wf_alloc = 0;
+    alloc = 0;
 }
 
+void free_wavefront(vt_context* c) { free_paths(c->wf, c->wf_alloc); }
+
 /* Path storage for `slots` paths (reallocated only when it grows). */
-int ensure_wavefront(vt_context* c, uint64_t slots)
+int alloc_paths(vt::WfPaths& P, uint64_t& alloc, uint64_t slots)
 {
-    if (c->wf_alloc >= slots) { c->wf.cap = slots; return VT_OK; }
-    free_wavefront(c);
-    vt::WfPaths& P = c->wf;
+    if (alloc >= slots) { P.cap = slots; return VT_OK; }
+    free_paths(P, alloc);
     const size_t n = (size_t)slots;
     VT_HIP(hipMalloc(&P.o, n * 16)); VT_HIP(hipMalloc(&P.d, n * 16)); VT_HIP(hipMalloc(&P.bo, n * 16));
     VT_HIP(hipMalloc(&P.thr, n * 16)); VT_HIP(hipMalloc(&P.rad, n * 16)); VT_HIP(hipMalloc(&P.dir, n * 16));
@@ -327,9 +327,11 @@ int ensure_wavefront(vt_context* c, uint64_t slots)
     VT_HIP(hipMalloc(&P.qa1, n * 16)); VT_HIP(hipMalloc(&P.qb1, n * 16));
     VT_HIP(hipMalloc(&P.cnt, 16));
     P.cap = slots;
-    c->wf_alloc = slots;
+    alloc = slots;
     return VT_OK;
 }
+
+int ensure_wavefront(vt_context* c, uint64_t slots) { return alloc_paths(c->wf, c->wf_alloc, slots); }
 
 }  // namespace
 
@@ -655,6 +657,236 @@ int vt_render(vt_context* c, uint64_t seed, uint32_t n0, uint32_t count, uint32_
     }
     c->timed = true;
     c->stats_valid = c->stats_enabled;
+    return VT_OK;
+}
+
+/* ---- batches ---------------------------------------------------------------- */
+struct vt_batch {
+    std::vector<vt_context*> ctxs;
+    vt_context* lead = nullptr;
+    std::vector<vt::TraceArgs> h_tab;
+    std::vector<uint64_t> h_base;         /* path prefix (n + 1), then pixel prefix (n + 1) */
+    vt::TraceArgs* d_tab = nullptr;
+    uint64_t* d_base = nullptr;
+    unsigned long long* d_stats = nullptr;
+    vt::WfPaths wf{};
+    uint64_t wf_alloc = 0;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> graph_key;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false, stats_valid = false;
+};
+
+int vt_batch_create(vt_context* const* ctxs, int32_t n, vt_batch** out)
+{
+    if (!ctxs || !out || n <= 0 || n > 65536) return fail(VT_ERR_ARG, "need 1..65536 contexts");
+    for (int32_t i = 0; i < n; ++i) {
+        if (!ctxs[i]) return fail(VT_ERR_ARG, "null context in batch");
+        if (ctxs[i]->device != ctxs[0]->device) return fail(VT_ERR_ARG, "batch contexts must share one device");
+    }
+    VT_HIP(hipSetDevice(ctxs[0]->device));
+    vt_batch* b = new vt_batch();
+    b->ctxs.assign(ctxs, ctxs + n);
+    b->lead = ctxs[0];
+    b->h_tab.resize((size_t)n);
+    b->h_base.resize(2 * ((size_t)n + 1));
+    if (hipMalloc(&b->d_tab, sizeof(vt::TraceArgs) * (size_t)n) != hipSuccess ||
+        hipMalloc(&b->d_base, sizeof(uint64_t) * b->h_base.size()) != hipSuccess ||
+        hipMalloc(&b->d_stats, 6 * sizeof(unsigned long long)) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        vt_batch_destroy(b);
+        return fail(VT_ERR_HIP, "batch allocation failed");
+    }
+    *out = b;
+    return VT_OK;
+}
+
+void vt_batch_destroy(vt_batch* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->lead->device);
+    if (b->lead->stream) (void)hipStreamSynchronize(b->lead->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    dfree(b->d_tab); dfree(b->d_base); dfree(b->d_stats);
+    free_paths(b->wf, b->wf_alloc);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    delete b;
+}
+
+int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, const uint32_t* counts,
+                    uint32_t rng_frame_max, int32_t flags)
+{
+    if (!b || !seeds || !counts) return fail(VT_ERR_ARG, "null argument");
+    const size_t n = b->ctxs.size();
+    vt_context* L = b->lead;
+    VT_HIP(hipSetDevice(L->device));
+    const bool stats = (flags & 2) != 0, graph = (flags & 1) != 0;
+    uint64_t* pb = b->h_base.data();
+    uint64_t* xb = pb + n + 1;
+    pb[0] = xb[0] = 0;
+    int max_bounces = 0;
+    for (size_t i = 0; i < n; ++i) {
+        vt_context* c = b->ctxs[i];
+        if (!c->has_volume) return fail(VT_ERR_STATE, "batch scene without a volume");
+        if (!c->d_accum) return fail(VT_ERR_STATE, "batch scene without a frame");
+        vt::TraceArgs a = trace_args(c);
+        a.seed = seeds[i];
+        a.n0 = n0 ? n0[i] : 0u;
+        a.count = counts[i];
+        a.rng_frame_max = rng_frame_max;
+        a.stats = stats ? b->d_stats : nullptr;
+        b->h_tab[i] = a;
+        const uint64_t WH = (uint64_t)c->W * (uint64_t)c->H;
+        pb[i + 1] = pb[i] + WH * counts[i];
+        xb[i + 1] = xb[i] + WH;
+        max_bounces = std::max(max_bounces, c->st.max_bounces);
+    }
+    const uint64_t total = pb[n], pixels = xb[n];
+    if (total == 0) return VT_OK;
+    /* uploads and clears still queued on the scenes' own streams */
+    for (vt_context* c : b->ctxs) VT_HIP(hipStreamSynchronize(c->stream));
+    if (total > (1ull << 40)) return fail(VT_ERR_LIMIT, "batch too large");
+    /* the previous launch may still read the table */
+    VT_HIP(hipStreamSynchronize(L->stream));
+    VT_HIP(hipMemcpy(b->d_tab, b->h_tab.data(), sizeof(vt::TraceArgs) * n, hipMemcpyHostToDevice));
+    VT_HIP(hipMemcpy(b->d_base, b->h_base.data(), sizeof(uint64_t) * b->h_base.size(), hipMemcpyHostToDevice));
+    int r = alloc_paths(b->wf, b->wf_alloc, std::min<uint64_t>(total, L->max_paths));
+    if (r != VT_OK) return r;
+    vt::BatchTable B;
+    B.tab = b->d_tab;
+    B.path_base = b->d_base;
+    B.pix_base = b->d_base + n + 1;
+    B.n = (uint32_t)n;
+    B.stats = stats ? b->d_stats : nullptr;
+    vt::TraceArgs a0 = trace_args(L);
+    if (stats) VT_HIP(vt::launch_reset_counter(L->d_tile, b->d_stats, L->stream));
+    if (graph) {
+        const std::vector<uint64_t> key = { (uint64_t)n, total, pixels, b->wf.cap, (uint64_t)max_bounces,
+                                            (uint64_t)stats, (uint64_t)a0.dda_budget, (uint64_t)a0.refill_min,
+                                            (uint64_t)a0.refill_chunk, (uint64_t)(uintptr_t)b->wf.o };
+        if (!b->exec || key != b->graph_key) {
+            if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+            hipGraph_t g = nullptr;
+            VT_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t le = vt::launch_render_batch(a0, B, b->wf, total, pixels, max_bounces, L->cus, L->stream,
+                                                          stats);
+            const hipError_t ce = hipStreamEndCapture(L->stream, &g);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(VT_ERR_HIP, std::string("batch graph capture failed: ") +
+                                            hipGetErrorString(le != hipSuccess ? le : ce));
+            }
+            const hipError_t ie = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                b->exec = nullptr;
+                return fail(VT_ERR_HIP, std::string("batch graph instantiation failed: ") + hipGetErrorString(ie));
+            }
+            b->graph_key = key;
+        }
+        VT_HIP(hipEventRecord(b->ev0, L->stream));
+        VT_HIP(hipGraphLaunch(b->exec, L->stream));
+        VT_HIP(hipEventRecord(b->ev1, L->stream));
+    } else {
+        VT_HIP(hipEventRecord(b->ev0, L->stream));
+        VT_HIP(vt::launch_render_batch(a0, B, b->wf, total, pixels, max_bounces, L->cus, L->stream, stats));
+        VT_HIP(hipEventRecord(b->ev1, L->stream));
+    }
+    /* each context's later reads and renders run on its own stream */
+    VT_HIP(hipStreamSynchronize(L->stream));
+    b->timed = true;
+    b->stats_valid = stats;
+    return VT_OK;
+}
+
+int vt_batch_get_stats(vt_batch* b, vt_stats* out)
+{
+    if (!b || !out) return fail(VT_ERR_ARG, "null argument");
+    VT_HIP(hipSetDevice(b->lead->device));
+    VT_HIP(hipStreamSynchronize(b->lead->stream));
+    std::memset(out, 0, sizeof(*out));
+    if (b->stats_valid) {
+        unsigned long long s[6];
+        VT_HIP(hipMemcpy(s, b->d_stats, sizeof(s), hipMemcpyDeviceToHost));
+        out->rays = s[0]; out->visits = s[1]; out->interactions = s[2];
+        out->samples = s[3]; out->env_samples = s[4]; out->env_evals = s[5];
+    }
+    if (b->timed) {
+        float ms = 0.0f;
+        VT_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        out->kernel_ms = ms;
+    }
+    return VT_OK;
+}
+
+/* ---- autotuning --------------------------------------------------------------- */
+int vt_autotune(vt_context* c, uint32_t passes, vt_tuning* out)
+{
+    if (!c || !out) return fail(VT_ERR_ARG, "null argument");
+    if (!c->has_volume || !c->d_accum) return fail(VT_ERR_STATE, "autotune needs a volume and a frame");
+    if (passes == 0) passes = 1;
+    VT_HIP(hipSetDevice(c->device));
+    const size_t acc_bytes = (size_t)c->W * c->H * sizeof(float4);
+    const size_t sum_bytes = c->d_sum64 ? (size_t)c->W * c->H * 3 * sizeof(double) : 0;
+    void* save_acc = nullptr;
+    void* save_sum = nullptr;
+    VT_HIP(hipMalloc(&save_acc, acc_bytes));
+    if (sum_bytes) VT_HIP(hipMalloc(&save_sum, sum_bytes));
+    VT_HIP(hipMemcpyAsync(save_acc, c->d_accum, acc_bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (sum_bytes) VT_HIP(hipMemcpyAsync(save_sum, c->d_sum64, sum_bytes, hipMemcpyDeviceToDevice, c->stream));
+    const bool was_stats = c->stats_enabled;
+    c->stats_enabled = false;
+    const int mode0 = c->render_mode, b0 = c->dda_budget, r0 = c->refill_min, k0 = c->refill_chunk;
+    struct Cand { int mode, budget, refill, chunk; };
+    std::vector<Cand> cands;
+    cands.push_back({ 0, b0, r0, k0 });
+    for (int budget : { 16, 32, 64 })
+        for (int refill : { 8, 16, 24 })
+            for (int chunk : { 128, 256, 512 }) cands.push_back({ 1, budget, refill, chunk });
+    double best = 1e30;
+    Cand pick = { mode0, b0, r0, k0 };
+    int rc = VT_OK;
+    for (size_t i = 0; i <= cands.size() && rc == VT_OK; ++i) {
+        const Cand& k = cands[i == 0 ? 1 : i - 1];   /* the first run warms up */
+        c->render_mode = k.mode;
+        c->dda_budget = k.budget;
+        c->refill_min = k.refill;
+        c->refill_chunk = k.chunk;
+        rc = vt_render(c, 0, 0, passes, 0xffffffffu);
+        if (rc != VT_OK) break;
+        float ms = 0.0f;
+        if (hipStreamSynchronize(c->stream) != hipSuccess || hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
+            rc = fail(VT_ERR_HIP, "autotune timing failed");
+            break;
+        }
+        if (i > 0 && ms < best) {
+            best = ms;
+            pick = k;
+        }
+    }
+    VT_HIP(hipMemcpyAsync(c->d_accum, save_acc, acc_bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (sum_bytes) VT_HIP(hipMemcpyAsync(c->d_sum64, save_sum, sum_bytes, hipMemcpyDeviceToDevice, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    dfree(save_acc);
+    dfree(save_sum);
+    c->stats_enabled = was_stats;
+    c->timed = false;
+    if (rc != VT_OK) {
+        c->render_mode = mode0; c->dda_budget = b0; c->refill_min = r0; c->refill_chunk = k0;
+        return rc;
+    }
+    c->render_mode = pick.mode;
+    c->dda_budget = pick.budget;
+    c->refill_min = pick.refill;
+    c->refill_chunk = pick.chunk;
+    out->render_mode = pick.mode;
+    out->dda_budget = pick.budget;
+    out->refill_min = pick.refill;
+    out->refill_chunk = pick.chunk;
+    out->max_paths = c->max_paths;
+    out->ms = best;
+    out->candidates = (int32_t)cands.size();
     return VT_OK;
 }
 

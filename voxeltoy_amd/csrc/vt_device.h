@@ -159,77 +159,32 @@ __device__ __forceinline__ float aabb(f3 o, f3 d, f3 bmin, f3 bmax)
  * `slot` receives the hit voxel's material slot. */
 struct Hit { int32_t x, y, z, steps; uint32_t slot; };
 
-template <bool COUNT>
-__device__ __forceinline__ int dda(const GridView& g, const uint32_t* __restrict__ bits, f3 o, f3 d, Hit& h,
-                                   uint32_t& visits)
+/* Smallest of three floats in one instruction.  The DDA's distances are
+ * finite and non-negative (never NaN), where this equals fminf(fminf(a, b), c);
+ * the instruction form avoids the NaN-quieting the library min inserts. */
+__device__ __forceinline__ float min3f(float a, float b, float c)
 {
-    o = o + mk(vt_signf(d.x), vt_signf(d.y), vt_signf(d.z)) * 0.001f;
-    const float vox = ((o.x - g.bmin[0]) * g.ext[0]) * g.rx;
-    const float voy = ((o.y - g.bmin[1]) * g.ext[1]) * g.ry;
-    const float voz = ((o.z - g.bmin[2]) * g.ext[2]) * g.rz;
-    const float fx = floorf(vox), fy = floorf(voy), fz = floorf(voz);
-    if (fx < 0.0f || fy < 0.0f || fz < 0.0f || fx >= g.rx || fy >= g.ry || fz >= g.rz) {
-        h.steps = -1;
-        return -1;
-    }
-    /* mix(d, 1e-5, step(|d|, 1e-5)) as an exact select */
-    const float dx = vt_stepf(vt_absf(d.x), 1e-5f) != 0.0f ? 1e-5f : d.x;
-    const float dy = vt_stepf(vt_absf(d.y), 1e-5f) != 0.0f ? 1e-5f : d.y;
-    const float dz = vt_stepf(vt_absf(d.z), 1e-5f) != 0.0f ? 1e-5f : d.z;
-    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
-    const float sx = vt_signf(dx), sy = vt_signf(dy), sz = vt_signf(dz);
-    float disx = (((fx - vox) + 0.5f) + sx * 0.5f) * ix;
-    float disy = (((fy - voy) + 0.5f) + sy * 0.5f) * iy;
-    float disz = (((fz - voz) + 0.5f) + sz * 0.5f) * iz;
-    /* (mask*sign)*inc with mask in {0,1} is |inc| when the axis steps */
-    const float ax = vt_absf(ix), ay = vt_absf(iy), az = vt_absf(iz);
-    const int32_t stx = sx > 0.0f ? 1 : (sx < 0.0f ? -1 : 0);
-    const int32_t sty = sy > 0.0f ? 1 : (sy < 0.0f ? -1 : 0);
-    const int32_t stz = sz > 0.0f ? 1 : (sz < 0.0f ? -1 : 0);
-    int32_t x = (int32_t)fx, y = (int32_t)fy, z = (int32_t)fz;
-    int32_t steps = 0;
-    int hit = 0;
-    uint32_t block = 0xffffffffu;
-    uint64_t mask = 0;
-    while (steps < g.max_steps) {
-        if ((uint32_t)x >= (uint32_t)g.nx || (uint32_t)y >= (uint32_t)g.ny || (uint32_t)z >= (uint32_t)g.nz) break;
-        if (COUNT) visits++;
-        const uint32_t bk = block_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
-        if (bk != block) {
-            block = bk;
-            mask = ((bits[bk >> 5] >> (bk & 31u)) & 1u) ? g.occ64[bk] : 0ull;
-        }
-        if ((mask >> block_bit((uint32_t)x, (uint32_t)y, (uint32_t)z)) & 1ull) { hit = 1; break; }
-        const bool mx = !(disy < disx) && !(disz < disx);
-        const bool my = !(disx < disy) && !(disz < disy);
-        const bool mz = !(disy < disz) && !(disx < disz);
-        /* select form of the masked add: adding +0 instead of skipping can
-         * only change the sign of a zero dis, which nothing observes */
-        disx = disx + (mx ? ax : 0.0f);
-        disy = disy + (my ? ay : 0.0f);
-        disz = disz + (mz ? az : 0.0f);
-        x += mx ? stx : 0;
-        y += my ? sty : 0;
-        z += mz ? stz : 0;
-        steps++;
-    }
-    h.x = x; h.y = y; h.z = z; h.steps = steps;
-    h.slot = hit ? g.bricks[(size_t)brick_morton((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3) * 512u +
-                            local_morton((uint32_t)x, (uint32_t)y, (uint32_t)z)]
-                 : 0u;
-    return hit;
+    float r;
+    asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
 }
 
-/* The same march as resumable state, for traversal kernels whose lanes
- * take a new ray as soon as theirs finishes.  init() returns false when the
- * start point lies outside the grid (steps = -1, nothing else written);
- * advance() runs at most `budget` steps and returns 0 while the march goes
- * on, 1 on a hit, 2 when it left the grid or ran out of steps. */
+/* Key of the 4x4x4 block of an in-grid voxel (10 bits per axis). */
+__device__ __forceinline__ uint32_t block_key(int32_t x, int32_t y, int32_t z)
+{
+    return ((uint32_t)x >> 2) | (((uint32_t)y >> 2) << 10) | (((uint32_t)z >> 2) << 20);
+}
+
+/* The march as resumable state, for traversal kernels whose lanes take a
+ * new ray as soon as theirs finishes.  init() returns false when the start
+ * point lies outside the grid (steps = -1, nothing else written); advance()
+ * runs at most `budget` steps and returns 0 while the march goes on, 1 on a
+ * hit, 2 when it left the grid or ran out of steps. */
 struct DdaState {
     int32_t x, y, z, steps;
     float disx, disy, disz, ax, ay, az;
     int32_t stx, sty, stz;
-    uint32_t block;
+    uint32_t key;               /* block key of `mask` */
     uint64_t mask;
 
     __device__ __forceinline__ bool init(const GridView& g, f3 o, f3 d)
@@ -241,6 +196,7 @@ struct DdaState {
         const float fx = floorf(vox), fy = floorf(voy), fz = floorf(voz);
         steps = -1;
         if (fx < 0.0f || fy < 0.0f || fz < 0.0f || fx >= g.rx || fy >= g.ry || fz >= g.rz) return false;
+        /* mix(d, 1e-5, step(|d|, 1e-5)) as an exact select */
         const float dx = vt_stepf(vt_absf(d.x), 1e-5f) != 0.0f ? 1e-5f : d.x;
         const float dy = vt_stepf(vt_absf(d.y), 1e-5f) != 0.0f ? 1e-5f : d.y;
         const float dz = vt_stepf(vt_absf(d.z), 1e-5f) != 0.0f ? 1e-5f : d.z;
@@ -249,45 +205,65 @@ struct DdaState {
         disx = (((fx - vox) + 0.5f) + sx * 0.5f) * ix;
         disy = (((fy - voy) + 0.5f) + sy * 0.5f) * iy;
         disz = (((fz - voz) + 0.5f) + sz * 0.5f) * iz;
+        /* (mask*sign)*inc with mask in {0,1} is |inc| when the axis steps */
         ax = vt_absf(ix); ay = vt_absf(iy); az = vt_absf(iz);
         stx = sx > 0.0f ? 1 : (sx < 0.0f ? -1 : 0);
         sty = sy > 0.0f ? 1 : (sy < 0.0f ? -1 : 0);
         stz = sz > 0.0f ? 1 : (sz < 0.0f ? -1 : 0);
         x = (int32_t)fx; y = (int32_t)fy; z = (int32_t)fz;
         steps = 0;
-        block = 0xffffffffu;
+        key = 0xffffffffu;
         mask = 0ull;
         return true;
     }
 
+    /* One loop iteration per step in the order of dda.h:37-56: step limit,
+     * grid bounds, occupancy, then the compare/add step.  The axes that step
+     * are those whose dis equals the minimum of the three, i.e.
+     * lessThanEqual(dis, min(dis.yzx, dis.zxy)); dis is a running FP32 sum
+     * replayed step by step.  Lanes that stop keep their state and step no
+     * further (predicated updates), so the loop has one wave-uniform exit and
+     * no divergent branches except the block reload. */
     template <bool COUNT>
     __device__ __forceinline__ int advance(const GridView& g, const uint32_t* __restrict__ bits, int budget,
                                            uint32_t& visits)
     {
-        /* one limit for both the step budget and MAX_STEPS */
-        const int32_t limit = steps + budget < g.max_steps ? steps + budget : g.max_steps;
-        while (steps < limit) {
-            if ((uint32_t)x >= (uint32_t)g.nx || (uint32_t)y >= (uint32_t)g.ny || (uint32_t)z >= (uint32_t)g.nz) return 2;
-            if (COUNT) visits++;
-            const uint32_t bk = block_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
-            if (bk != block) {
-                block = bk;
+        const int32_t limit = budget < g.max_steps - steps ? steps + budget : g.max_steps;
+        bool done = false;
+        for (;;) {
+            const bool inside = (uint32_t)x < (uint32_t)g.nx && (uint32_t)y < (uint32_t)g.ny &&
+                                (uint32_t)z < (uint32_t)g.nz;
+            const bool more = steps < limit;
+            const uint32_t k = block_key(x, y, z);
+            if (inside && k != key) {
+                key = k;
+                const uint32_t bk = block_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
                 mask = ((bits[bk >> 5] >> (bk & 31u)) & 1u) ? g.occ64[bk] : 0ull;
             }
-            if ((mask >> block_bit((uint32_t)x, (uint32_t)y, (uint32_t)z)) & 1ull) return 1;
-            const bool mx = !(disy < disx) && !(disz < disx);
-            const bool my = !(disx < disy) && !(disz < disy);
-            const bool mz = !(disy < disz) && !(disx < disz);
-            /* select form of the masked add, as in dda() */
+            const uint32_t bit = ((uint32_t)x & 3u) | (((uint32_t)y & 3u) << 2) | (((uint32_t)z & 3u) << 4);
+            const bool hit = inside && ((mask >> bit) & 1ull);
+            if (COUNT && !done && more && inside) visits++;
+            done = done || !more || !inside || hit;
+            /* stopped lanes step no axis, so the step can run before the
+             * exit test without changing their state */
+            const float m = min3f(disx, disy, disz);
+            const bool mx = !done && disx == m, my = !done && disy == m, mz = !done && disz == m;
+            /* select form of the masked add: adding +0 instead of skipping can
+             * only change the sign of a zero dis, which nothing observes */
             disx = disx + (mx ? ax : 0.0f);
             disy = disy + (my ? ay : 0.0f);
             disz = disz + (mz ? az : 0.0f);
             x += mx ? stx : 0;
             y += my ? sty : 0;
             z += mz ? stz : 0;
-            steps++;
+            steps += done ? 0 : 1;
+            if (__ballot(!done) == 0ull) break;
         }
-        return steps >= g.max_steps ? 2 : 0;
+        /* every lane holds the state it stopped in */
+        if (steps >= limit) return steps >= g.max_steps ? 2 : 0;
+        const bool inside = (uint32_t)x < (uint32_t)g.nx && (uint32_t)y < (uint32_t)g.ny &&
+                            (uint32_t)z < (uint32_t)g.nz;
+        return inside ? 1 : 2;
     }
 
     /* traverse() result of a finished march: kind and hit record. */
@@ -302,6 +278,24 @@ struct DdaState {
         return y < 0 ? 3 : 0;
     }
 };
+
+/* ---- 3D-DDA (dda.h:7-61) -----------------------------------------------------------
+ * Returns 1 on hit, 0 when the march ended without a hit (position written),
+ * -1 when the start point lies outside the grid (position not written).
+ * `slot` receives the hit voxel's material slot. */
+template <bool COUNT>
+__device__ __forceinline__ int dda(const GridView& g, const uint32_t* __restrict__ bits, f3 o, f3 d, Hit& h,
+                                   uint32_t& visits)
+{
+    DdaState D;
+    if (!D.init(g, o, d)) {
+        h.steps = -1;
+        return -1;
+    }
+    const int status = D.advance<COUNT>(g, bits, g.max_steps, visits);
+    D.finish(g, status, h);
+    return status == 1 ? 1 : 0;
+}
 
 /* traverse (dda.h:63-79): 2 voxel hit, 3 ground hit (final y < 0), 0 miss. */
 enum { KIND_MISS = 0, KIND_VOXEL = 2, KIND_GROUND = 3 };

@@ -74,7 +74,23 @@ struct WfPaths {
     uint64_t cap;     /* slots */
 };
 
+/* A batch of scenes rendered by one wavefront launch: per-scene arguments
+ * and the prefix sums of their path counts (W*H*count) and pixel counts
+ * (W*H), n + 1 entries each. */
+struct BatchTable {
+    const TraceArgs* tab;
+    const uint64_t* path_base;
+    const uint64_t* pix_base;
+    uint32_t n;
+    unsigned long long* stats;   /* 6 counters or null */
+};
+
 size_t render_lds_bytes(const TraceArgs& a);
+/* Wavefront render of every scene of B (paths of all scenes share the
+ * queues; rounds follow the largest max_bounces, paths of scenes with fewer
+ * bounces finish earlier).  a0 supplies the traversal tuning. */
+hipError_t launch_render_batch(const TraceArgs& a0, const BatchTable& B, const WfPaths& P, uint64_t total_paths,
+                               uint64_t total_pixels, int max_bounces, int cus, hipStream_t s, bool stats);
 /* Wavefront render of the launch in a: path chunks of at most P.cap slots,
  * 1 + 2*max_bounces traverse/shade rounds per chunk. */
 hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats);

@@ -301,11 +301,11 @@ __global__ __launch_bounds__(256) void render_kernel(TraceArgs a)
  * After SHADOW, basis_o = ray_o and ray_d = next_d, so CONT reads them from
  * o and nd. */
 __device__ __forceinline__ void store_generated(const WfPaths& P, uint32_t id, const PathState& S, uint32_t pix,
-                                                uint32_t pass)
+                                                uint32_t pass, uint32_t scene)
 {
     P.bo[id] = make_float4(S.basis_o.x, S.basis_o.y, S.basis_o.z, 0.0f);
     P.d[id] = make_float4(S.ray_d.x, S.ray_d.y, S.ray_d.z, 0.0f);
-    P.meta[id] = make_uint4(pix, pass, (uint32_t)S.bounces, 0u);
+    P.meta[id] = make_uint4(pix, pass, (uint32_t)S.bounces, scene);
     P.rng[id] = S.rng.state;
 }
 
@@ -380,7 +380,7 @@ __device__ __forceinline__ void store_phase(const WfPaths& P, uint32_t id, const
 }
 
 template <bool STATS>
-__device__ __forceinline__ void flush_counters(const TraceArgs& a, const Counters& cnt)
+__device__ __forceinline__ void flush_counters(unsigned long long* stats, const Counters& cnt)
 {
     if (!STATS) return;
     __shared__ unsigned long long red[6];
@@ -393,16 +393,18 @@ __device__ __forceinline__ void flush_counters(const TraceArgs& a, const Counter
     atomicAdd(&red[4], (unsigned long long)cnt.env_samples);
     atomicAdd(&red[5], (unsigned long long)cnt.env_evals);
     __syncthreads();
-    if (threadIdx.x < 6 && red[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], red[threadIdx.x]);
+    if (threadIdx.x < 6 && red[threadIdx.x]) atomicAdd(&stats[threadIdx.x], red[threadIdx.x]);
 }
 
 /* Append the ray (o, d) of path `id` for the threads with `keep` to a ray
- * queue (length counter *n); the queue stores the rays themselves so the
- * traversal reads them in queue order.  One atomic per workgroup of NT
+ * queue (length counter *n); the queue stores the rays themselves (and the
+ * path's scene index `tag` in a batch) so the traversal reads them in queue
+ * order.  One atomic per workgroup of NT
  * threads: every queue counter is a single device address that all XCDs
  * contend for.  Every thread of the workgroup must call it. */
 template <int NT>
-__device__ __forceinline__ void block_append(bool keep, uint32_t id, f3 o, f3 d, float4* qa, float4* qb, uint32_t* n)
+__device__ __forceinline__ void block_append(bool keep, uint32_t id, uint32_t tag, f3 o, f3 d, float4* qa, float4* qb,
+                                             uint32_t* n)
 {
     __shared__ uint32_t wsum[NT / 64];
     __shared__ uint32_t bbase;
@@ -419,7 +421,7 @@ __device__ __forceinline__ void block_append(bool keep, uint32_t id, f3 o, f3 d,
     if (keep) {
         const uint32_t j = bbase + wsum[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
         qa[j] = make_float4(o.x, o.y, o.z, __uint_as_float(id));
-        qb[j] = make_float4(d.x, d.y, d.z, 0.0f);
+        qb[j] = make_float4(d.x, d.y, d.z, __uint_as_float(tag));
     }
     __syncthreads();
 }
@@ -427,25 +429,46 @@ __device__ __forceinline__ void block_append(bool keep, uint32_t id, f3 o, f3 d,
 #define WF_GENERATE_THREADS 1024
 #define WF_SHADE_THREADS 512
 
-template <bool STATS>
-__global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceArgs a, WfPaths P, uint64_t g0,
-                                                                         uint32_t n)
+/* Scene of global path index gi in a batch: base[s] <= gi < base[s + 1]. */
+__device__ __forceinline__ uint32_t scene_of(const uint64_t* __restrict__ base, uint32_t n, uint64_t gi)
+{
+    uint32_t lo = 0, hi = n;       /* invariant: base[lo] <= gi < base[hi] */
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= gi) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* Camera paths of the chunk [g0, g0 + n) of the launch's (scene,) pass and
+ * pixel space: a path index enumerates passes outermost and pixels row by
+ * row within a pass (within a scene, scenes back to back in a batch). */
+template <bool STATS, bool BATCH>
+__global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceArgs a0, BatchTable B, WfPaths P,
+                                                                         uint64_t g0, uint32_t n)
 {
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
-    const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         bool keep = false;
+        uint32_t sid = 0;
         f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
         if (i < n) {
-            const uint64_t gi = g0 + i;
+            uint64_t gi = g0 + i;
+            if (BATCH) {
+                sid = scene_of(B.path_base, B.n, gi);
+                gi -= B.path_base[sid];
+            }
+            const TraceArgs& a = BATCH ? B.tab[sid] : a0;
+            const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
             const uint32_t k = (uint32_t)(gi / WH), pix = (uint32_t)(gi % WH);
             const int32_t px = (int32_t)(pix % (uint32_t)a.W), py = (int32_t)(pix / (uint32_t)a.W);
             const uint32_t nn = a.n0 + k;
             const uint32_t rf = nn < a.rng_frame_max ? nn : a.rng_frame_max;
             PathState S;
             if (start_path<STATS>(a, px, py, rf, S, cnt)) {
-                store_generated(P, i, S, pix, k);
+                store_generated(P, i, S, pix, k, sid);
                 keep = true;
                 ro = S.ray_o;
                 rd = S.ray_d;
@@ -454,9 +477,9 @@ __global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceA
                 P.smp[i] = make_float4(c.x, c.y, c.z, 1.0f);
             }
         }
-        block_append<WF_GENERATE_THREADS>(keep, i, ro, rd, P.qa0, P.qb0, &P.cnt[0]);
+        block_append<WF_GENERATE_THREADS>(keep, i, sid, ro, rd, P.qa0, P.qb0, &P.cnt[0]);
     }
-    flush_counters<STATS>(a, cnt);
+    flush_counters<STATS>(BATCH ? B.stats : a0.stats, cnt);
 }
 
 /* Persistent traversal: each lane marches its ray a.dda_budget steps at a
@@ -469,8 +492,9 @@ __global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceA
  * of the block occupancy bits. */
 #define WF_TRAVERSE_THREADS 1024
 
-template <bool STATS, bool LDSB>
-__global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceArgs a, WfPaths P, int sel)
+template <bool STATS, bool LDSB, bool BATCH>
+__global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceArgs a, BatchTable B, WfPaths P,
+                                                                         int sel)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     MatRec* recs;
@@ -488,6 +512,8 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
     const uint32_t chunk = (uint32_t)a.refill_chunk;
     const int budget = a.dda_budget;
     DdaState D;
+    GridView gl;                   /* the lane's scene grid (batch only) */
+    const GridView& G = BATCH ? gl : a.g;
     uint32_t id = 0;
     bool has = false;
     /* wave-uniform: the wave's reserved queue range [c_pos, c_end) */
@@ -515,7 +541,8 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
                 if (j != 0xffffffffu) {
                     const float4 o = qa[j], d = qb[j];
                     id = __float_as_uint(o.w);
-                    if (D.init(a.g, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z))) {
+                    if (BATCH) gl = B.tab[__float_as_uint(d.w)].g;
+                    if (D.init(G, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z))) {
                         has = true;
                     } else {
                         P.hit[id] = make_int4(0, 0, 0, KIND_MISS << 8);
@@ -535,26 +562,26 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
             continue;
         }
         if (has) {
-            const int status = D.advance<STATS>(a.g, bits, budget, cnt.visits);
+            const int status = D.advance<STATS>(G, BATCH ? G.bits : bits, budget, cnt.visits);
             if (status) {
                 Hit h;
-                const int kind = D.finish(a.g, status, h);
+                const int kind = D.finish(G, status, h);
                 P.hit[id] = make_int4(h.x, h.y, h.z, (kind << 8) | (int32_t)h.slot);
                 has = false;
             }
         }
     }
-    flush_counters<STATS>(a, cnt);
+    flush_counters<STATS>(BATCH ? B.stats : a.stats, cnt);
 }
 
-template <bool STATS, int PHASE>
-__global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a, WfPaths P, int sel)
+template <bool STATS, int PHASE, bool BATCH>
+__global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0, BatchTable B, WfPaths P, int sel)
 {
     extern __shared__ __align__(16) uint8_t smem[];
-    MatRec* recs;
-    float* cdfv;
-    const uint32_t* bits;
-    stage_lds<false>(a, smem, recs, cdfv, bits);
+    MatRec* recs0 = nullptr;
+    float* cdfv0 = nullptr;
+    const uint32_t* bits0 = nullptr;
+    if (!BATCH) stage_lds<false>(a0, smem, recs0, cdfv0, bits0);
     const float4* qa = sel ? P.qa1 : P.qa0;
     float4* qao = sel ? P.qa0 : P.qa1;
     float4* qbo = sel ? P.qb0 : P.qb1;
@@ -564,10 +591,15 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a,
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t j = base + threadIdx.x;
         bool keep = false;
-        uint32_t id = 0;
+        uint32_t id = 0, sid = 0;
         f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
         if (j < n) {
             id = __float_as_uint(qa[j].w);
+            if (BATCH) sid = P.meta[id].w;
+            const TraceArgs& a = BATCH ? B.tab[sid] : a0;
+            const MatRec* recs = BATCH ? a.recs : recs0;
+            const float* cdfv = BATCH ? a.e.cdfv : cdfv0;
+            const uint32_t* bits = BATCH ? a.g.bits : bits0;
             PathState S;
             load_phase<PHASE>(P, id, S);
             const int4 hv = P.hit[id];
@@ -585,23 +617,39 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a,
                 P.smp[id] = make_float4(c.x, c.y, c.z, 1.0f);
             }
         }
-        block_append<WF_SHADE_THREADS>(keep, id, ro, rd, qao, qbo, &P.cnt[sel ^ 1]);
+        block_append<WF_SHADE_THREADS>(keep, id, sid, ro, rd, qao, qbo, &P.cnt[sel ^ 1]);
     }
-    flush_counters<STATS>(a, cnt);
+    flush_counters<STATS>(BATCH ? B.stats : a0.stats, cnt);
 }
 
-/* accumulation.fs:17 over the chunk's samples, pass by pass per pixel. */
-__global__ __launch_bounds__(256) void wf_accumulate_kernel(TraceArgs a, WfPaths P, uint64_t g0, uint64_t g1)
+/* accumulation.fs:17 over the chunk's samples, pass by pass per pixel (of
+ * every scene of a batch: pixels of the scenes back to back). */
+template <bool BATCH>
+__global__ __launch_bounds__(256) void wf_accumulate_kernel(TraceArgs a0, BatchTable B, WfPaths P, uint64_t g0,
+                                                            uint64_t g1, uint64_t n_pixels)
 {
-    const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
-    const uint32_t k_lo = (uint32_t)(g0 / WH), k_hi = (uint32_t)((g1 - 1) / WH);
-    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < WH; p += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_pixels;
+         q += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t sid = 0;
+        uint64_t p = q, pbase = 0;
+        if (BATCH) {
+            sid = scene_of(B.pix_base, B.n, q);
+            p = q - B.pix_base[sid];
+            pbase = B.path_base[sid];
+        }
+        const TraceArgs& a = BATCH ? B.tab[sid] : a0;
+        const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
+        /* passes of this scene whose paths fall in [g0, g1) */
+        const uint64_t lo = g0 > pbase ? g0 - pbase : 0, hi = g1 - pbase;
+        if (g1 <= pbase || lo >= WH * a.count) continue;
+        const uint64_t last = (hi - 1) / WH;
+        const uint32_t k_lo = (uint32_t)(lo / WH), k_hi = last < a.count - 1 ? (uint32_t)last : a.count - 1;
         float4 acc = a.accum[p];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
         if (a.sum64) { s0 = a.sum64[3 * p]; s1 = a.sum64[3 * p + 1]; s2 = a.sum64[3 * p + 2]; }
         bool any = false;
         for (uint32_t k = k_lo; k <= k_hi; ++k) {
-            const uint64_t gi = (uint64_t)k * WH + p;
+            const uint64_t gi = pbase + (uint64_t)k * WH + p;
             if (gi < g0 || gi >= g1) continue;
             any = true;
             const float4 c = P.smp[gi - g0];
@@ -634,46 +682,63 @@ static int blocks_per_cu(K kernel, int threads, size_t lds)
     return b;
 }
 
-template <bool STATS, bool LDSB>
-static hipError_t wf_run(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s)
+/* generate -> (traverse -> shade) x rounds -> accumulate per chunk of at
+ * most P.cap paths.  `a` describes the single scene (or, in a batch, carries
+ * the traversal tuning and the LDS sizes of scene 0). */
+template <bool STATS, bool LDSB, bool BATCH>
+static hipError_t wf_run(const TraceArgs& a, const BatchTable& B, const WfPaths& P, uint64_t total,
+                         uint64_t n_pixels, int rounds, int cus, hipStream_t s)
 {
-    const size_t lds_t = render_lds_bytes(a);
-    const size_t lds_s = 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
-    const int bt = blocks_per_cu(wf_traverse_kernel<STATS, LDSB>, WF_TRAVERSE_THREADS, lds_t);
-    const int bs0 = blocks_per_cu(wf_shade_kernel<STATS, PH_PRIMARY>, WF_SHADE_THREADS, lds_s);
-    const int bs1 = blocks_per_cu(wf_shade_kernel<STATS, PH_SHADOW>, WF_SHADE_THREADS, lds_s);
-    const int bs2 = blocks_per_cu(wf_shade_kernel<STATS, PH_CONT>, WF_SHADE_THREADS, lds_s);
-    const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
-    const uint64_t total = WH * a.count;
-    const int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
+    const size_t lds_t = BATCH ? 0 : render_lds_bytes(a);
+    const size_t lds_s = BATCH ? 0 : 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
+    const int bt = blocks_per_cu(wf_traverse_kernel<STATS, LDSB, BATCH>, WF_TRAVERSE_THREADS, lds_t);
+    const int bs0 = blocks_per_cu(wf_shade_kernel<STATS, PH_PRIMARY, BATCH>, WF_SHADE_THREADS, lds_s);
+    const int bs1 = blocks_per_cu(wf_shade_kernel<STATS, PH_SHADOW, BATCH>, WF_SHADE_THREADS, lds_s);
+    const int bs2 = blocks_per_cu(wf_shade_kernel<STATS, PH_CONT, BATCH>, WF_SHADE_THREADS, lds_s);
     for (uint64_t g0 = 0; g0 < total; g0 += P.cap) {
         const uint64_t g1 = g0 + P.cap < total ? g0 + P.cap : total;
         const uint32_t n = (uint32_t)(g1 - g0);
         wf_reset_kernel<<<1, 64, 0, s>>>(P.cnt);
         const uint32_t gn = (n + WF_GENERATE_THREADS - 1) / WF_GENERATE_THREADS;
         const unsigned gg = (unsigned)(gn < (uint32_t)cus * 2u ? gn : (uint32_t)cus * 2u);
-        wf_generate_kernel<STATS><<<gg, WF_GENERATE_THREADS, 0, s>>>(a, P, g0, n);
+        wf_generate_kernel<STATS, BATCH><<<gg, WF_GENERATE_THREADS, 0, s>>>(a, B, P, g0, n);
         int sel = 0;
         for (int r = 0; r < rounds; ++r) {
-            wf_traverse_kernel<STATS, LDSB><<<cus * bt, WF_TRAVERSE_THREADS, lds_t, s>>>(a, P, sel);
+            wf_traverse_kernel<STATS, LDSB, BATCH><<<cus * bt, WF_TRAVERSE_THREADS, lds_t, s>>>(a, B, P, sel);
             if (r == 0)
-                wf_shade_kernel<STATS, PH_PRIMARY><<<cus * bs0, WF_SHADE_THREADS, lds_s, s>>>(a, P, sel);
+                wf_shade_kernel<STATS, PH_PRIMARY, BATCH><<<cus * bs0, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel);
             else if (r & 1)
-                wf_shade_kernel<STATS, PH_SHADOW><<<cus * bs1, WF_SHADE_THREADS, lds_s, s>>>(a, P, sel);
+                wf_shade_kernel<STATS, PH_SHADOW, BATCH><<<cus * bs1, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel);
             else
-                wf_shade_kernel<STATS, PH_CONT><<<cus * bs2, WF_SHADE_THREADS, lds_s, s>>>(a, P, sel);
+                wf_shade_kernel<STATS, PH_CONT, BATCH><<<cus * bs2, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel);
             sel ^= 1;
         }
-        const unsigned ga = (unsigned)((WH + 255) / 256 < (uint64_t)cus * 8u ? (WH + 255) / 256 : (uint64_t)cus * 8u);
-        wf_accumulate_kernel<<<ga, 256, 0, s>>>(a, P, g0, g1);
+        const uint64_t gp = (n_pixels + 255) / 256;
+        const unsigned ga = (unsigned)(gp < (uint64_t)cus * 8u ? gp : (uint64_t)cus * 8u);
+        wf_accumulate_kernel<BATCH><<<ga, 256, 0, s>>>(a, B, P, g0, g1, n_pixels);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats)
 {
-    if (stats) return a.lds_bits ? wf_run<true, true>(a, P, cus, s) : wf_run<true, false>(a, P, cus, s);
-    return a.lds_bits ? wf_run<false, true>(a, P, cus, s) : wf_run<false, false>(a, P, cus, s);
+    const BatchTable B = { nullptr, nullptr, nullptr, 0u, nullptr };
+    const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
+    const uint64_t total = WH * a.count;
+    const int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
+    if (stats)
+        return a.lds_bits ? wf_run<true, true, false>(a, B, P, total, WH, rounds, cus, s)
+                          : wf_run<true, false, false>(a, B, P, total, WH, rounds, cus, s);
+    return a.lds_bits ? wf_run<false, true, false>(a, B, P, total, WH, rounds, cus, s)
+                      : wf_run<false, false, false>(a, B, P, total, WH, rounds, cus, s);
+}
+
+hipError_t launch_render_batch(const TraceArgs& a0, const BatchTable& B, const WfPaths& P, uint64_t total_paths,
+                               uint64_t total_pixels, int max_bounces, int cus, hipStream_t s, bool stats)
+{
+    const int rounds = 1 + 2 * (max_bounces > 0 ? max_bounces : 0);
+    if (stats) return wf_run<true, false, true>(a0, B, P, total_paths, total_pixels, rounds, cus, s);
+    return wf_run<false, false, true>(a0, B, P, total_paths, total_pixels, rounds, cus, s);
 }
 
 /* First hit of the primary ray of every pixel (the parity probe). */

@@ -125,6 +125,38 @@ def full_scene(seed: int = 2, n: int = 256, n_ids: int = 40, n_boxes: int = 24) 
     return sc
 
 
+def mixed_scene(rng, n: int, occupancy: float, n_ids: int = 24) -> Scene:
+    """n^3 noise terrain plus boxes filled to roughly `occupancy` of the
+    grid; IDs 70% Lambert, 20% metal, 10% emissive as in full_scene."""
+    heights = value_noise_2d(rng, n, max(2, n // 32)) * 0.5 + 0.5 * value_noise_2d(rng, n, max(3, n // 12))
+    # terrain holds about 80% of the target occupancy, boxes the rest
+    h = heights * (1.6 * occupancy * n)
+    y = np.arange(n, dtype=np.float32)[None, :, None]
+    occ = y < h[:, None, :]
+    deficit = occupancy * n ** 3 - float(occ.sum())
+    added = 0.0
+    while added < deficit and n >= 8:   # box volumes, overlaps ignored
+        s = rng.integers(max(1, n // 16), max(2, n // 4), size=3)
+        p = rng.integers(0, n - s + 1)
+        occ[p[2]:p[2] + s[2], p[1]:p[1] + s[1], p[0]:p[0] + s[0]] = True
+        added += float(np.prod(s))
+    n_lam = int(round(0.7 * n_ids))
+    n_met = int(round(0.2 * n_ids))
+    n_emi = n_ids - n_lam - n_met
+    recs = [lambert((0, 0, 0), rng.uniform(0.2, 0.8, 3)) for _ in range(n_lam)]
+    recs += [metal((0, 0, 0), rng.uniform(0.5, 0.95, 3), rng.uniform(0.05, 0.6)) for _ in range(n_met)]
+    recs += [lambert(rng.lognormal(1.0, 0.5) * np.ones(3) * rng.uniform(0.7, 1.0, 3), rng.uniform(0.2, 0.8, 3))
+             for _ in range(n_emi)]
+    mat, offs = build_materials(recs)
+    id_offset = np.full(256, -1, np.int32)
+    id_offset[1:n_ids + 1] = offs
+    ids = np.zeros(occ.shape, np.uint8)
+    ids[occ] = rng.integers(1, n_ids + 1, size=int(occ.sum()), dtype=np.uint8)
+    sc = Scene(ids, id_offset, mat)
+    sc.emissive = emissive_list(ids, id_offset, mat)
+    return sc
+
+
 def direction_from_uv(u, v, rotation=0.0):
     """coordinates.h:111-116 in float64 (for building environment images)."""
     u, v = np.broadcast_arrays(u, v)

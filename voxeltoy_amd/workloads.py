@@ -87,6 +87,34 @@ def full(width: int = 1024, height: int = 1024, seed_scene: int = 2, n: int = 25
                     bmax=bmax)
 
 
+def batch(count: int = 64, seed: int = 7, bounces: int = 4, n_range=(32, 512), res_range=(128, 1024),
+          spp_range=(1, 64), occ_range=(0.05, 0.6)) -> list:
+    """Many independent scenes: grid edge, image edge and spp drawn
+    log-uniformly from their ranges, occupancy uniformly; materials as in
+    `full`; gradient environment; pinhole camera at the focusOnBounds
+    distance from a random direction above the horizon."""
+    rng = np.random.default_rng(seed)
+
+    def log_uniform(lo, hi):
+        return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+
+    out = []
+    for i in range(count):
+        n = log_uniform(*n_range)
+        res = log_uniform(*res_range)
+        spp = log_uniform(*spp_range)
+        occ = float(rng.uniform(*occ_range))
+        sc = scenes.mixed_scene(rng, n, occ)
+        offsets = sc.offsets()
+        sc.emissive = api.prune_emissive(offsets, sc.emissive)
+        bmin, bmax = reference_bounds(sc.shape)
+        cam = scenes.orbit_camera(bmin, bmax, math.radians(rng.uniform(100, 150)), rng.uniform(0, 2 * math.pi),
+                                  res, res)
+        out.append(Workload(f"batch{i}", sc, cam, res, res, spp=spp, bounces=bounces, seed=1000 + i, bmin=bmin,
+                            bmax=bmax, env_top=(1.6, 1.7, 1.9), env_bottom=(0.35, 0.3, 0.25)))
+    return out
+
+
 def load(ctx: api.Context, wl: Workload, fp64: bool = False, int32_grid: bool = False):
     """Upload scene, camera, environment and settings of `wl` into ctx."""
     sc = wl.scene
