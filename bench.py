@@ -84,9 +84,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--mode", type=int, default=1, help="0 megakernel, 1 wavefront (default)")
-    ap.add_argument("--workload", default="full", choices=["full", "batch"],
+    ap.add_argument("--workload", default="full", choices=["full", "batch", "cache"],
                     help="full: the 256^3 full-shading scene (default); batch: 64 mixed scenes, one batched "
-                         "graph launch against 64 sequential renders")
+                         "graph launch against 64 sequential renders; cache: time to RMSE 1e-2 against an FP64 "
+                         "reference, plain tracer against the radiance cache")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="cache workload: passes of the FP64 reference")
+    ap.add_argument("--budget-s", type=float, default=3.0, help="cache workload: render time of each contender")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the path shards by "
@@ -97,6 +100,8 @@ def main():
 
     if args.workload == "batch":
         return bench_batch(args, api, workloads)
+    if args.workload == "cache":
+        return bench_cache(args, api, workloads)
 
     wl = workloads.full()
     ctx = api.Context(0)
@@ -233,6 +238,98 @@ def bench_batch(args, api, workloads):
     b.close()
     for c in ctxs:
         c.close()
+
+
+def bench_cache(args, api, workloads, target=1e-2):
+    """Time to RMSE `target` (over pixels x RGB of the tonemapped mean)
+    against an FP64-accumulated reference of the same scene: the plain
+    tracer (8 bounces) against radiance-cache rendering (paths end into the
+    cache after 2 bounces, 16k training paths per frame), one sample per
+    pixel per frame, each given the same render-time budget."""
+    import numpy as np
+    wl = workloads.full(width=960, height=540, spp=1, bounces=8)
+    ref_ctx = api.Context(0)
+    workloads.load(ref_ctx, wl, fp64=True)
+    done = 0
+    while done < args.ref_spp:
+        k = min(256, args.ref_spp - done)
+        ref_ctx.render(wl.seed + 1, done, k)
+        done += k
+    ref = (ref_ctx.read_sum64() / args.ref_spp).astype(np.float64)
+    ref_ctx.close()
+    finite = np.isfinite(ref).all(axis=-1)
+
+    def rmse(img):
+        d = img[..., :3].astype(np.float64) - ref
+        d = d[finite & np.isfinite(img[..., :3]).all(axis=-1)]
+        return float(np.sqrt(np.mean(d * d)))
+
+    def run(step_fn, read_fn):
+        """Frames until the time budget is spent; the RMSE is read at frames
+        1, 2, 4, ... and every 64 frames (reading is not timed)."""
+        t, n, curve, first = 0.0, 0, [], None
+        prev = None
+        while t < args.budget_s * 1000.0:
+            t += step_fn(n)
+            n += 1
+            if (n & (n - 1)) == 0 or n % 64 == 0:
+                e = rmse(read_fn())
+                curve.append((round(t, 2), round(e, 5)))
+                if first is None and e <= target:
+                    # interpolate between this and the previous checkpoint
+                    if prev is not None and prev[1] > target:
+                        f = (prev[1] - target) / (prev[1] - e)
+                        first = prev[0] + f * (t - prev[0])
+                    else:
+                        first = t
+                prev = (t, e)
+        e = rmse(read_fn())
+        curve.append((round(t, 2), round(e, 5)))
+        return {"frames": n, "ms": round(t, 1), "final_rmse": curve[-1][1],
+                "time_to_target_s": None if first is None else round(first / 1000.0, 3),
+                "curve": curve}
+
+    plain_ctx = api.Context(0)
+    workloads.load(plain_ctx, wl)
+
+    def plain_step(n):
+        plain_ctx.render(wl.seed, n, 1)
+        return plain_ctx.stats()["kernel_ms"]
+
+    plain = run(plain_step, plain_ctx.read_accum)
+    cache_ctx = api.Context(0)
+    workloads.load(cache_ctx, wl)
+    cache = api.RadianceCache(cache_ctx, seed=1)
+
+    def cache_step(n):
+        return cache.render(wl.seed, n, 1, cache_bounce=2, train_paths=16384, lr=3e-3)["ms"]
+
+    cached = run(cache_step, cache_ctx.read_accum)
+    tp, tc = plain["time_to_target_s"], cached["time_to_target_s"]
+    out = {
+        "metric": METRIC,
+        "value": tc,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": plain["frames"] + cached["frames"],
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 tracer, fp16 mlp",
+        "data": "synthetic",
+        "config": {"workload": "full-shading 256^3 scene at 960x540, 1 spp per frame; FP64 reference of "
+                               f"{args.ref_spp} spp; target RMSE {target} (tonemapped mean, pixels x RGB)",
+                   "plain": plain, "cache": cached,
+                   "speedup_time_to_target": None if (tp is None or tc is None) else round(tp / tc, 3)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    cache.close()
+    plain_ctx.close()
+    cache_ctx.close()
 
 
 if __name__ == "__main__":

@@ -195,6 +195,44 @@ typedef struct vt_tuning {
 } vt_tuning;
 int vt_autotune(vt_context* ctx, uint32_t passes, vt_tuning* out);
 
+/* ---- learned radiance cache ---------------------------------------------------
+ * A 288-64-64-64-64-64-3 ReLU MLP over a 12-band sin/cos encoding of a
+ * surface query (12 raw floats: position normalised to the volume bounds,
+ * (direction+1)/2, (normal+1)/2, albedo) predicting outgoing RGB radiance.
+ * Inference runs FP16 matrix-core tiles with FP32 accumulation; training is
+ * FP32 (relative L2 loss, Adam).  Parameter vector layout: W1[64][288], b1,
+ * W2..W5[64][64], b2..b5, W6[3][64], b6.  No counterpart in the reference. */
+typedef struct vt_cache vt_cache;
+int vt_cache_create(vt_context* ctx, uint64_t seed, vt_cache** out);
+void vt_cache_destroy(vt_cache* cache);
+int32_t vt_cache_num_params(void);
+int vt_cache_get_params(vt_cache* cache, float* params);
+int vt_cache_set_params(vt_cache* cache, const float* params);
+/* n queries (raw, n x 12) -> radiance (n x 3), host arrays. */
+int vt_cache_infer(vt_cache* cache, int64_t n, const float* raw, float* out);
+/* Gradient of the loss over n samples (targets n x 3) without updating. */
+int vt_cache_gradient(vt_cache* cache, int64_t n, const float* raw, const float* target, float* grad, float* loss);
+/* One Adam step (lr; beta1 0.9, beta2 0.99, eps 1e-8) on n samples. */
+int vt_cache_train(vt_cache* cache, int64_t n, const float* raw, const float* target, float lr, float* loss);
+
+/* Render `count` passes (pass n0+k, one sample per pixel each) with the
+ * cache: paths end into the cache at their vertex of bounce cache_bounce
+ * (-1: never), except about train_paths training paths per pass, which run
+ * the plain integrator and give one sample per shaded vertex (target =
+ * radiance gathered after the vertex / throughput at the vertex); after each
+ * pass the cache takes Adam steps (learning rate lr) over those samples in
+ * minibatches of at most `batch`.  Statistics of the last call in *out
+ * (may be NULL). */
+typedef struct vt_cache_stats {
+    uint64_t queries;       /* paths that ended into the cache */
+    uint64_t samples;       /* training samples used */
+    uint32_t steps;         /* Adam steps taken */
+    float last_loss;        /* loss of the last step */
+    double ms;              /* wall time of the call */
+} vt_cache_stats;
+int vt_render_cached(vt_context* ctx, vt_cache* cache, uint64_t seed, uint32_t n0, uint32_t count,
+                     int32_t cache_bounce, uint32_t train_paths, float lr, uint32_t batch, vt_cache_stats* out);
+
 /* ---- probes -------------------------------------------------------------
  * First hit of every pixel's primary ray of frame `frame`: rec[8] per
  * pixel = kind (0 AABB miss, 1 traversal miss, 2 voxel, 3 ground), voxel

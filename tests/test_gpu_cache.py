@@ -1,0 +1,175 @@
+"""GPU: the learned radiance cache's kernels against the CPU reference model
+(oracle/cache_ref.py).  Tolerances: inference vs the float16-operand
+emulation, max |err| <= 2e-3 * (|ref| + mean |ref|) (float32 accumulation
+order differs and a float16 rounding of an activation may flip); gradient vs
+float64, relative error of each layer's gradient <= 1e-3 of its norm."""
+from __future__ import annotations
+
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import cache_ref as cr  # noqa: E402
+
+
+def vsincos(orc):
+    def f(x):
+        s = np.empty(len(x), np.float32)
+        c = np.empty(len(x), np.float32)
+        for i, v in enumerate(x):
+            s[i], c[i] = orc.sincos(float(v))
+        return s, c
+    return f
+
+
+@pytest.fixture(scope="module")
+def cache(ctx, vtapi):
+    c = vtapi.RadianceCache(ctx, seed=5)
+    yield c
+    c.close()
+
+
+def test_params_roundtrip(cache):
+    p = cache.params()
+    assert p.size == cr.N_PARAMS
+    layers = cr.unpack(p)
+    for li, (w, b) in enumerate(layers):
+        assert np.all(b == 0)
+        bound = math.sqrt(6.0 / w.shape[1]) * (0.1 if li == 5 else 1.0)
+        assert np.abs(w).max() <= bound and np.abs(w).std() > 0.3 * bound
+    q = p.copy()
+    q[::7] *= -1.5
+    cache.set_params(q)
+    assert np.array_equal(cache.params(), q)
+    cache.set_params(p)
+
+
+@pytest.mark.parametrize("n", [1, 16, 1000, 4097])
+def test_infer_matches_fp16_emulation(cache, orc, n):
+    rng = np.random.default_rng(n)
+    p = cache.params().copy()
+    layers = cr.unpack(p)
+    for _, b in layers:
+        b[:] = rng.uniform(-0.2, 0.2, b.shape)
+    cache.set_params(p)
+    raw = rng.uniform(0, 1, (n, 12)).astype(np.float32)
+    got = cache.infer(raw)
+    x = cr.encode(raw, vsincos(orc))
+    ref = cr.forward_fp16(p, x)
+    err = np.abs(got - ref)
+    assert np.all(err <= 2e-3 * (np.abs(ref) + np.mean(np.abs(ref)))), float(err.max())
+
+
+def test_gradient_matches_fp64(cache, orc):
+    rng = np.random.default_rng(9)
+    n = 3000
+    raw = rng.uniform(0, 1, (n, 12)).astype(np.float32)
+    tgt = rng.uniform(0, 3, (n, 3)).astype(np.float32)
+    p = cache.params()
+    g, loss = cache.gradient(raw, tgt)
+    x = cr.encode(raw, vsincos(orc))
+    lref, gref = cr.loss_and_grad64(p.astype(np.float64), x.astype(np.float64), tgt.astype(np.float64))
+    assert abs(loss - lref) <= 1e-4 * lref
+    off = 0
+    for i, o in cr.LAYERS:
+        for sz in (o * i, o):
+            a, b = g[off:off + sz].astype(np.float64), gref[off:off + sz]
+            assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b) + 1e-12, (off, sz)
+            off += sz
+
+
+def test_training_reduces_loss(ctx, vtapi):
+    """Fit a smooth radiance-like function of the inputs."""
+    c = vtapi.RadianceCache(ctx, seed=11)
+    rng = np.random.default_rng(3)
+
+    def target(raw):
+        return np.stack([1.0 + np.sin(3 * raw[:, 0]) * raw[:, 9],
+                         0.5 + raw[:, 1] * raw[:, 4],
+                         2.0 * raw[:, 2] ** 2 + 0.1 * raw[:, 11]], 1).astype(np.float32)
+
+    test = rng.uniform(0, 1, (2048, 12)).astype(np.float32)
+    before = np.mean((c.infer(test) - target(test)) ** 2)
+    losses = []
+    for it in range(300):
+        raw = rng.uniform(0, 1, (4096, 12)).astype(np.float32)
+        losses.append(c.train(raw, target(raw), lr=3e-3))
+    after = np.mean((c.infer(test) - target(test)) ** 2)
+    assert np.mean(losses[-20:]) < 0.2 * np.mean(losses[:5])
+    assert after < 0.1 * before, (before, after)
+    c.close()
+
+
+def cached_scene(vtapi, workloads, bounces):
+    import math as _m
+    from voxeltoy_amd import scenes
+    sc = scenes.full_scene(4, n=48, n_ids=20, n_boxes=6)
+    bmin, bmax = workloads.reference_bounds(sc.shape)
+    cam = scenes.orbit_camera(bmin, bmax, _m.radians(125), _m.radians(40), 64, 48)
+    wl = workloads.Workload("c", sc, cam, 64, 48, spp=3, bounces=bounces, seed=21, bmin=bmin, bmax=bmax,
+                            env_top=(1.5, 1.6, 1.8), env_bottom=(0.3, 0.25, 0.2))
+    return wl
+
+
+def test_cached_render_without_cache_is_the_plain_render(vtapi):
+    """cache_bounce -1 with training paths recording: identical images."""
+    from voxeltoy_amd import workloads
+    wl = cached_scene(vtapi, workloads, 3)
+    a = vtapi.Context(0)
+    workloads.load(a, wl)
+    k = vtapi.RadianceCache(a, seed=2)
+    st = k.render(wl.seed, 0, wl.spp, cache_bounce=-1, train_paths=500, lr=1e-3)
+    assert st["queries"] == 0 and st["samples"] > 0 and st["steps"] >= wl.spp
+    b = vtapi.Context(0)
+    workloads.load(b, wl)
+    b.render(wl.seed, 0, wl.spp)
+    assert np.array_equal(a.read_accum().view(np.uint32), b.read_accum().view(np.uint32))
+    k.close()
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("cb", [1, 2])
+def test_zero_cache_equals_paths_cut_at_the_cache_bounce(vtapi, cb):
+    """A cache predicting 0 (all parameters zero), no training: a path ending
+    into the cache at bounce cb contributes exactly what the plain integrator
+    with max_bounces = cb does."""
+    from voxeltoy_amd import workloads
+    wl = cached_scene(vtapi, workloads, 5)
+    ctx = vtapi.Context(0)
+    workloads.load(ctx, wl)
+    k = vtapi.RadianceCache(ctx, seed=3)
+    k.set_params(np.zeros(k.n_params, np.float32))
+    st = k.render(wl.seed, 0, wl.spp, cache_bounce=cb, train_paths=0)
+    assert st["queries"] > 0 and st["steps"] == 0
+    ref = vtapi.Context(0)
+    wl.bounces = cb
+    workloads.load(ref, wl)
+    ref.render(wl.seed, 0, wl.spp)
+    assert np.array_equal(ctx.read_accum().view(np.uint32), ref.read_accum().view(np.uint32))
+    k.close()
+    ctx.close()
+    ref.close()
+
+
+def test_cached_render_trains(vtapi):
+    from voxeltoy_amd import workloads
+    wl = cached_scene(vtapi, workloads, 4)
+    ctx = vtapi.Context(0)
+    workloads.load(ctx, wl)
+    k = vtapi.RadianceCache(ctx, seed=4)
+    first = k.render(wl.seed, 0, 1, cache_bounce=1, train_paths=2000, lr=3e-3)
+    for n in range(1, 30):
+        last = k.render(wl.seed, n, 1, cache_bounce=1, train_paths=2000, lr=3e-3)
+    assert first["samples"] > 1000 and last["queries"] > 0
+    assert np.isfinite(last["last_loss"]) and last["last_loss"] < first["last_loss"]
+    img = ctx.read_accum()
+    assert np.isfinite(img).all()
+    k.close()
+    ctx.close()

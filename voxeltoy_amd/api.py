@@ -59,6 +59,14 @@ class Tuning(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class CacheStats(C.Structure):
+    _fields_ = [("queries", C.c_uint64), ("samples", C.c_uint64), ("steps", C.c_uint32), ("last_loss", C.c_float),
+                ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # symbol -> (restype, argtypes)
 SIGNATURES = {
     "vt_last_error": (C.c_char_p, []),
@@ -88,6 +96,16 @@ SIGNATURES = {
     "vt_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "vt_batch_destroy": (None, [C.c_void_p]),
     "vt_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Tuning)]),
+    "vt_cache_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "vt_cache_destroy": (None, [C.c_void_p]),
+    "vt_cache_num_params": (C.c_int32, []),
+    "vt_cache_get_params": (C.c_int, [C.c_void_p, f32p]),
+    "vt_cache_set_params": (C.c_int, [C.c_void_p, f32p]),
+    "vt_cache_infer": (C.c_int, [C.c_void_p, C.c_int64, f32p, f32p]),
+    "vt_cache_gradient": (C.c_int, [C.c_void_p, C.c_int64, f32p, f32p, f32p, C.POINTER(C.c_float)]),
+    "vt_cache_train": (C.c_int, [C.c_void_p, C.c_int64, f32p, f32p, C.c_float, C.POINTER(C.c_float)]),
+    "vt_render_cached": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32,
+                                   C.c_float, C.c_uint32, C.POINTER(CacheStats)]),
     "vt_render": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
     "vt_read_accum": (C.c_int, [C.c_void_p, f32p]),
     "vt_read_sum64": (C.c_int, [C.c_void_p, f64p]),
@@ -425,6 +443,65 @@ class Batch:
         if self._b:
             lib().vt_batch_destroy(self._b)
             self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RadianceCache:
+    """The learned radiance cache of one context (see vt_api.h)."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        self.ctx = ctx
+        self._p = C.c_void_p()
+        _check(lib().vt_cache_create(ctx._c, seed, C.byref(self._p)))
+        self.n_params = lib().vt_cache_num_params()
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        _check(lib().vt_cache_get_params(self._p, p))
+        return p
+
+    def set_params(self, p):
+        _check(lib().vt_cache_set_params(self._p, np.ascontiguousarray(p, np.float32)))
+
+    def infer(self, raw) -> np.ndarray:
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 12)
+        out = np.zeros((len(raw), 3), np.float32)
+        _check(lib().vt_cache_infer(self._p, len(raw), raw.reshape(-1), out.reshape(-1)))
+        return out
+
+    def gradient(self, raw, target):
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 12)
+        tgt = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        _check(lib().vt_cache_gradient(self._p, len(raw), raw.reshape(-1), tgt.reshape(-1), g, C.byref(loss)))
+        return g, loss.value
+
+    def train(self, raw, target, lr: float = 1e-3) -> float:
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 12)
+        tgt = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        loss = C.c_float()
+        _check(lib().vt_cache_train(self._p, len(raw), raw.reshape(-1), tgt.reshape(-1), lr, C.byref(loss)))
+        return loss.value
+
+    def render(self, seed: int, n0: int, count: int, cache_bounce: int = 2, train_paths: int = 16384,
+               lr: float = 3e-3, batch: int = 16384) -> dict:
+        """`count` passes of the context's frame with paths ending into the
+        cache at bounce `cache_bounce`, training after every pass."""
+        st = CacheStats()
+        _check(lib().vt_render_cached(self.ctx._c, self._p, seed, n0, count, cache_bounce, train_paths, lr, batch,
+                                      C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._p:
+            lib().vt_cache_destroy(self._p)
+            self._p = C.c_void_p()
 
     def __del__(self):
         try:

@@ -9,9 +9,11 @@
  */
 #include "../../include/vt_api.h"
 #include "vt_kernels.h"
+#include "vt_cache.h"
 #include "../host/vt_host_algos.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -817,6 +819,322 @@ int vt_batch_get_stats(vt_batch* b, vt_stats* out)
         VT_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
         out->kernel_ms = ms;
     }
+    return VT_OK;
+}
+
+/* ---- learned radiance cache ------------------------------------------------------- */
+struct vt_cache {
+    vt_context* ctx = nullptr;
+    float* d_params = nullptr;
+    float* d_grad = nullptr;
+    float* d_m = nullptr;
+    float* d_v = nullptr;
+    _Float16* d_pack = nullptr;
+    float* d_bias = nullptr;
+    float* d_loss = nullptr;
+    int step = 0;
+    /* training scratch for up to `cap` samples */
+    int64_t cap = 0;
+    float *d_X = nullptr, *d_H = nullptr, *d_DZ = nullptr, *d_DY = nullptr, *d_raw = nullptr, *d_tgt = nullptr;
+    /* inference staging */
+    int64_t icap = 0;
+    float *d_iraw = nullptr, *d_iout = nullptr;
+    /* cache rendering buffers */
+    uint64_t qcap = 0;
+    float *d_qraw = nullptr, *d_qout = nullptr;
+    uint32_t* d_qid = nullptr;
+    uint32_t* d_counts = nullptr;      /* [0] queries, [1] samples */
+    uint64_t vcap = 0;                 /* train slots x vertices */
+    float* d_vrec = nullptr;
+    uint32_t* d_vrec_n = nullptr;
+    uint64_t vslots = 0;
+    uint64_t scap = 0;
+    float *d_sraw = nullptr, *d_stgt = nullptr;
+};
+
+namespace {
+
+int cache_repack(vt_cache* c)
+{
+    VT_HIP(vt::launch_mlp_pack(c->d_params, c->d_pack, c->d_bias, c->ctx->stream));
+    return VT_OK;
+}
+
+int cache_scratch(vt_cache* c, int64_t n)
+{
+    if (n <= c->cap) return VT_OK;
+    dfree(c->d_X); dfree(c->d_H); dfree(c->d_DZ); dfree(c->d_DY); dfree(c->d_raw); dfree(c->d_tgt);
+    c->cap = 0;
+    VT_HIP(hipMalloc(&c->d_X, (size_t)n * vt::MLP_IN * 4));
+    VT_HIP(hipMalloc(&c->d_H, (size_t)n * 5 * vt::MLP_W * 4));
+    VT_HIP(hipMalloc(&c->d_DZ, (size_t)n * 5 * vt::MLP_W * 4));
+    VT_HIP(hipMalloc(&c->d_DY, (size_t)n * vt::MLP_OUT * 4));
+    VT_HIP(hipMalloc(&c->d_raw, (size_t)n * vt::MLP_RAW * 4));
+    VT_HIP(hipMalloc(&c->d_tgt, (size_t)n * vt::MLP_OUT * 4));
+    c->cap = n;
+    return VT_OK;
+}
+
+/* gradient of the loss over n device-resident samples into d_grad (zeroed
+ * first); loss mean into *loss when non-null */
+int cache_grad_device(vt_cache* c, int64_t n, const float* d_raw, const float* d_tgt, float* loss)
+{
+    hipStream_t s = c->ctx->stream;
+    int r = cache_scratch(c, n);
+    if (r != VT_OK) return r;
+    VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_grad, vt::MLP_N_PARAMS, 0u, s));
+    VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_loss, 1, 0u, s));
+    VT_HIP(vt::launch_mlp_fwd_bwd(c->d_params, d_raw, d_tgt, n, c->d_X, c->d_H, c->d_DZ, c->d_DY, c->d_loss, s));
+    VT_HIP(vt::launch_mlp_grad(c->d_X, c->d_H, c->d_DZ, c->d_DY, n, c->d_grad, s));
+    if (loss) {
+        float l = 0.0f;
+        VT_HIP(hipMemcpyAsync(&l, c->d_loss, 4, hipMemcpyDeviceToHost, s));
+        VT_HIP(hipStreamSynchronize(s));
+        *loss = l / (float)(vt::MLP_OUT * n);
+    }
+    return VT_OK;
+}
+
+int cache_adam(vt_cache* c, float lr)
+{
+    c->step++;
+    VT_HIP(vt::launch_adam(c->d_params, c->d_grad, c->d_m, c->d_v, vt::MLP_N_PARAMS, lr, 0.9f, 0.99f, 1e-8f, c->step,
+                           c->ctx->stream));
+    return cache_repack(c);
+}
+
+}  // namespace
+
+int32_t vt_cache_num_params(void) { return vt::MLP_N_PARAMS; }
+
+int vt_cache_create(vt_context* ctx, uint64_t seed, vt_cache** out)
+{
+    if (!ctx || !out) return fail(VT_ERR_ARG, "null argument");
+    VT_HIP(hipSetDevice(ctx->device));
+    vt_cache* c = new vt_cache();
+    c->ctx = ctx;
+    const size_t pb = (size_t)vt::MLP_N_PARAMS * 4;
+    if (hipMalloc(&c->d_params, pb) != hipSuccess || hipMalloc(&c->d_grad, pb) != hipSuccess ||
+        hipMalloc(&c->d_m, pb) != hipSuccess || hipMalloc(&c->d_v, pb) != hipSuccess ||
+        hipMalloc(&c->d_pack, (size_t)vt::MLP_PACK * 2) != hipSuccess ||
+        hipMalloc(&c->d_bias, (size_t)vt::MLP_BIAS * 4) != hipSuccess || hipMalloc(&c->d_loss, 4) != hipSuccess) {
+        vt_cache_destroy(c);
+        return fail(VT_ERR_HIP, "cache allocation failed");
+    }
+    /* He-uniform weights (bound sqrt(6 / fan_in)), output layer scaled by
+     * 0.1, zero biases; PCG32 stream `seed` */
+    std::vector<float> p((size_t)vt::MLP_N_PARAMS, 0.0f);
+    /* PCG32 (XSH-RR), initstate = seed, stream 0x5bd1e995 */
+    uint64_t st = 0u;
+    const uint64_t inc = (0x5bd1e995ull << 1u) | 1u;
+    auto next = [&]() {
+        const uint64_t old = st;
+        st = old * 6364136223846793005ull + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u), rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((32u - rot) & 31u));
+    };
+    next();
+    st += seed;
+    next();
+    for (int L = 0; L < 6; ++L) {
+        const int in = L == 0 ? vt::MLP_IN : vt::MLP_W, outn = L == 5 ? vt::MLP_OUT : vt::MLP_W;
+        const float bound = std::sqrt(6.0f / (float)in) * (L == 5 ? 0.1f : 1.0f);
+        for (int e = 0; e < outn * in; ++e) {
+            const float u = (float)(next() >> 8) * (1.0f / 16777216.0f);
+            p[(size_t)vt::mlp_w_off(L) + e] = (2.0f * u - 1.0f) * bound;
+        }
+    }
+    *out = c;
+    const int r = vt_cache_set_params(c, p.data());
+    if (r != VT_OK) { vt_cache_destroy(c); *out = nullptr; }
+    return r;
+}
+
+void vt_cache_destroy(vt_cache* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->ctx->device);
+    (void)hipStreamSynchronize(c->ctx->stream);
+    dfree(c->d_params); dfree(c->d_grad); dfree(c->d_m); dfree(c->d_v); dfree(c->d_pack); dfree(c->d_bias);
+    dfree(c->d_loss); dfree(c->d_X); dfree(c->d_H); dfree(c->d_DZ); dfree(c->d_DY); dfree(c->d_raw);
+    dfree(c->d_tgt); dfree(c->d_iraw); dfree(c->d_iout);
+    dfree(c->d_qraw); dfree(c->d_qout); dfree(c->d_qid); dfree(c->d_counts);
+    dfree(c->d_vrec); dfree(c->d_vrec_n); dfree(c->d_sraw); dfree(c->d_stgt);
+    delete c;
+}
+
+int vt_cache_get_params(vt_cache* c, float* p)
+{
+    if (!c || !p) return fail(VT_ERR_ARG, "null argument");
+    VT_HIP(hipSetDevice(c->ctx->device));
+    VT_HIP(hipMemcpyAsync(p, c->d_params, (size_t)vt::MLP_N_PARAMS * 4, hipMemcpyDeviceToHost, c->ctx->stream));
+    VT_HIP(hipStreamSynchronize(c->ctx->stream));
+    return VT_OK;
+}
+
+int vt_cache_set_params(vt_cache* c, const float* p)
+{
+    if (!c || !p) return fail(VT_ERR_ARG, "null argument");
+    VT_HIP(hipSetDevice(c->ctx->device));
+    hipStream_t s = c->ctx->stream;
+    VT_HIP(hipMemcpyAsync(c->d_params, p, (size_t)vt::MLP_N_PARAMS * 4, hipMemcpyHostToDevice, s));
+    VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_grad, vt::MLP_N_PARAMS, 0u, s));
+    VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_m, vt::MLP_N_PARAMS, 0u, s));
+    VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_v, vt::MLP_N_PARAMS, 0u, s));
+    c->step = 0;
+    int r = cache_repack(c);
+    if (r != VT_OK) return r;
+    VT_HIP(hipStreamSynchronize(s));
+    return VT_OK;
+}
+
+int vt_cache_infer(vt_cache* c, int64_t n, const float* raw, float* out)
+{
+    if (!c || n < 0 || (n > 0 && (!raw || !out))) return fail(VT_ERR_ARG, "bad argument");
+    if (n == 0) return VT_OK;
+    VT_HIP(hipSetDevice(c->ctx->device));
+    hipStream_t s = c->ctx->stream;
+    if (n > c->icap) {
+        dfree(c->d_iraw); dfree(c->d_iout);
+        c->icap = 0;
+        VT_HIP(hipMalloc(&c->d_iraw, (size_t)n * vt::MLP_RAW * 4));
+        VT_HIP(hipMalloc(&c->d_iout, (size_t)n * vt::MLP_OUT * 4));
+        c->icap = n;
+    }
+    VT_HIP(hipMemcpyAsync(c->d_iraw, raw, (size_t)n * vt::MLP_RAW * 4, hipMemcpyHostToDevice, s));
+    VT_HIP(vt::launch_mlp_infer(c->d_pack, c->d_bias, c->d_iraw, c->d_iout, n, c->ctx->cus, s));
+    VT_HIP(hipMemcpyAsync(out, c->d_iout, (size_t)n * vt::MLP_OUT * 4, hipMemcpyDeviceToHost, s));
+    VT_HIP(hipStreamSynchronize(s));
+    return VT_OK;
+}
+
+static int cache_upload_samples(vt_cache* c, int64_t n, const float* raw, const float* target)
+{
+    int r = cache_scratch(c, n);
+    if (r != VT_OK) return r;
+    hipStream_t s = c->ctx->stream;
+    VT_HIP(hipMemcpyAsync(c->d_raw, raw, (size_t)n * vt::MLP_RAW * 4, hipMemcpyHostToDevice, s));
+    VT_HIP(hipMemcpyAsync(c->d_tgt, target, (size_t)n * vt::MLP_OUT * 4, hipMemcpyHostToDevice, s));
+    return VT_OK;
+}
+
+int vt_cache_gradient(vt_cache* c, int64_t n, const float* raw, const float* target, float* grad, float* loss)
+{
+    if (!c || n <= 0 || !raw || !target || !grad) return fail(VT_ERR_ARG, "bad argument");
+    VT_HIP(hipSetDevice(c->ctx->device));
+    int r = cache_upload_samples(c, n, raw, target);
+    if (r == VT_OK) r = cache_grad_device(c, n, c->d_raw, c->d_tgt, loss);
+    if (r != VT_OK) return r;
+    VT_HIP(hipMemcpyAsync(grad, c->d_grad, (size_t)vt::MLP_N_PARAMS * 4, hipMemcpyDeviceToHost, c->ctx->stream));
+    VT_HIP(hipStreamSynchronize(c->ctx->stream));
+    return VT_OK;
+}
+
+int vt_cache_train(vt_cache* c, int64_t n, const float* raw, const float* target, float lr, float* loss)
+{
+    if (!c || n <= 0 || !raw || !target) return fail(VT_ERR_ARG, "bad argument");
+    if (!(lr > 0.0f)) return fail(VT_ERR_ARG, "learning rate must be positive");
+    VT_HIP(hipSetDevice(c->ctx->device));
+    int r = cache_upload_samples(c, n, raw, target);
+    if (r == VT_OK) r = cache_grad_device(c, n, c->d_raw, c->d_tgt, loss);
+    if (r == VT_OK) r = cache_adam(c, lr);
+    if (r != VT_OK) return r;
+    VT_HIP(hipStreamSynchronize(c->ctx->stream));
+    return VT_OK;
+}
+
+int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uint32_t count, int32_t cache_bounce,
+                     uint32_t train_paths, float lr, uint32_t batch, vt_cache_stats* out)
+{
+    if (!c || !k || k->ctx != c) return fail(VT_ERR_ARG, "cache must belong to the context");
+    if (!c->has_volume || !c->d_accum) return fail(VT_ERR_STATE, "no volume or no frame");
+    if (cache_bounce < -1 || cache_bounce > 64) return fail(VT_ERR_ARG, "cache_bounce must be in [-1, 64]");
+    if (train_paths > 0 && !(lr > 0.0f)) return fail(VT_ERR_ARG, "learning rate must be positive");
+    if (batch == 0) batch = 16384;
+    VT_HIP(hipSetDevice(c->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = c->stream;
+    const uint64_t WH = (uint64_t)c->W * (uint64_t)c->H;
+    if (WH > 0xffffffffull) return fail(VT_ERR_LIMIT, "frame too large");
+    int r = ensure_wavefront(c, std::max<uint64_t>(WH, std::min<uint64_t>(WH, c->max_paths)));
+    if (r != VT_OK) return r;
+    /* buffers */
+    if (k->qcap < WH) {
+        dfree(k->d_qraw); dfree(k->d_qout); dfree(k->d_qid);
+        k->qcap = 0;
+        VT_HIP(hipMalloc(&k->d_qraw, WH * 12 * 4));
+        VT_HIP(hipMalloc(&k->d_qout, WH * 3 * 4));
+        VT_HIP(hipMalloc(&k->d_qid, WH * 4));
+        k->qcap = WH;
+    }
+    if (!k->d_counts) VT_HIP(hipMalloc(&k->d_counts, 16));
+    const uint32_t maxv = (uint32_t)std::max(1, c->st.max_bounces);
+    const uint32_t stride = train_paths ? (uint32_t)std::max<uint64_t>(1, WH / train_paths) : (uint32_t)WH + 1u;
+    const uint64_t slots = train_paths ? (WH + stride - 1) / stride : 0;
+    if (k->vcap < slots * maxv || k->vslots < slots) {
+        dfree(k->d_vrec); dfree(k->d_vrec_n);
+        k->vcap = k->vslots = 0;
+        VT_HIP(hipMalloc(&k->d_vrec, std::max<uint64_t>(1, slots * maxv) * 18 * 4));
+        VT_HIP(hipMalloc(&k->d_vrec_n, std::max<uint64_t>(1, slots) * 4));
+        k->vcap = slots * maxv;
+        k->vslots = slots;
+    }
+    const uint64_t scap = std::max<uint64_t>(1, slots * maxv);
+    if (k->scap < scap) {
+        dfree(k->d_sraw); dfree(k->d_stgt);
+        k->scap = 0;
+        VT_HIP(hipMalloc(&k->d_sraw, scap * 12 * 4));
+        VT_HIP(hipMalloc(&k->d_stgt, scap * 3 * 4));
+        k->scap = scap;
+    }
+    vt_cache_stats st;
+    std::memset(&st, 0, sizeof(st));
+    for (uint32_t pass = 0; pass < count; ++pass) {
+        const uint32_t n = n0 + pass;
+        vt::TraceArgs a = trace_args(c);
+        a.seed = seed;
+        a.n0 = n;
+        a.count = 1;
+        a.rng_frame_max = 0xffffffffu;
+        a.stats = nullptr;
+        vt::CacheRender C;
+        std::memset(&C, 0, sizeof(C));
+        C.cache_bounce = cache_bounce;
+        C.train_stride = stride;
+        /* the training paths move across the image from pass to pass */
+        C.train_offset = (uint32_t)(((uint64_t)n * 2654435761u) % stride);
+        C.train_slots = (uint32_t)slots;
+        C.max_vertices = maxv;
+        C.vrec = k->d_vrec;
+        C.vrec_n = k->d_vrec_n;
+        C.q_raw = k->d_qraw;
+        C.q_id = k->d_qid;
+        C.q_out = k->d_qout;
+        C.q_count = k->d_counts;
+        C.s_raw = k->d_sraw;
+        C.s_tgt = k->d_stgt;
+        C.s_count = k->d_counts + 1;
+        C.s_cap = (uint32_t)k->scap;
+        VT_HIP(vt::launch_fill_u32(k->d_counts, 4, 0u, s));
+        VT_HIP(vt::launch_render_cached(a, c->wf, C, k->d_pack, k->d_bias, c->cus, s));
+        uint32_t counts[2] = { 0, 0 };
+        VT_HIP(hipMemcpyAsync(counts, k->d_counts, 8, hipMemcpyDeviceToHost, s));
+        VT_HIP(hipStreamSynchronize(s));
+        st.queries += counts[0];
+        const uint64_t ns = std::min<uint64_t>(counts[1], k->scap);
+        for (uint64_t off = 0; train_paths && off < ns; off += batch) {
+            const int64_t nb = (int64_t)std::min<uint64_t>(batch, ns - off);
+            r = cache_grad_device(k, nb, k->d_sraw + off * 12, k->d_stgt + off * 3, &st.last_loss);
+            if (r == VT_OK) r = cache_adam(k, lr);
+            if (r != VT_OK) return r;
+            st.steps++;
+            st.samples += (uint64_t)nb;
+        }
+    }
+    VT_HIP(hipStreamSynchronize(s));
+    st.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->timed = false;
+    if (out) *out = st;
     return VT_OK;
 }
 

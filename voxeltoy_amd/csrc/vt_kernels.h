@@ -85,7 +85,31 @@ struct BatchTable {
     unsigned long long* stats;   /* 6 counters or null */
 };
 
+/* Radiance-cache rendering of one pass: paths that are not training paths
+ * end into the cache at their vertex of bounce cache_bounce (query appended
+ * to q_*); training paths (path index % train_stride == train_offset, slot =
+ * index / train_stride) run the plain integrator and record each shaded
+ * vertex (query, throughput, radiance so far: 18 floats), and when they end
+ * append one training sample per vertex: target = (R_end - R_v) / T_v. */
+struct CacheRender {
+    int32_t cache_bounce;
+    uint32_t train_stride, train_offset, train_slots, max_vertices;
+    float* vrec;            /* [train_slots][max_vertices][18] */
+    uint32_t* vrec_n;       /* [train_slots] */
+    float* q_raw;           /* [paths][12] */
+    uint32_t* q_id;
+    float* q_out;           /* [paths][3] */
+    uint32_t* q_count;
+    float* s_raw;           /* [s_cap][12] */
+    float* s_tgt;           /* [s_cap][3] */
+    uint32_t* s_count;
+    uint32_t s_cap;
+};
+
 size_t render_lds_bytes(const TraceArgs& a);
+/* One pass (a.count must be 1) of cache rendering into a.accum. */
+hipError_t launch_render_cached(const TraceArgs& a, const WfPaths& P, const CacheRender& C, const void* pack,
+                                const float* bias, int cus, hipStream_t s);
 /* Wavefront render of every scene of B (paths of all scenes share the
  * queues; rounds follow the largest max_bounces, paths of scenes with fewer
  * bounces finish earlier).  a0 supplies the traversal tuning. */

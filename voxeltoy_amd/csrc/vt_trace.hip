@@ -12,6 +12,7 @@
  * (when they fit) and the environment's marginal CDF.
  */
 #include "vt_kernels.h"
+#include "vt_cache.h"
 
 namespace vt {
 
@@ -55,11 +56,17 @@ __device__ __forceinline__ bool start_path(const TraceArgs& a, int32_t px, int32
     return true;
 }
 
-/* Consume the traversal result of the ray in S; returns true when S holds a
- * new ray to trace, false when the path is finished. */
-template <bool STATS>
-__device__ __forceinline__ bool shade_step(const TraceArgs& a, const MatRec* recs, const uint32_t* bits,
-                                           const float* cdfv, PathState& S, const Hit& h, int kind, Counters& cnt)
+/* Consume the traversal result of the ray in S; returns 1 when S holds a
+ * new ray to trace, 0 when the path is finished.  With VERT, a shaded
+ * vertex also yields the radiance-cache query of the vertex in vraw (12
+ * floats: position in volume units [0,1]^3, (wo+1)/2, (normal+1)/2, albedo)
+ * with *vrec set, taken after the bounce-0 emission and before any random
+ * draw; when the vertex is at bounce `cache_bounce` the step returns 2
+ * there instead (the path ends into the cache). */
+template <bool STATS, bool VERT = false>
+__device__ __forceinline__ int shade_step(const TraceArgs& a, const MatRec* recs, const uint32_t* bits,
+                                          const float* cdfv, PathState& S, const Hit& h, int kind, Counters& cnt,
+                                          int cache_bounce = -1, float* vraw = nullptr, bool* vrec = nullptr)
 {
     const GridView& g = a.g;
     const Settings& st = a.s;
@@ -104,6 +111,22 @@ __device__ __forceinline__ bool shade_step(const TraceArgs& a, const MatRec* rec
     const f3 wo = -S.ray_d;
     const f3 lsWo = world_to_local(wo, B);
     if (S.bounces == 0) S.radiance = S.radiance + S.throughput * mk(m.e[0], m.e[1], m.e[2]);
+    if (VERT) {
+        vraw[0] = (B.p.x - g.bmin[0]) * g.ext[0];
+        vraw[1] = (B.p.y - g.bmin[1]) * g.ext[1];
+        vraw[2] = (B.p.z - g.bmin[2]) * g.ext[2];
+        vraw[3] = (wo.x + 1.0f) * 0.5f;
+        vraw[4] = (wo.y + 1.0f) * 0.5f;
+        vraw[5] = (wo.z + 1.0f) * 0.5f;
+        vraw[6] = (B.n.x + 1.0f) * 0.5f;
+        vraw[7] = (B.n.y + 1.0f) * 0.5f;
+        vraw[8] = (B.n.z + 1.0f) * 0.5f;
+        vraw[9] = m.r[0];
+        vraw[10] = m.r[1];
+        vraw[11] = m.r[2];
+        *vrec = true;
+        if (S.bounces == cache_bounce) return 2;
+    }
     /* light sample (directLighting :77-124) */
     float u[4];
     vt_rand4(&S.rng, u);
@@ -426,6 +449,29 @@ __device__ __forceinline__ void block_append(bool keep, uint32_t id, uint32_t ta
     __syncthreads();
 }
 
+/* Workgroup-aggregated reservation of one entry of a counter for every
+ * thread with `want`; returns the entry (undefined without `want`).  Every
+ * thread of the workgroup must call it. */
+template <int NT>
+__device__ __forceinline__ uint32_t block_reserve(bool want, uint32_t* counter)
+{
+    __shared__ uint32_t rsum[NT / 64];
+    __shared__ uint32_t rbase;
+    const unsigned long long mask = __ballot(want);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) rsum[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < NT / 64; ++w) { const uint32_t c = rsum[w]; rsum[w] = t; t += c; }
+        rbase = t ? atomicAdd(counter, t) : 0u;
+    }
+    __syncthreads();
+    const uint32_t idx = rbase + rsum[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();
+    return idx;
+}
+
 #define WF_GENERATE_THREADS 1024
 #define WF_SHADE_THREADS 512
 
@@ -444,9 +490,10 @@ __device__ __forceinline__ uint32_t scene_of(const uint64_t* __restrict__ base, 
 /* Camera paths of the chunk [g0, g0 + n) of the launch's (scene,) pass and
  * pixel space: a path index enumerates passes outermost and pixels row by
  * row within a pass (within a scene, scenes back to back in a batch). */
-template <bool STATS, bool BATCH>
+template <bool STATS, bool BATCH, bool CACHE = false>
 __global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceArgs a0, BatchTable B, WfPaths P,
-                                                                         uint64_t g0, uint32_t n)
+                                                                         uint64_t g0, uint32_t n,
+                                                                         CacheRender C = CacheRender())
 {
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -467,8 +514,13 @@ __global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceA
             const uint32_t nn = a.n0 + k;
             const uint32_t rf = nn < a.rng_frame_max ? nn : a.rng_frame_max;
             PathState S;
+            uint32_t tag = sid;
+            if (CACHE && gi % C.train_stride == C.train_offset && gi / C.train_stride < C.train_slots) {
+                tag = (uint32_t)(gi / C.train_stride) + 1u;   /* training slot + 1 */
+                C.vrec_n[tag - 1u] = 0u;
+            }
             if (start_path<STATS>(a, px, py, rf, S, cnt)) {
-                store_generated(P, i, S, pix, k, sid);
+                store_generated(P, i, S, pix, k, tag);
                 keep = true;
                 ro = S.ray_o;
                 rd = S.ray_d;
@@ -574,8 +626,37 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
     flush_counters<STATS>(BATCH ? B.stats : a.stats, cnt);
 }
 
-template <bool STATS, int PHASE, bool BATCH>
-__global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0, BatchTable B, WfPaths P, int sel)
+/* Training samples of a finished training path: one per recorded vertex,
+ * target = (radiance at the end - radiance at the vertex) / throughput at
+ * the vertex, per channel; vertices with a zero throughput channel give no
+ * sample. */
+__device__ __forceinline__ void emit_training(const CacheRender& C, uint32_t slot, f3 r_end)
+{
+    const uint32_t nv = C.vrec_n[slot];
+    const float* rec = C.vrec + (size_t)slot * C.max_vertices * 18u;
+    uint32_t ok = 0;
+    for (uint32_t v = 0; v < nv; ++v) {
+        const float* e = rec + v * 18u;
+        ok += (e[12] > 0.0f && e[13] > 0.0f && e[14] > 0.0f) ? 1u : 0u;
+    }
+    if (!ok) return;
+    uint32_t base = atomicAdd(C.s_count, ok);
+    for (uint32_t v = 0; v < nv; ++v) {
+        const float* e = rec + v * 18u;
+        if (!(e[12] > 0.0f && e[13] > 0.0f && e[14] > 0.0f)) continue;
+        if (base < C.s_cap) {
+            for (int k = 0; k < 12; ++k) C.s_raw[(size_t)base * 12u + k] = e[k];
+            C.s_tgt[(size_t)base * 3u + 0] = (r_end.x - e[15]) / e[12];
+            C.s_tgt[(size_t)base * 3u + 1] = (r_end.y - e[16]) / e[13];
+            C.s_tgt[(size_t)base * 3u + 2] = (r_end.z - e[17]) / e[14];
+        }
+        base++;
+    }
+}
+
+template <bool STATS, int PHASE, bool BATCH, bool CACHE = false>
+__global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0, BatchTable B, WfPaths P, int sel,
+                                                                    CacheRender C = CacheRender())
 {
     extern __shared__ __align__(16) uint8_t smem[];
     MatRec* recs0 = nullptr;
@@ -590,9 +671,10 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t j = base + threadIdx.x;
-        bool keep = false;
+        bool keep = false, query = false;
         uint32_t id = 0, sid = 0;
         f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
+        float vraw[12];
         if (j < n) {
             id = __float_as_uint(qa[j].w);
             if (BATCH) sid = P.meta[id].w;
@@ -607,14 +689,40 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0
             h.x = hv.x; h.y = hv.y; h.z = hv.z; h.slot = (uint32_t)hv.w & 0xffu;
             h.steps = 0;
             const int kind = hv.w >> 8;
-            keep = shade_step<STATS>(a, recs, bits, cdfv, S, h, kind, cnt);
+            const uint32_t slot1 = CACHE ? P.meta[id].w : 0u;   /* training slot + 1 */
+            bool vrec = false;
+            const int st = shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, kind, cnt,
+                                                    slot1 ? -1 : C.cache_bounce, vraw, &vrec);
+            if (CACHE && vrec && slot1) {
+                const uint32_t slot = slot1 - 1u, nv = C.vrec_n[slot];
+                if (nv < C.max_vertices) {
+                    float* e = C.vrec + ((size_t)slot * C.max_vertices + nv) * 18u;
+                    for (int k = 0; k < 12; ++k) e[k] = vraw[k];
+                    e[12] = S.throughput.x; e[13] = S.throughput.y; e[14] = S.throughput.z;
+                    e[15] = S.radiance.x; e[16] = S.radiance.y; e[17] = S.radiance.z;
+                    C.vrec_n[slot] = nv + 1u;
+                }
+            }
+            keep = st == 1;
+            query = CACHE && st == 2;
             if (keep) {
                 store_phase<PHASE>(P, id, S);
                 ro = S.ray_o;
                 rd = S.ray_d;
+            } else if (query) {
+                P.rad[id] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
+                P.thr[id] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
             } else {
                 const f3 c = tonemap(a.s, S.radiance);
                 P.smp[id] = make_float4(c.x, c.y, c.z, 1.0f);
+                if (CACHE && slot1) emit_training(C, slot1 - 1u, S.radiance);
+            }
+        }
+        if (CACHE) {
+            const uint32_t qi = block_reserve<WF_SHADE_THREADS>(query, C.q_count);
+            if (query) {
+                for (int k = 0; k < 12; ++k) C.q_raw[(size_t)qi * 12u + k] = vraw[k];
+                C.q_id[qi] = id;
             }
         }
         block_append<WF_SHADE_THREADS>(keep, id, sid, ro, rd, qao, qbo, &P.cnt[sel ^ 1]);
@@ -739,6 +847,71 @@ hipError_t launch_render_batch(const TraceArgs& a0, const BatchTable& B, const W
     const int rounds = 1 + 2 * (max_bounces > 0 ? max_bounces : 0);
     if (stats) return wf_run<true, false, true>(a0, B, P, total_paths, total_pixels, rounds, cus, s);
     return wf_run<false, false, true>(a0, B, P, total_paths, total_pixels, rounds, cus, s);
+}
+
+/* Paths that ended into the cache: radiance += throughput * max(0, cache). */
+__global__ __launch_bounds__(256) void cache_apply_kernel(TraceArgs a, WfPaths P, CacheRender C)
+{
+    const uint32_t n = *C.q_count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t id = C.q_id[i];
+        const float4 r = P.rad[id], t = P.thr[id];
+        const float y0 = C.q_out[(size_t)i * 3u], y1 = C.q_out[(size_t)i * 3u + 1], y2 = C.q_out[(size_t)i * 3u + 2];
+        /* a channel the cache predicts as 0 (or less) adds nothing */
+        const f3 rad = mk(y0 > 0.0f ? r.x + t.x * y0 : r.x, y1 > 0.0f ? r.y + t.y * y1 : r.y,
+                          y2 > 0.0f ? r.z + t.z * y2 : r.z);
+        const f3 c = tonemap(a.s, rad);
+        P.smp[id] = make_float4(c.x, c.y, c.z, 1.0f);
+    }
+}
+
+template <bool LDSB>
+static hipError_t wf_run_cached(const TraceArgs& a, const WfPaths& P, const CacheRender& C, const void* pack,
+                                const float* bias, int cus, hipStream_t s)
+{
+    const BatchTable B = { nullptr, nullptr, nullptr, 0u, nullptr };
+    const size_t lds_t = render_lds_bytes(a);
+    const size_t lds_s = 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
+    const int bt = blocks_per_cu(wf_traverse_kernel<false, LDSB, false>, WF_TRAVERSE_THREADS, lds_t);
+    const int bs0 = blocks_per_cu(wf_shade_kernel<false, PH_PRIMARY, false, true>, WF_SHADE_THREADS, lds_s);
+    const int bs1 = blocks_per_cu(wf_shade_kernel<false, PH_SHADOW, false, true>, WF_SHADE_THREADS, lds_s);
+    const int bs2 = blocks_per_cu(wf_shade_kernel<false, PH_CONT, false, true>, WF_SHADE_THREADS, lds_s);
+    const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
+    if (WH > P.cap) return hipErrorInvalidValue;
+    const uint32_t n = (uint32_t)WH;
+    const int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
+    const int qround = C.cache_bounce >= 0 ? 2 * C.cache_bounce : -1;
+    wf_reset_kernel<<<1, 64, 0, s>>>(P.cnt);
+    const uint32_t gn = (n + WF_GENERATE_THREADS - 1) / WF_GENERATE_THREADS;
+    const unsigned gg = (unsigned)(gn < (uint32_t)cus * 2u ? gn : (uint32_t)cus * 2u);
+    wf_generate_kernel<false, false, true><<<gg, WF_GENERATE_THREADS, 0, s>>>(a, B, P, 0, n, C);
+    int sel = 0;
+    for (int r = 0; r < rounds; ++r) {
+        wf_traverse_kernel<false, LDSB, false><<<cus * bt, WF_TRAVERSE_THREADS, lds_t, s>>>(a, B, P, sel);
+        if (r == 0)
+            wf_shade_kernel<false, PH_PRIMARY, false, true><<<cus * bs0, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel, C);
+        else if (r & 1)
+            wf_shade_kernel<false, PH_SHADOW, false, true><<<cus * bs1, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel, C);
+        else
+            wf_shade_kernel<false, PH_CONT, false, true><<<cus * bs2, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel, C);
+        if (r == qround) {
+            hipError_t e = launch_mlp_infer_dev((const _Float16*)pack, bias, C.q_raw, C.q_out, C.q_count, n, cus, s);
+            if (e != hipSuccess) return e;
+            cache_apply_kernel<<<(unsigned)cus * 4u, 256, 0, s>>>(a, P, C);
+        }
+        sel ^= 1;
+    }
+    const uint64_t gp = (WH + 255) / 256;
+    const unsigned ga = (unsigned)(gp < (uint64_t)cus * 8u ? gp : (uint64_t)cus * 8u);
+    wf_accumulate_kernel<false><<<ga, 256, 0, s>>>(a, B, P, 0, WH, WH);
+    return hipGetLastError();
+}
+
+hipError_t launch_render_cached(const TraceArgs& a, const WfPaths& P, const CacheRender& C, const void* pack,
+                                const float* bias, int cus, hipStream_t s)
+{
+    return a.lds_bits ? wf_run_cached<true>(a, P, C, pack, bias, cus, s)
+                      : wf_run_cached<false>(a, P, C, pack, bias, cus, s);
 }
 
 /* First hit of the primary ray of every pixel (the parity probe). */
