@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s; time-to-RMSE 1e-2 vs FP64 ref, plain vs batched+radiance-cache"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel on this workload, from
+# rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a copy kernel of known size);
+# None until profiled.
+TRAFFIC: dict = {}
 
 
 def env_lookup_bytes(cu_w: int, cv_n: int) -> tuple[int, int]:
@@ -90,6 +94,8 @@ def main():
                          "reference, plain tracer against the radiance cache")
     ap.add_argument("--ref-spp", type=int, default=4096, help="cache workload: passes of the FP64 reference")
     ap.add_argument("--budget-s", type=float, default=3.0, help="cache workload: render time of each contender")
+    ap.add_argument("--cache-bounce", type=int, default=2, help="cache workload: bounce at which paths end")
+    ap.add_argument("--train-paths", type=int, default=16384, help="cache workload: training paths per frame")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the path shards by "
@@ -121,13 +127,18 @@ def main():
         ctx.render(wl.seed, 0, wl.spp)
     ctx.synchronize()
 
-    kernel_ms = []
+    # traversal launches are bracketed by HIP events on the render stream
+    ctx.set_launch_timing(args.mode == 1)
+    kernel_ms, trav_ms, trav_launches = [], [], []
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.reset_accum()
         ctx.render(wl.seed, 0, wl.spp)
         kernel_ms.append(ctx.stats()["kernel_ms"])
+        lt = ctx.launch_timing()
+        trav_ms.append(lt["traverse_ms"])
+        trav_launches.append(lt["launches"])
     ctx.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1000.0 / args.steps
@@ -136,7 +147,16 @@ def main():
     rays = st["rays"]
     value = rays / (ms_per_step / 1000.0) / 1e6
     bytes_step = algorithmic_bytes(st, cu_w, cv_n)
-    achieved = bytes_step / (k_ms / 1000.0) / 1e9
+    render_gbs = bytes_step / (k_ms / 1000.0) / 1e9
+    if args.mode == 1 and sum(trav_launches) > 0:
+        # dominant kernel: the traversal (1 B per DDA voxel visit)
+        n_launch = sum(trav_launches) / len(trav_launches)
+        launch_ms = sum(trav_ms) / sum(trav_launches)
+        launch_bytes = st["visits"] / n_launch
+        kernel_name, achieved = "wf_traverse_kernel", launch_bytes / (launch_ms / 1000.0) / 1e9
+    else:
+        n_launch, launch_ms, launch_bytes = 1, k_ms, bytes_step
+        kernel_name, achieved = "render_kernel", render_gbs
     cpu = None if args.no_cpu_baseline else cpu_baseline(wl, args.cpu_budget)
 
     out = {
@@ -162,8 +182,11 @@ def main():
                                                              ("visits", "interactions", "samples", "env_samples",
                                                               "env_evals")}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": int(bytes_step)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(kernel_name),
+                     "kernel": kernel_name, "launches_per_step": round(n_launch, 2),
+                     "avg_launch_ms": round(launch_ms, 4), "algorithmic_bytes_per_launch": int(launch_bytes),
+                     "render_algorithmic_bytes_per_step": int(bytes_step),
+                     "render_achieved_gbs": round(render_gbs, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -302,7 +325,11 @@ def bench_cache(args, api, workloads, target=1e-2):
     cache = api.RadianceCache(cache_ctx, seed=1)
 
     def cache_step(n):
-        return cache.render(wl.seed, n, 1, cache_bounce=2, train_paths=16384, lr=3e-3)["ms"]
+        st = cache.render(wl.seed, n, 1, cache_bounce=args.cache_bounce, train_paths=args.train_paths, lr=3e-3)
+        stats.append(st)
+        return st["ms"]
+
+    stats = []
 
     cached = run(cache_step, cache_ctx.read_accum)
     tp, tc = plain["time_to_target_s"], cached["time_to_target_s"]
@@ -321,7 +348,9 @@ def bench_cache(args, api, workloads, target=1e-2):
         "data": "synthetic",
         "config": {"workload": "full-shading 256^3 scene at 960x540, 1 spp per frame; FP64 reference of "
                                f"{args.ref_spp} spp; target RMSE {target} (tonemapped mean, pixels x RGB)",
-                   "plain": plain, "cache": cached,
+                   "plain": plain, "cache": cached, "cache_bounce": args.cache_bounce,
+                   "train_paths_per_frame": args.train_paths,
+                   "cache_last_frame": stats[-1] if stats else None,
                    "speedup_time_to_target": None if (tp is None or tc is None) else round(tp / tc, 3)},
         "roofline": None,
         "cpu_baseline": None,

@@ -160,6 +160,13 @@ int vt_read_accum(vt_context* ctx, float* rgba);
 /* FP64 running sums (W*H*3 doubles) when vt_set_frame(fp64=1). */
 int vt_read_sum64(vt_context* ctx, double* rgb);
 int vt_get_stats(vt_context* ctx, vt_stats* out);
+/* Per-launch timing of the traversal kernel: when enabled, each wavefront
+ * vt_render brackets every traversal launch (up to 256 per call) with HIP
+ * events on the context's stream; vt_get_launch_timing returns the summed
+ * durations (ms) and the number of timed launches of the last vt_render.
+ * Measurement only (no counterpart in the reference). */
+int vt_set_launch_timing(vt_context* ctx, int32_t enabled);
+int vt_get_launch_timing(vt_context* ctx, double* traverse_ms, uint32_t* launches);
 
 /* ---- batches -------------------------------------------------------------
  * Many scenes (contexts on one device, each with its own volume, camera,

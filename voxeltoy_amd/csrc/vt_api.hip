@@ -32,6 +32,7 @@ int fail(int code, const std::string& msg)
 
 /* Wavefront defaults (vt_set_render_mode, vt_set_traversal_tuning). */
 #define VT_DEFAULT_MAX_PATHS (16ull << 20)
+#define VT_TIMED_LAUNCHES 256     /* traversal launches timed per render call */
 #define VT_DEFAULT_DDA_BUDGET 32
 #define VT_DEFAULT_REFILL_MIN 8
 #define VT_DEFAULT_REFILL_CHUNK 256
@@ -59,6 +60,10 @@ struct vt_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
+    /* per-launch traversal timing (wavefront mode) */
+    std::vector<hipEvent_t> lt_ev;
+    int lt_used = 0;
+    bool lt_on = false;
 
     /* volume */
     bool has_volume = false;
@@ -386,6 +391,7 @@ void vt_destroy(vt_context* c)
     free_wavefront(c);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->lt_ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -609,6 +615,35 @@ int vt_set_stats(vt_context* c, int32_t enabled)
     return VT_OK;
 }
 
+int vt_set_launch_timing(vt_context* c, int32_t enabled)
+{
+    if (!c) return fail(VT_ERR_ARG, "null context");
+    VT_HIP(hipSetDevice(c->device));
+    if (enabled && c->lt_ev.empty()) {
+        c->lt_ev.assign(2 * VT_TIMED_LAUNCHES, nullptr);
+        for (hipEvent_t& e : c->lt_ev) VT_HIP(hipEventCreate(&e));
+    }
+    c->lt_on = enabled != 0;
+    c->lt_used = 0;
+    return VT_OK;
+}
+
+int vt_get_launch_timing(vt_context* c, double* traverse_ms, uint32_t* launches)
+{
+    if (!c || !traverse_ms || !launches) return fail(VT_ERR_ARG, "null argument");
+    VT_HIP(hipSetDevice(c->device));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    double sum = 0.0;
+    for (int k = 0; k < c->lt_used; ++k) {
+        float ms = 0.0f;
+        VT_HIP(hipEventElapsedTime(&ms, c->lt_ev[2 * k], c->lt_ev[2 * k + 1]));
+        sum += ms;
+    }
+    *traverse_ms = sum;
+    *launches = (uint32_t)c->lt_used;
+    return VT_OK;
+}
+
 int vt_set_render_mode(vt_context* c, int32_t mode, uint64_t max_paths)
 {
     if (!c) return fail(VT_ERR_ARG, "null context");
@@ -647,8 +682,10 @@ int vt_render(vt_context* c, uint64_t seed, uint32_t n0, uint32_t count, uint32_
         const uint64_t total = (uint64_t)c->W * c->H * count;
         int r = ensure_wavefront(c, std::min<uint64_t>(total, c->max_paths));
         if (r != VT_OK) return r;
+        c->lt_used = 0;
+        const vt::LaunchTimer lt = { c->lt_ev.data(), (int)c->lt_ev.size() / 2, &c->lt_used };
         VT_HIP(hipEventRecord(c->ev0, c->stream));
-        VT_HIP(vt::launch_render_wavefront(a, c->wf, c->cus, c->stream, c->stats_enabled));
+        VT_HIP(vt::launch_render_wavefront(a, c->wf, c->cus, c->stream, c->stats_enabled, c->lt_on ? &lt : nullptr));
         VT_HIP(hipEventRecord(c->ev1, c->stream));
     } else {
         const int per_cu = vt::render_blocks_per_cu(a, c->stats_enabled);

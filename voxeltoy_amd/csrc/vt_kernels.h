@@ -115,9 +115,18 @@ hipError_t launch_render_cached(const TraceArgs& a, const WfPaths& P, const Cach
  * bounces finish earlier).  a0 supplies the traversal tuning. */
 hipError_t launch_render_batch(const TraceArgs& a0, const BatchTable& B, const WfPaths& P, uint64_t total_paths,
                                uint64_t total_pixels, int max_bounces, int cus, hipStream_t s, bool stats);
+/* Events recorded on the launch stream around traversal launches: pair k
+ * brackets the k-th timed launch (at most cap pairs; *used counts them). */
+struct LaunchTimer {
+    hipEvent_t* ev;
+    int cap;
+    int* used;
+};
+
 /* Wavefront render of the launch in a: path chunks of at most P.cap slots,
  * 1 + 2*max_bounces traverse/shade rounds per chunk. */
-hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats);
+hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats,
+                                   const LaunchTimer* timer = nullptr);
 hipError_t launch_render(const TraceArgs& a, int grid, hipStream_t s, bool stats);
 int render_blocks_per_cu(const TraceArgs& a, bool stats);
 hipError_t launch_first_hit(const TraceArgs& a, int32_t* out, hipStream_t s);

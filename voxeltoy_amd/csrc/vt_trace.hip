@@ -795,7 +795,7 @@ static int blocks_per_cu(K kernel, int threads, size_t lds)
  * the traversal tuning and the LDS sizes of scene 0). */
 template <bool STATS, bool LDSB, bool BATCH>
 static hipError_t wf_run(const TraceArgs& a, const BatchTable& B, const WfPaths& P, uint64_t total,
-                         uint64_t n_pixels, int rounds, int cus, hipStream_t s)
+                         uint64_t n_pixels, int rounds, int cus, hipStream_t s, const LaunchTimer* timer = nullptr)
 {
     const size_t lds_t = BATCH ? 0 : render_lds_bytes(a);
     const size_t lds_s = BATCH ? 0 : 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
@@ -812,7 +812,17 @@ static hipError_t wf_run(const TraceArgs& a, const BatchTable& B, const WfPaths&
         wf_generate_kernel<STATS, BATCH><<<gg, WF_GENERATE_THREADS, 0, s>>>(a, B, P, g0, n);
         int sel = 0;
         for (int r = 0; r < rounds; ++r) {
+            const bool timed = timer && *timer->used < timer->cap;
+            if (timed) {
+                const hipError_t e = hipEventRecord(timer->ev[2 * *timer->used], s);
+                if (e != hipSuccess) return e;
+            }
             wf_traverse_kernel<STATS, LDSB, BATCH><<<cus * bt, WF_TRAVERSE_THREADS, lds_t, s>>>(a, B, P, sel);
+            if (timed) {
+                const hipError_t e = hipEventRecord(timer->ev[2 * *timer->used + 1], s);
+                if (e != hipSuccess) return e;
+                ++*timer->used;
+            }
             if (r == 0)
                 wf_shade_kernel<STATS, PH_PRIMARY, BATCH><<<cus * bs0, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel);
             else if (r & 1)
@@ -828,17 +838,18 @@ static hipError_t wf_run(const TraceArgs& a, const BatchTable& B, const WfPaths&
     return hipGetLastError();
 }
 
-hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats)
+hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats,
+                                   const LaunchTimer* timer)
 {
     const BatchTable B = { nullptr, nullptr, nullptr, 0u, nullptr };
     const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
     const uint64_t total = WH * a.count;
     const int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
     if (stats)
-        return a.lds_bits ? wf_run<true, true, false>(a, B, P, total, WH, rounds, cus, s)
-                          : wf_run<true, false, false>(a, B, P, total, WH, rounds, cus, s);
-    return a.lds_bits ? wf_run<false, true, false>(a, B, P, total, WH, rounds, cus, s)
-                      : wf_run<false, false, false>(a, B, P, total, WH, rounds, cus, s);
+        return a.lds_bits ? wf_run<true, true, false>(a, B, P, total, WH, rounds, cus, s, timer)
+                          : wf_run<true, false, false>(a, B, P, total, WH, rounds, cus, s, timer);
+    return a.lds_bits ? wf_run<false, true, false>(a, B, P, total, WH, rounds, cus, s, timer)
+                      : wf_run<false, false, false>(a, B, P, total, WH, rounds, cus, s, timer);
 }
 
 hipError_t launch_render_batch(const TraceArgs& a0, const BatchTable& B, const WfPaths& P, uint64_t total_paths,
