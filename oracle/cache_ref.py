@@ -9,7 +9,12 @@ model defined in voxeltoy_amd/csrc/vt_cache.h:
   - MLP 288-64-64-64-64-64-3, ReLU hidden layers, linear output;
   - loss: mean over samples and channels of (y - t)^2 / (sg(y)^2 + 0.01);
   - the FP16-operand / FP32-accumulate inference emulated with exact
-    float64 products of float16-rounded operands.
+    float64 products of float16-rounded operands;
+  - the mixed-precision training step (forward as the inference, keeping
+    the float16 layer inputs; float32 loss gradient; backward and weight
+    gradients with the float32 weights and float64 sums) emulated the same
+    way, so the matrix-core training kernels can be held to float32
+    accumulation-order error.
 """
 from __future__ import annotations
 
@@ -109,3 +114,48 @@ def forward_fp16(p, x):
             return z.astype(np.float32)
         h = np.maximum(z, np.float32(0)).astype(np.float16).astype(np.float64)
     return h
+
+
+def _to_fp16(a):
+    return np.asarray(a).astype(np.float16).astype(np.float64)
+
+
+def loss_and_grad_mixed(p, x, t, r16=_to_fp16, with_output=False):
+    """Emulation of the training kernels: float16 weights and layer inputs
+    in the forward (float32 sums, float32 bias, ReLU, float16 rounding of
+    each activation), y and dL/dy in float32, then the backward through the
+    float32 weights with float64 sums, ReLU masks from the float16
+    activations, weight gradients from the float16 layer inputs.  Returns
+    (mean loss, gradient) (and y with with_output).  r16 is the float16
+    rounding (replaceable, so that without it the result can be held to the
+    float64 gradient)."""
+    p32 = np.asarray(p, np.float32)
+    layers = unpack(p32)
+    t = np.asarray(t, np.float32)
+    n = x.shape[0]
+    a = r16(np.asarray(x, np.float32))
+    acts = [a]
+    for li, (w, b) in enumerate(layers):
+        w16 = r16(w)
+        z = (a @ w16.T).astype(np.float32) + b.astype(np.float32)
+        if li == 5:
+            y = z.astype(np.float32)
+            break
+        a = r16(np.maximum(z, np.float32(0)))
+        acts.append(a)
+    w_ = (np.float32(1) / (y * y + np.float32(0.01))).astype(np.float32)
+    d = (y - t).astype(np.float32)
+    loss = float(np.sum((d.astype(np.float64) ** 2) * w_)) / (OUT * n)
+    inv = np.float32(1) / np.float32(OUT * n)
+    dz = (((np.float32(2) * d) * w_) * inv).astype(np.float64)
+    grads = []
+    for li in range(5, -1, -1):
+        w, _ = layers[li]
+        a_in = acts[li]
+        grads.append((dz.T @ a_in, dz.sum(0)))
+        if li > 0:
+            dh = dz @ w.astype(np.float64)
+            dz = dh * (acts[li] > 0)
+    grads.reverse()
+    g = np.concatenate([np.concatenate([gw.reshape(-1), gb]) for gw, gb in grads])
+    return (loss, g, y) if with_output else (loss, g)

@@ -1,7 +1,9 @@
 """CPU checks of the radiance-cache reference model (oracle/cache_ref.py):
 the analytic gradient against central finite differences, the encoding's
 first band against sin/cos and the recurrence against float64 sin(2^k pi v),
-and the float16 inference emulation against the float64 forward."""
+the float16 inference emulation against the float64 forward, and the
+mixed-precision training emulation: without its float16 rounding it is the
+float64 gradient, and its forward is the inference emulation."""
 from __future__ import annotations
 
 import math
@@ -90,3 +92,22 @@ def test_fp16_emulation_close_to_fp64():
     y64, _ = cr.forward64(p, x)
     y16 = cr.forward_fp16(p, x)
     assert np.max(np.abs(y16 - y64)) <= 0.02 * np.mean(np.abs(y64)) + 1e-3
+
+
+def test_mixed_training_emulation():
+    rng = np.random.default_rng(8)
+    p = small_params(rng)
+    n = 64
+    x = rng.uniform(-1, 1, (n, cr.IN)).astype(np.float32)
+    t = rng.uniform(0, 2, (n, 3)).astype(np.float32)
+    # no rounding: the float64 gradient (up to float32 y and dL/dy)
+    l0, g0 = cr.loss_and_grad_mixed(p, x, t, r16=lambda a: np.asarray(a, np.float64))
+    l64, g64 = cr.loss_and_grad64(p.astype(np.float32).astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    assert abs(l0 - l64) <= 1e-5 * l64
+    assert np.linalg.norm(g0 - g64) <= 1e-5 * np.linalg.norm(g64)
+    # the forward of the emulation is the inference emulation
+    _, _, y = cr.loss_and_grad_mixed(p, x, t, with_output=True)
+    assert np.array_equal(y, cr.forward_fp16(p, x))
+    # the rounding matters, but only at float16 precision
+    l1, g1 = cr.loss_and_grad_mixed(p, x, t)
+    assert 0 < np.linalg.norm(g1 - g64) <= 3e-2 * np.linalg.norm(g64)

@@ -860,6 +860,8 @@ int vt_batch_get_stats(vt_batch* b, vt_stats* out)
 }
 
 /* ---- learned radiance cache ------------------------------------------------------- */
+#define VT_CACHE_LOSS_SLOTS 1024
+
 struct vt_cache {
     vt_context* ctx = nullptr;
     float* d_params = nullptr;
@@ -868,11 +870,19 @@ struct vt_cache {
     float* d_v = nullptr;
     _Float16* d_pack = nullptr;
     float* d_bias = nullptr;
-    float* d_loss = nullptr;
-    int step = 0;
-    /* training scratch for up to `cap` samples */
+    float* d_tpack = nullptr;
+    float* d_part = nullptr;           /* [MLP_GRAD_SPLITS][MLP_N_PARAMS] */
+    float* d_loss = nullptr;           /* [VT_CACHE_LOSS_SLOTS] loss sum per batch */
+    uint32_t* d_steps = nullptr;       /* Adam step count */
+    uint32_t* d_nsamp = nullptr;       /* sample count of host-supplied batches */
+    unsigned long long* d_tally = nullptr;   /* queries, samples over a cached render */
+    /* training scratch for batches of up to 16 * tiles samples */
+    uint32_t tiles = 0;
+    _Float16 *d_X = nullptr, *d_H = nullptr;
+    float *d_DZ = nullptr, *d_DY = nullptr;
+    /* host-supplied samples */
     int64_t cap = 0;
-    float *d_X = nullptr, *d_H = nullptr, *d_DZ = nullptr, *d_DY = nullptr, *d_raw = nullptr, *d_tgt = nullptr;
+    float *d_raw = nullptr, *d_tgt = nullptr;
     /* inference staging */
     int64_t icap = 0;
     float *d_iraw = nullptr, *d_iout = nullptr;
@@ -893,51 +903,82 @@ namespace {
 
 int cache_repack(vt_cache* c)
 {
-    VT_HIP(vt::launch_mlp_pack(c->d_params, c->d_pack, c->d_bias, c->ctx->stream));
+    VT_HIP(vt::launch_mlp_pack(c->d_params, c->d_pack, c->d_bias, c->d_tpack, c->ctx->stream));
     return VT_OK;
 }
 
-int cache_scratch(vt_cache* c, int64_t n)
+/* training scratch for batches of up to `batch` samples */
+int cache_train_scratch(vt_cache* c, uint32_t batch)
 {
-    if (n <= c->cap) return VT_OK;
-    dfree(c->d_X); dfree(c->d_H); dfree(c->d_DZ); dfree(c->d_DY); dfree(c->d_raw); dfree(c->d_tgt);
-    c->cap = 0;
-    VT_HIP(hipMalloc(&c->d_X, (size_t)n * vt::MLP_IN * 4));
-    VT_HIP(hipMalloc(&c->d_H, (size_t)n * 5 * vt::MLP_W * 4));
-    VT_HIP(hipMalloc(&c->d_DZ, (size_t)n * 5 * vt::MLP_W * 4));
-    VT_HIP(hipMalloc(&c->d_DY, (size_t)n * vt::MLP_OUT * 4));
-    VT_HIP(hipMalloc(&c->d_raw, (size_t)n * vt::MLP_RAW * 4));
-    VT_HIP(hipMalloc(&c->d_tgt, (size_t)n * vt::MLP_OUT * 4));
-    c->cap = n;
+    const uint32_t tiles = (batch + 15u) / 16u;
+    if (tiles <= c->tiles) return VT_OK;
+    dfree(c->d_X); dfree(c->d_H); dfree(c->d_DZ); dfree(c->d_DY);
+    c->tiles = 0;
+    const size_t t16 = (size_t)tiles * 16u;
+    VT_HIP(hipMalloc(&c->d_X, t16 * vt::MLP_IN * 2));
+    VT_HIP(hipMalloc(&c->d_H, t16 * 5 * vt::MLP_W * 2));
+    VT_HIP(hipMalloc(&c->d_DZ, t16 * 5 * vt::MLP_W * 4));
+    VT_HIP(hipMalloc(&c->d_DY, t16 * vt::MLP_OUT * 4));
+    c->tiles = tiles;
     return VT_OK;
 }
 
-/* gradient of the loss over n device-resident samples into d_grad (zeroed
- * first); loss mean into *loss when non-null */
-int cache_grad_device(vt_cache* c, int64_t n, const float* d_raw, const float* d_tgt, float* loss)
+/* host samples -> device arrays, count -> d_nsamp */
+int cache_upload_samples(vt_cache* c, int64_t n, const float* raw, const float* target)
 {
     hipStream_t s = c->ctx->stream;
-    int r = cache_scratch(c, n);
-    if (r != VT_OK) return r;
-    VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_grad, vt::MLP_N_PARAMS, 0u, s));
-    VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_loss, 1, 0u, s));
-    VT_HIP(vt::launch_mlp_fwd_bwd(c->d_params, d_raw, d_tgt, n, c->d_X, c->d_H, c->d_DZ, c->d_DY, c->d_loss, s));
-    VT_HIP(vt::launch_mlp_grad(c->d_X, c->d_H, c->d_DZ, c->d_DY, n, c->d_grad, s));
-    if (loss) {
-        float l = 0.0f;
-        VT_HIP(hipMemcpyAsync(&l, c->d_loss, 4, hipMemcpyDeviceToHost, s));
-        VT_HIP(hipStreamSynchronize(s));
-        *loss = l / (float)(vt::MLP_OUT * n);
+    if (n > c->cap) {
+        dfree(c->d_raw); dfree(c->d_tgt);
+        c->cap = 0;
+        VT_HIP(hipMalloc(&c->d_raw, (size_t)n * vt::MLP_RAW * 4));
+        VT_HIP(hipMalloc(&c->d_tgt, (size_t)n * vt::MLP_OUT * 4));
+        c->cap = n;
     }
+    VT_HIP(hipMemcpyAsync(c->d_raw, raw, (size_t)n * vt::MLP_RAW * 4, hipMemcpyHostToDevice, s));
+    VT_HIP(hipMemcpyAsync(c->d_tgt, target, (size_t)n * vt::MLP_OUT * 4, hipMemcpyHostToDevice, s));
+    VT_HIP(vt::launch_fill_u32(c->d_nsamp, 1, (uint32_t)n, s));
     return VT_OK;
 }
 
-int cache_adam(vt_cache* c, float lr)
+/* One training batch: forward/backward, gradient GEMM, then the summed
+ * gradient into `grad` (grad != 0, Adam state untouched) or an Adam step
+ * with learning rate lr.  The batch's loss sum is added to d_loss[slot]. */
+int cache_train_batch(vt_cache* c, const vt::TrainBatch& b, float* grad, float lr, int slot)
 {
-    c->step++;
-    VT_HIP(vt::launch_adam(c->d_params, c->d_grad, c->d_m, c->d_v, vt::MLP_N_PARAMS, lr, 0.9f, 0.99f, 1e-8f, c->step,
-                           c->ctx->stream));
-    return cache_repack(c);
+    hipStream_t s = c->ctx->stream;
+    int r = cache_train_scratch(c, b.batch);
+    if (r != VT_OK) return r;
+    vt::TrainScratch S;
+    S.X = c->d_X; S.H = c->d_H; S.DZ = c->d_DZ; S.DY = c->d_DY; S.part = c->d_part;
+    S.tiles = c->tiles;
+    S.loss = c->d_loss + slot;
+    S.steps = grad ? nullptr : c->d_steps;
+    VT_HIP(vt::launch_mlp_train_fb(c->d_pack, c->d_bias, c->d_tpack, b, S, c->ctx->cus, s));
+    VT_HIP(vt::launch_mlp_train_grad(b, S, s));
+    VT_HIP(vt::launch_mlp_train_update(b, S, grad, c->d_params, c->d_m, c->d_v, c->d_pack, c->d_bias, c->d_tpack, lr,
+                                       0.9f, 0.99f, 1e-8f, s));
+    return VT_OK;
+}
+
+/* gradient or Adam step over n host samples in one batch; mean loss out */
+int cache_host_step(vt_cache* c, int64_t n, const float* raw, const float* target, float* grad_host, float lr,
+                    float* loss)
+{
+    if (n > (int64_t)0x7fffffff) return fail(VT_ERR_LIMIT, "too many samples in one batch");
+    hipStream_t s = c->ctx->stream;
+    int r = cache_upload_samples(c, n, raw, target);
+    if (r != VT_OK) return r;
+    VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_loss, 1, 0u, s));
+    const vt::TrainBatch b = { c->d_raw, c->d_tgt, c->d_nsamp, (uint32_t)n, 0u, (uint32_t)n };
+    r = cache_train_batch(c, b, grad_host ? c->d_grad : nullptr, lr, 0);
+    if (r != VT_OK) return r;
+    if (grad_host)
+        VT_HIP(hipMemcpyAsync(grad_host, c->d_grad, (size_t)vt::MLP_N_PARAMS * 4, hipMemcpyDeviceToHost, s));
+    float l = 0.0f;
+    VT_HIP(hipMemcpyAsync(&l, c->d_loss, 4, hipMemcpyDeviceToHost, s));
+    VT_HIP(hipStreamSynchronize(s));
+    if (loss) *loss = l / (float)(vt::MLP_OUT * n);
+    return VT_OK;
 }
 
 }  // namespace
@@ -951,10 +992,37 @@ int vt_cache_create(vt_context* ctx, uint64_t seed, vt_cache** out)
     vt_cache* c = new vt_cache();
     c->ctx = ctx;
     const size_t pb = (size_t)vt::MLP_N_PARAMS * 4;
+    /* the pack layouts must be exact inverses (checked once, on the host) */
+    for (int e = 0; e < vt::MLP_PACK; ++e) {
+        const int p = vt::pack_source(e);
+        if (p < 0) continue;
+        int layer = 5;
+        while (layer > 0 && p < vt::mlp_w_off(layer)) --layer;
+        const int in = layer == 0 ? vt::MLP_IN : vt::MLP_W, local = p - vt::mlp_w_off(layer);
+        if (vt::pack_dest(layer, local / in, local % in) != e) {
+            delete c;
+            return fail(VT_ERR_STATE, "internal error: FP16 weight pack mapping is not invertible");
+        }
+    }
+    for (int e = 0; e < vt::MLP_TPACK; ++e) {
+        const int p = vt::tpack_source(e);
+        if (p < 0) continue;
+        int layer = 5;
+        while (layer > 0 && p < vt::mlp_w_off(layer)) --layer;
+        const int local = p - vt::mlp_w_off(layer);
+        if (layer == 0 || vt::tpack_dest(layer, local / vt::MLP_W, local % vt::MLP_W) != e) {
+            delete c;
+            return fail(VT_ERR_STATE, "internal error: FP32 weight pack mapping is not invertible");
+        }
+    }
     if (hipMalloc(&c->d_params, pb) != hipSuccess || hipMalloc(&c->d_grad, pb) != hipSuccess ||
         hipMalloc(&c->d_m, pb) != hipSuccess || hipMalloc(&c->d_v, pb) != hipSuccess ||
         hipMalloc(&c->d_pack, (size_t)vt::MLP_PACK * 2) != hipSuccess ||
-        hipMalloc(&c->d_bias, (size_t)vt::MLP_BIAS * 4) != hipSuccess || hipMalloc(&c->d_loss, 4) != hipSuccess) {
+        hipMalloc(&c->d_bias, (size_t)vt::MLP_BIAS * 4) != hipSuccess ||
+        hipMalloc(&c->d_tpack, (size_t)vt::MLP_TPACK * 4) != hipSuccess ||
+        hipMalloc(&c->d_part, (size_t)vt::MLP_GRAD_SPLITS * vt::MLP_N_PARAMS * 4) != hipSuccess ||
+        hipMalloc(&c->d_loss, VT_CACHE_LOSS_SLOTS * 4) != hipSuccess || hipMalloc(&c->d_steps, 4) != hipSuccess ||
+        hipMalloc(&c->d_nsamp, 4) != hipSuccess || hipMalloc(&c->d_tally, 16) != hipSuccess) {
         vt_cache_destroy(c);
         return fail(VT_ERR_HIP, "cache allocation failed");
     }
@@ -993,7 +1061,8 @@ void vt_cache_destroy(vt_cache* c)
     (void)hipSetDevice(c->ctx->device);
     (void)hipStreamSynchronize(c->ctx->stream);
     dfree(c->d_params); dfree(c->d_grad); dfree(c->d_m); dfree(c->d_v); dfree(c->d_pack); dfree(c->d_bias);
-    dfree(c->d_loss); dfree(c->d_X); dfree(c->d_H); dfree(c->d_DZ); dfree(c->d_DY); dfree(c->d_raw);
+    dfree(c->d_tpack); dfree(c->d_part); dfree(c->d_loss); dfree(c->d_steps); dfree(c->d_nsamp); dfree(c->d_tally);
+    dfree(c->d_X); dfree(c->d_H); dfree(c->d_DZ); dfree(c->d_DY); dfree(c->d_raw);
     dfree(c->d_tgt); dfree(c->d_iraw); dfree(c->d_iout);
     dfree(c->d_qraw); dfree(c->d_qout); dfree(c->d_qid); dfree(c->d_counts);
     dfree(c->d_vrec); dfree(c->d_vrec_n); dfree(c->d_sraw); dfree(c->d_stgt);
@@ -1018,7 +1087,7 @@ int vt_cache_set_params(vt_cache* c, const float* p)
     VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_grad, vt::MLP_N_PARAMS, 0u, s));
     VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_m, vt::MLP_N_PARAMS, 0u, s));
     VT_HIP(vt::launch_fill_u32((uint32_t*)c->d_v, vt::MLP_N_PARAMS, 0u, s));
-    c->step = 0;
+    VT_HIP(vt::launch_fill_u32(c->d_steps, 1, 0u, s));
     int r = cache_repack(c);
     if (r != VT_OK) return r;
     VT_HIP(hipStreamSynchronize(s));
@@ -1045,26 +1114,11 @@ int vt_cache_infer(vt_cache* c, int64_t n, const float* raw, float* out)
     return VT_OK;
 }
 
-static int cache_upload_samples(vt_cache* c, int64_t n, const float* raw, const float* target)
-{
-    int r = cache_scratch(c, n);
-    if (r != VT_OK) return r;
-    hipStream_t s = c->ctx->stream;
-    VT_HIP(hipMemcpyAsync(c->d_raw, raw, (size_t)n * vt::MLP_RAW * 4, hipMemcpyHostToDevice, s));
-    VT_HIP(hipMemcpyAsync(c->d_tgt, target, (size_t)n * vt::MLP_OUT * 4, hipMemcpyHostToDevice, s));
-    return VT_OK;
-}
-
 int vt_cache_gradient(vt_cache* c, int64_t n, const float* raw, const float* target, float* grad, float* loss)
 {
     if (!c || n <= 0 || !raw || !target || !grad) return fail(VT_ERR_ARG, "bad argument");
     VT_HIP(hipSetDevice(c->ctx->device));
-    int r = cache_upload_samples(c, n, raw, target);
-    if (r == VT_OK) r = cache_grad_device(c, n, c->d_raw, c->d_tgt, loss);
-    if (r != VT_OK) return r;
-    VT_HIP(hipMemcpyAsync(grad, c->d_grad, (size_t)vt::MLP_N_PARAMS * 4, hipMemcpyDeviceToHost, c->ctx->stream));
-    VT_HIP(hipStreamSynchronize(c->ctx->stream));
-    return VT_OK;
+    return cache_host_step(c, n, raw, target, grad, 0.0f, loss);
 }
 
 int vt_cache_train(vt_cache* c, int64_t n, const float* raw, const float* target, float lr, float* loss)
@@ -1072,12 +1126,7 @@ int vt_cache_train(vt_cache* c, int64_t n, const float* raw, const float* target
     if (!c || n <= 0 || !raw || !target) return fail(VT_ERR_ARG, "bad argument");
     if (!(lr > 0.0f)) return fail(VT_ERR_ARG, "learning rate must be positive");
     VT_HIP(hipSetDevice(c->ctx->device));
-    int r = cache_upload_samples(c, n, raw, target);
-    if (r == VT_OK) r = cache_grad_device(c, n, c->d_raw, c->d_tgt, loss);
-    if (r == VT_OK) r = cache_adam(c, lr);
-    if (r != VT_OK) return r;
-    VT_HIP(hipStreamSynchronize(c->ctx->stream));
-    return VT_OK;
+    return cache_host_step(c, n, raw, target, nullptr, lr, loss);
 }
 
 int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uint32_t count, int32_t cache_bounce,
@@ -1088,6 +1137,7 @@ int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uin
     if (cache_bounce < -1 || cache_bounce > 64) return fail(VT_ERR_ARG, "cache_bounce must be in [-1, 64]");
     if (train_paths > 0 && !(lr > 0.0f)) return fail(VT_ERR_ARG, "learning rate must be positive");
     if (batch == 0) batch = 16384;
+    if (batch > (1u << 20)) return fail(VT_ERR_LIMIT, "training batch above 2^20 samples");
     VT_HIP(hipSetDevice(c->device));
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = c->stream;
@@ -1124,8 +1174,15 @@ int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uin
         VT_HIP(hipMalloc(&k->d_stgt, scap * 3 * 4));
         k->scap = scap;
     }
+    if (scap > 0xffffffffull) return fail(VT_ERR_LIMIT, "too many training samples per pass");
+    const uint32_t nb = train_paths ? (uint32_t)((scap + batch - 1) / batch) : 0u;
+    if (nb > VT_CACHE_LOSS_SLOTS) return fail(VT_ERR_LIMIT, "too many training batches per pass");
     vt_cache_stats st;
     std::memset(&st, 0, sizeof(st));
+    uint32_t steps0 = 0;
+    VT_HIP(hipMemcpyAsync(&steps0, k->d_steps, 4, hipMemcpyDeviceToHost, s));
+    VT_HIP(hipStreamSynchronize(s));
+    VT_HIP(vt::launch_fill_u32((uint32_t*)k->d_tally, 4, 0u, s));
     for (uint32_t pass = 0; pass < count; ++pass) {
         const uint32_t n = n0 + pass;
         vt::TraceArgs a = trace_args(c);
@@ -1154,21 +1211,35 @@ int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uin
         C.s_cap = (uint32_t)k->scap;
         VT_HIP(vt::launch_fill_u32(k->d_counts, 4, 0u, s));
         VT_HIP(vt::launch_render_cached(a, c->wf, C, k->d_pack, k->d_bias, c->cus, s));
-        uint32_t counts[2] = { 0, 0 };
-        VT_HIP(hipMemcpyAsync(counts, k->d_counts, 8, hipMemcpyDeviceToHost, s));
-        VT_HIP(hipStreamSynchronize(s));
-        st.queries += counts[0];
-        const uint64_t ns = std::min<uint64_t>(counts[1], k->scap);
-        for (uint64_t off = 0; train_paths && off < ns; off += batch) {
-            const int64_t nb = (int64_t)std::min<uint64_t>(batch, ns - off);
-            r = cache_grad_device(k, nb, k->d_sraw + off * 12, k->d_stgt + off * 3, &st.last_loss);
-            if (r == VT_OK) r = cache_adam(k, lr);
+        VT_HIP(vt::launch_cache_tally(k->d_counts, (uint32_t)k->scap, k->d_tally, s));
+        /* training batches over the samples the pass stored; the count stays
+         * on the device (empty batches exit at once) */
+        if (nb) VT_HIP(vt::launch_fill_u32((uint32_t*)k->d_loss, nb, 0u, s));
+        for (uint32_t bi = 0; bi < nb; ++bi) {
+            const vt::TrainBatch tb = { k->d_sraw, k->d_stgt, k->d_counts + 1, (uint32_t)k->scap, bi * batch, batch };
+            r = cache_train_batch(k, tb, nullptr, lr, (int)bi);
             if (r != VT_OK) return r;
-            st.steps++;
-            st.samples += (uint64_t)nb;
         }
     }
+    /* statistics: totals, Adam steps taken, loss of the last pass's last batch */
+    unsigned long long tally[2] = { 0, 0 };
+    uint32_t counts[2] = { 0, 0 }, steps1 = 0;
+    std::vector<float> losses(std::max<uint32_t>(nb, 1u), 0.0f);
+    VT_HIP(hipMemcpyAsync(tally, k->d_tally, 16, hipMemcpyDeviceToHost, s));
+    VT_HIP(hipMemcpyAsync(counts, k->d_counts, 8, hipMemcpyDeviceToHost, s));
+    VT_HIP(hipMemcpyAsync(&steps1, k->d_steps, 4, hipMemcpyDeviceToHost, s));
+    if (nb) VT_HIP(hipMemcpyAsync(losses.data(), k->d_loss, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
     VT_HIP(hipStreamSynchronize(s));
+    st.queries = tally[0];
+    st.samples = train_paths ? tally[1] : 0;
+    st.steps = steps1 - steps0;
+    const uint64_t last_n = std::min<uint64_t>(counts[1], k->scap);
+    for (uint32_t bi = 0; bi < nb; ++bi) {
+        const uint64_t lo = (uint64_t)bi * batch;
+        if (lo >= last_n) break;
+        const uint64_t nbat = std::min<uint64_t>(batch, last_n - lo);
+        st.last_loss = losses[bi] / (float)(vt::MLP_OUT * nbat);
+    }
     st.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->timed = false;
     if (out) *out = st;
