@@ -270,7 +270,7 @@ def bench_cache(args, api, workloads, target=1e-2):
     cache after 2 bounces, 16k training paths per frame), one sample per
     pixel per frame, each given the same render-time budget."""
     import numpy as np
-    wl = workloads.full(width=960, height=540, spp=1, bounces=8)
+    wl = workloads.full(width=1920, height=1080, spp=1, bounces=8)
     ref_ctx = api.Context(0)
     workloads.load(ref_ctx, wl, fp64=True)
     done = 0
@@ -346,11 +346,13 @@ def bench_cache(args, api, workloads, target=1e-2):
         "vs_baseline": None,
         "dtype": "f32 tracer, fp16 mlp",
         "data": "synthetic",
-        "config": {"workload": "full-shading 256^3 scene at 960x540, 1 spp per frame; FP64 reference of "
+        "config": {"workload": "full-shading 256^3 scene at 1920x1080, 1 spp per frame; FP64 reference of "
                                f"{args.ref_spp} spp; target RMSE {target} (tonemapped mean, pixels x RGB)",
                    "plain": plain, "cache": cached, "cache_bounce": args.cache_bounce,
                    "train_paths_per_frame": args.train_paths,
                    "cache_last_frame": stats[-1] if stats else None,
+                   "cache_mean_samples_per_frame": round(sum(x["samples"] for x in stats) / max(1, len(stats)), 1),
+                   "cache_mean_queries_per_frame": round(sum(x["queries"] for x in stats) / max(1, len(stats)), 1),
                    "speedup_time_to_target": None if (tp is None or tc is None) else round(tp / tc, 3)},
         "roofline": None,
         "cpu_baseline": None,

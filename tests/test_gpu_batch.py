@@ -132,3 +132,21 @@ def test_autotune_keeps_image_and_accumulation():
     assert np.array_equal(tuned.view(np.uint32), d.read_accum().view(np.uint32))
     c.close()
     d.close()
+
+
+def test_context_destroyed_before_its_batch():
+    from voxeltoy_amd import api, workloads
+    wls = mixed_scenes()[:2]
+    ctxs = []
+    for wl in wls:
+        c = api.Context(0)
+        workloads.load(c, wl)
+        ctxs.append(c)
+    b = api.Batch(ctxs)
+    ctxs[1].close()
+    with pytest.raises(api.VTError):
+        b.render([1, 2], [1, 1])
+    b.close()
+    ctxs[0].render(wls[0].seed, 0, 1)
+    assert ctxs[0].read_accum()[..., 3].min() == 1.0
+    ctxs[0].close()

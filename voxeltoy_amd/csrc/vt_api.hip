@@ -37,11 +37,15 @@ int fail(int code, const std::string& msg)
 #define VT_DEFAULT_REFILL_MIN 8
 #define VT_DEFAULT_REFILL_CHUNK 256
 
+/* On failure the runtime's last-error state is cleared too, so a failed
+ * call is not reported again by the next launch check. */
 #define VT_HIP(expr)                                                                      \
     do {                                                                                  \
         hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
+        if (e_ != hipSuccess) {                                                           \
+            (void)hipGetLastError();                                                      \
             return fail(VT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+        }                                                                                 \
     } while (0)
 
 template <typename T>
@@ -54,6 +58,11 @@ void dfree(T*& p)
 }  // namespace
 
 struct vt_context {
+    /* batches and caches built on this context; destroying the context
+     * releases their device memory and leaves them unusable (their own
+     * destroy call then only frees the handle) */
+    std::vector<vt_batch*> batches;
+    std::vector<vt_cache*> caches;
     int device = 0;
     int cus = 0;
     size_t lds_limit = 65536;       /* dynamic LDS per workgroup */
@@ -380,9 +389,14 @@ int vt_create(int device, vt_context** out)
     return VT_OK;
 }
 
+static void batch_release(vt_batch* b);
+static void cache_release(vt_cache* k);
+
 void vt_destroy(vt_context* c)
 {
     if (!c) return;
+    while (!c->caches.empty()) cache_release(c->caches.back());
+    while (!c->batches.empty()) batch_release(c->batches.back());
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_bricks); dfree(c->d_occ64); dfree(c->d_bits); dfree(c->d_emissive); dfree(c->d_recs);
@@ -729,10 +743,13 @@ int vt_batch_create(vt_context* const* ctxs, int32_t n, vt_batch** out)
     b->lead = ctxs[0];
     b->h_tab.resize((size_t)n);
     b->h_base.resize(2 * ((size_t)n + 1));
+    for (vt_context* c : b->ctxs)
+        if (std::find(c->batches.begin(), c->batches.end(), b) == c->batches.end()) c->batches.push_back(b);
     if (hipMalloc(&b->d_tab, sizeof(vt::TraceArgs) * (size_t)n) != hipSuccess ||
         hipMalloc(&b->d_base, sizeof(uint64_t) * b->h_base.size()) != hipSuccess ||
         hipMalloc(&b->d_stats, 6 * sizeof(unsigned long long)) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        (void)hipGetLastError();
         vt_batch_destroy(b);
         return fail(VT_ERR_HIP, "batch allocation failed");
     }
@@ -740,16 +757,29 @@ int vt_batch_create(vt_context* const* ctxs, int32_t n, vt_batch** out)
     return VT_OK;
 }
 
-void vt_batch_destroy(vt_batch* b)
+/* Free the batch's device resources and detach it from its contexts. */
+static void batch_release(vt_batch* b)
 {
-    if (!b) return;
+    if (!b->lead) return;
     (void)hipSetDevice(b->lead->device);
     if (b->lead->stream) (void)hipStreamSynchronize(b->lead->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
     dfree(b->d_tab); dfree(b->d_base); dfree(b->d_stats);
     free_paths(b->wf, b->wf_alloc);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    b->ev0 = b->ev1 = nullptr;
+    (void)hipGetLastError();
+    for (vt_context* c : b->ctxs) c->batches.erase(std::remove(c->batches.begin(), c->batches.end(), b), c->batches.end());
+    b->ctxs.clear();
+    b->lead = nullptr;
+}
+
+void vt_batch_destroy(vt_batch* b)
+{
+    if (!b) return;
+    batch_release(b);
     delete b;
 }
 
@@ -757,6 +787,7 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
                     uint32_t rng_frame_max, int32_t flags)
 {
     if (!b || !seeds || !counts) return fail(VT_ERR_ARG, "null argument");
+    if (!b->lead) return fail(VT_ERR_STATE, "a context of the batch was destroyed");
     const size_t n = b->ctxs.size();
     vt_context* L = b->lead;
     VT_HIP(hipSetDevice(L->device));
@@ -842,6 +873,7 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
 int vt_batch_get_stats(vt_batch* b, vt_stats* out)
 {
     if (!b || !out) return fail(VT_ERR_ARG, "null argument");
+    if (!b->lead) return fail(VT_ERR_STATE, "a context of the batch was destroyed");
     VT_HIP(hipSetDevice(b->lead->device));
     VT_HIP(hipStreamSynchronize(b->lead->stream));
     std::memset(out, 0, sizeof(*out));
@@ -991,6 +1023,7 @@ int vt_cache_create(vt_context* ctx, uint64_t seed, vt_cache** out)
     VT_HIP(hipSetDevice(ctx->device));
     vt_cache* c = new vt_cache();
     c->ctx = ctx;
+    ctx->caches.push_back(c);
     const size_t pb = (size_t)vt::MLP_N_PARAMS * 4;
     /* the pack layouts must be exact inverses (checked once, on the host) */
     for (int e = 0; e < vt::MLP_PACK; ++e) {
@@ -1000,7 +1033,7 @@ int vt_cache_create(vt_context* ctx, uint64_t seed, vt_cache** out)
         while (layer > 0 && p < vt::mlp_w_off(layer)) --layer;
         const int in = layer == 0 ? vt::MLP_IN : vt::MLP_W, local = p - vt::mlp_w_off(layer);
         if (vt::pack_dest(layer, local / in, local % in) != e) {
-            delete c;
+            vt_cache_destroy(c);
             return fail(VT_ERR_STATE, "internal error: FP16 weight pack mapping is not invertible");
         }
     }
@@ -1011,7 +1044,7 @@ int vt_cache_create(vt_context* ctx, uint64_t seed, vt_cache** out)
         while (layer > 0 && p < vt::mlp_w_off(layer)) --layer;
         const int local = p - vt::mlp_w_off(layer);
         if (layer == 0 || vt::tpack_dest(layer, local / vt::MLP_W, local % vt::MLP_W) != e) {
-            delete c;
+            vt_cache_destroy(c);
             return fail(VT_ERR_STATE, "internal error: FP32 weight pack mapping is not invertible");
         }
     }
@@ -1055,9 +1088,10 @@ int vt_cache_create(vt_context* ctx, uint64_t seed, vt_cache** out)
     return r;
 }
 
-void vt_cache_destroy(vt_cache* c)
+/* Free the cache's device memory and detach it from its context. */
+static void cache_release(vt_cache* c)
 {
-    if (!c) return;
+    if (!c->ctx) return;
     (void)hipSetDevice(c->ctx->device);
     (void)hipStreamSynchronize(c->ctx->stream);
     dfree(c->d_params); dfree(c->d_grad); dfree(c->d_m); dfree(c->d_v); dfree(c->d_pack); dfree(c->d_bias);
@@ -1066,12 +1100,24 @@ void vt_cache_destroy(vt_cache* c)
     dfree(c->d_tgt); dfree(c->d_iraw); dfree(c->d_iout);
     dfree(c->d_qraw); dfree(c->d_qout); dfree(c->d_qid); dfree(c->d_counts);
     dfree(c->d_vrec); dfree(c->d_vrec_n); dfree(c->d_sraw); dfree(c->d_stgt);
+    (void)hipGetLastError();
+    auto& v = c->ctx->caches;
+    v.erase(std::remove(v.begin(), v.end(), c), v.end());
+    c->ctx = nullptr;
+    c->tiles = 0; c->cap = 0; c->icap = 0; c->qcap = 0; c->vcap = 0; c->vslots = 0; c->scap = 0;
+}
+
+void vt_cache_destroy(vt_cache* c)
+{
+    if (!c) return;
+    cache_release(c);
     delete c;
 }
 
 int vt_cache_get_params(vt_cache* c, float* p)
 {
     if (!c || !p) return fail(VT_ERR_ARG, "null argument");
+    if (!c->ctx) return fail(VT_ERR_STATE, "the cache's context was destroyed");
     VT_HIP(hipSetDevice(c->ctx->device));
     VT_HIP(hipMemcpyAsync(p, c->d_params, (size_t)vt::MLP_N_PARAMS * 4, hipMemcpyDeviceToHost, c->ctx->stream));
     VT_HIP(hipStreamSynchronize(c->ctx->stream));
@@ -1081,6 +1127,7 @@ int vt_cache_get_params(vt_cache* c, float* p)
 int vt_cache_set_params(vt_cache* c, const float* p)
 {
     if (!c || !p) return fail(VT_ERR_ARG, "null argument");
+    if (!c->ctx) return fail(VT_ERR_STATE, "the cache's context was destroyed");
     VT_HIP(hipSetDevice(c->ctx->device));
     hipStream_t s = c->ctx->stream;
     VT_HIP(hipMemcpyAsync(c->d_params, p, (size_t)vt::MLP_N_PARAMS * 4, hipMemcpyHostToDevice, s));
@@ -1097,6 +1144,7 @@ int vt_cache_set_params(vt_cache* c, const float* p)
 int vt_cache_infer(vt_cache* c, int64_t n, const float* raw, float* out)
 {
     if (!c || n < 0 || (n > 0 && (!raw || !out))) return fail(VT_ERR_ARG, "bad argument");
+    if (!c->ctx) return fail(VT_ERR_STATE, "the cache's context was destroyed");
     if (n == 0) return VT_OK;
     VT_HIP(hipSetDevice(c->ctx->device));
     hipStream_t s = c->ctx->stream;
@@ -1117,6 +1165,7 @@ int vt_cache_infer(vt_cache* c, int64_t n, const float* raw, float* out)
 int vt_cache_gradient(vt_cache* c, int64_t n, const float* raw, const float* target, float* grad, float* loss)
 {
     if (!c || n <= 0 || !raw || !target || !grad) return fail(VT_ERR_ARG, "bad argument");
+    if (!c->ctx) return fail(VT_ERR_STATE, "the cache's context was destroyed");
     VT_HIP(hipSetDevice(c->ctx->device));
     return cache_host_step(c, n, raw, target, grad, 0.0f, loss);
 }
@@ -1124,6 +1173,7 @@ int vt_cache_gradient(vt_cache* c, int64_t n, const float* raw, const float* tar
 int vt_cache_train(vt_cache* c, int64_t n, const float* raw, const float* target, float lr, float* loss)
 {
     if (!c || n <= 0 || !raw || !target) return fail(VT_ERR_ARG, "bad argument");
+    if (!c->ctx) return fail(VT_ERR_STATE, "the cache's context was destroyed");
     if (!(lr > 0.0f)) return fail(VT_ERR_ARG, "learning rate must be positive");
     VT_HIP(hipSetDevice(c->ctx->device));
     return cache_host_step(c, n, raw, target, nullptr, lr, loss);
@@ -1132,7 +1182,7 @@ int vt_cache_train(vt_cache* c, int64_t n, const float* raw, const float* target
 int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uint32_t count, int32_t cache_bounce,
                      uint32_t train_paths, float lr, uint32_t batch, vt_cache_stats* out)
 {
-    if (!c || !k || k->ctx != c) return fail(VT_ERR_ARG, "cache must belong to the context");
+    if (!c || !k || k->ctx != c) return fail(VT_ERR_ARG, "cache must belong to the context (and be alive)");
     if (!c->has_volume || !c->d_accum) return fail(VT_ERR_STATE, "no volume or no frame");
     if (cache_bounce < -1 || cache_bounce > 64) return fail(VT_ERR_ARG, "cache_bounce must be in [-1, 64]");
     if (train_paths > 0 && !(lr > 0.0f)) return fail(VT_ERR_ARG, "learning rate must be positive");

@@ -64,7 +64,7 @@ struct WfPaths {
     int4* tgt;        /* emissive target voxel, light_is_voxel */
     uint4* meta;      /* pixel, pass, phase | bounces << 8 */
     uint64_t* rng;    /* PCG32 state (increment = 2*pixel+1) */
-    int4* hit;        /* traversal result: x, y, z, kind << 8 | slot */
+    int4* hit;        /* traversal result per queue position: x, y, z, kind << 8 | slot */
     float4* smp;      /* finished tonemapped sample of each path */
     float4* qa0;      /* ray queue 0: origin.xyz, path slot id (as bits) */
     float4* qb0;      /*              direction.xyz */

@@ -561,12 +561,22 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t refill_min = (uint32_t)a.refill_min;
-    const uint32_t chunk = (uint32_t)a.refill_chunk;
+    /* A short queue is spread over the waves: no wave takes more than its
+     * fair share (at least one ray per lane), so the launch is not held by
+     * a few waves tracing several rays per lane one after another. */
+    uint32_t chunk = (uint32_t)a.refill_chunk;
+    if (chunk) {
+        const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+        const uint32_t per = (n + waves - 1u) / waves;
+        uint32_t fair = ((per + 63u) >> 6) << 6;      /* >= 64: a wave never takes fewer than its idle lanes */
+        fair = fair < 64u ? 64u : fair;
+        if (fair < chunk) chunk = fair;
+    }
     const int budget = a.dda_budget;
     DdaState D;
     GridView gl;                   /* the lane's scene grid (batch only) */
     const GridView& G = BATCH ? gl : a.g;
-    uint32_t id = 0;
+    uint32_t jq = 0;               /* queue position of the lane's ray (hits are stored there) */
     bool has = false;
     /* wave-uniform: the wave's reserved queue range [c_pos, c_end) */
     uint32_t c_pos = 0, c_end = 0;
@@ -592,12 +602,12 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
                 else if (grab && nb + (p - avail) < n) j = nb + (p - avail);
                 if (j != 0xffffffffu) {
                     const float4 o = qa[j], d = qb[j];
-                    id = __float_as_uint(o.w);
+                    jq = j;
                     if (BATCH) gl = B.tab[__float_as_uint(d.w)].g;
                     if (D.init(G, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z))) {
                         has = true;
                     } else {
-                        P.hit[id] = make_int4(0, 0, 0, KIND_MISS << 8);
+                        P.hit[j] = make_int4(0, 0, 0, KIND_MISS << 8);
                     }
                 }
             }
@@ -618,7 +628,7 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
             if (status) {
                 Hit h;
                 const int kind = D.finish(G, status, h);
-                P.hit[id] = make_int4(h.x, h.y, h.z, (kind << 8) | (int32_t)h.slot);
+                P.hit[jq] = make_int4(h.x, h.y, h.z, (kind << 8) | (int32_t)h.slot);
                 has = false;
             }
         }
@@ -626,29 +636,44 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
     flush_counters<STATS>(BATCH ? B.stats : a.stats, cnt);
 }
 
-/* Training samples of a finished training path: one per recorded vertex,
- * target = (radiance at the end - radiance at the vertex) / throughput at
- * the vertex, per channel; vertices with a zero throughput channel give no
- * sample. */
+/* Largest training target kept: above it a sample is an outlier of the
+ * path estimate (throughput ratios of near-zero-pdf bounces), whose
+ * relative-loss gradient would dominate whole batches. */
+#define CACHE_TARGET_MAX 1.0e4f
+
+/* Target of a recorded vertex e (18 floats: query, throughput, radiance):
+ * (radiance at the end - radiance at the vertex) / throughput, per channel;
+ * false for a zero throughput channel or a non-finite or too large target. */
+__device__ __forceinline__ bool vertex_target(const float* e, f3 r_end, float* t)
+{
+    if (!(e[12] > 0.0f && e[13] > 0.0f && e[14] > 0.0f)) return false;
+    t[0] = (r_end.x - e[15]) / e[12];
+    t[1] = (r_end.y - e[16]) / e[13];
+    t[2] = (r_end.z - e[17]) / e[14];
+    for (int c = 0; c < 3; ++c) {
+        if (!(t[c] <= CACHE_TARGET_MAX)) return false;   /* also NaN */
+        t[c] = t[c] > 0.0f ? t[c] : 0.0f;                /* rounding of R_end - R_v */
+    }
+    return true;
+}
+
+/* Training samples of a finished training path: one per recorded vertex
+ * with a valid target. */
 __device__ __forceinline__ void emit_training(const CacheRender& C, uint32_t slot, f3 r_end)
 {
     const uint32_t nv = C.vrec_n[slot];
     const float* rec = C.vrec + (size_t)slot * C.max_vertices * 18u;
     uint32_t ok = 0;
-    for (uint32_t v = 0; v < nv; ++v) {
-        const float* e = rec + v * 18u;
-        ok += (e[12] > 0.0f && e[13] > 0.0f && e[14] > 0.0f) ? 1u : 0u;
-    }
+    float t[3];
+    for (uint32_t v = 0; v < nv; ++v) ok += vertex_target(rec + v * 18u, r_end, t) ? 1u : 0u;
     if (!ok) return;
     uint32_t base = atomicAdd(C.s_count, ok);
     for (uint32_t v = 0; v < nv; ++v) {
         const float* e = rec + v * 18u;
-        if (!(e[12] > 0.0f && e[13] > 0.0f && e[14] > 0.0f)) continue;
+        if (!vertex_target(e, r_end, t)) continue;
         if (base < C.s_cap) {
             for (int k = 0; k < 12; ++k) C.s_raw[(size_t)base * 12u + k] = e[k];
-            C.s_tgt[(size_t)base * 3u + 0] = (r_end.x - e[15]) / e[12];
-            C.s_tgt[(size_t)base * 3u + 1] = (r_end.y - e[16]) / e[13];
-            C.s_tgt[(size_t)base * 3u + 2] = (r_end.z - e[17]) / e[14];
+            for (int c = 0; c < 3; ++c) C.s_tgt[(size_t)base * 3u + c] = t[c];
         }
         base++;
     }
@@ -684,7 +709,7 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0
             const uint32_t* bits = BATCH ? a.g.bits : bits0;
             PathState S;
             load_phase<PHASE>(P, id, S);
-            const int4 hv = P.hit[id];
+            const int4 hv = P.hit[j];
             Hit h;
             h.x = hv.x; h.y = hv.y; h.z = hv.z; h.slot = (uint32_t)hv.w & 0xffu;
             h.steps = 0;
