@@ -31,10 +31,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s; time-to-RMSE 1e-2 vs FP64 ref, plain vs batched+radiance-cache"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the dominant kernel on this workload, from
-# rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a copy kernel of known size);
-# None until profiled.
-TRAFFIC: dict = {}
+# HBM bytes per launch of the dominant kernel on the full workload, from
+# rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), each factor being what
+# the counter reads for a copy kernel of known size
+# (profiles/wf_traverse_kernel_full_1024x1024_16spp.md).
+TRAFFIC: dict = {"wf_traverse_kernel": 340897464}
 
 
 def env_lookup_bytes(cu_w: int, cv_n: int) -> tuple[int, int]:
@@ -182,7 +183,8 @@ def main():
                                                              ("visits", "interactions", "samples", "env_samples",
                                                               "env_evals")}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(kernel_name),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC.get(kernel_name) if args.mode == 1 else None,
                      "kernel": kernel_name, "launches_per_step": round(n_launch, 2),
                      "avg_launch_ms": round(launch_ms, 4), "algorithmic_bytes_per_launch": int(launch_bytes),
                      "render_algorithmic_bytes_per_step": int(bytes_step),
