@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), each factor being what
 # the counter reads for a copy kernel of known size
 # (profiles/wf_traverse_kernel_full_1024x1024_16spp.md).
-TRAFFIC: dict = {"wf_traverse_kernel": 340897464}
+TRAFFIC: dict = {("wf_traverse_kernel", 1): 340897464}
 
 
 def env_lookup_bytes(cu_w: int, cv_n: int) -> tuple[int, int]:
@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--mode", type=int, default=1, help="0 megakernel, 1 wavefront (default)")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="wavefront: 2 (default) runs the step's pass halves on two streams, 1 on one")
     ap.add_argument("--workload", default="full", choices=["full", "batch", "cache"],
                     help="full: the 256^3 full-shading scene (default); batch: 64 mixed scenes, one batched "
                          "graph launch against 64 sequential renders; cache: time to RMSE 1e-2 against an FP64 "
@@ -114,6 +116,7 @@ def main():
     ctx = api.Context(0)
     workloads.load(ctx, wl)
     ctx.set_render_mode(args.mode)
+    ctx.set_overlap(args.streams)
     cu_w, cv_n = wl.cdf[0].shape[1], len(wl.cdf[1])
 
     # statistics launch (untimed): exact work of one step
@@ -130,7 +133,7 @@ def main():
 
     # traversal launches are bracketed by HIP events on the render stream
     ctx.set_launch_timing(args.mode == 1)
-    kernel_ms, trav_ms, trav_launches = [], [], []
+    kernel_ms, trav_ms, trav_launches, trav_union = [], [], [], []
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -140,6 +143,7 @@ def main():
         lt = ctx.launch_timing()
         trav_ms.append(lt["traverse_ms"])
         trav_launches.append(lt["launches"])
+        trav_union.append(lt["union_ms"])
     ctx.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1000.0 / args.steps
@@ -155,9 +159,12 @@ def main():
         launch_ms = sum(trav_ms) / sum(trav_launches)
         launch_bytes = st["visits"] / n_launch
         kernel_name, achieved = "wf_traverse_kernel", launch_bytes / (launch_ms / 1000.0) / 1e9
+        busy_ms = sum(trav_union) / len(trav_union)
+        busy_gbs = st["visits"] / (busy_ms / 1000.0) / 1e9
     else:
         n_launch, launch_ms, launch_bytes = 1, k_ms, bytes_step
         kernel_name, achieved = "render_kernel", render_gbs
+        busy_ms, busy_gbs = k_ms, render_gbs
     cpu = None if args.no_cpu_baseline else cpu_baseline(wl, args.cpu_budget)
 
     out = {
@@ -178,15 +185,16 @@ def main():
                                "16 spp, 8 bounces (plain tracer)",
                    "grid": 256, "width": wl.width, "height": wl.height, "spp": wl.spp, "bounces": wl.bounces,
                    "emissive_voxels": int(len(wl.scene.emissive)), "rays_per_step": int(rays),
-                   "render_mode": ["megakernel", "wavefront"][args.mode],
+                   "render_mode": ["megakernel", "wavefront"][args.mode], "streams": args.streams,
                    "kernel_ms": round(k_ms, 3), "counters": {k: int(st[k]) for k in
                                                              ("visits", "interactions", "samples", "env_samples",
                                                               "env_evals")}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": TRAFFIC.get(kernel_name) if args.mode == 1 else None,
+                     "traffic": TRAFFIC.get((kernel_name, args.streams)) if args.mode == 1 else None,
                      "kernel": kernel_name, "launches_per_step": round(n_launch, 2),
                      "avg_launch_ms": round(launch_ms, 4), "algorithmic_bytes_per_launch": int(launch_bytes),
+                     "busy_ms_per_step": round(busy_ms, 3), "busy_achieved_gbs": round(busy_gbs, 1),
                      "render_algorithmic_bytes_per_step": int(bytes_step),
                      "render_achieved_gbs": round(render_gbs, 1)},
         "cpu_baseline": cpu,

@@ -160,13 +160,22 @@ int vt_read_accum(vt_context* ctx, float* rgba);
 /* FP64 running sums (W*H*3 doubles) when vt_set_frame(fp64=1). */
 int vt_read_sum64(vt_context* ctx, double* rgb);
 int vt_get_stats(vt_context* ctx, vt_stats* out);
+/* Wavefront renders of at least 2 passes and 4M paths run their first and
+ * second half of the passes on two streams of the device (1 = one stream).
+ * Images are identical (the halves accumulate in pass order); the second
+ * half's paths take their own path buffers, so the in-flight capacity
+ * (vt_set_render_mode max_paths) is split between the halves.  Default 2;
+ * no counterpart in the reference. */
+int vt_set_overlap(vt_context* ctx, int32_t streams);
 /* Per-launch timing of the traversal kernel: when enabled, each wavefront
  * vt_render brackets every traversal launch (up to 256 per call) with HIP
- * events on the context's stream; vt_get_launch_timing returns the summed
- * durations (ms) and the number of timed launches of the last vt_render.
- * Measurement only (no counterpart in the reference). */
+ * events on the stream it runs on; vt_get_launch_timing returns the summed
+ * durations (ms), the number of timed launches of the last vt_render and,
+ * when union_ms is non-null, the time during which at least one of them
+ * ran (less than the sum when two streams overlap).  Measurement only (no
+ * counterpart in the reference). */
 int vt_set_launch_timing(vt_context* ctx, int32_t enabled);
-int vt_get_launch_timing(vt_context* ctx, double* traverse_ms, uint32_t* launches);
+int vt_get_launch_timing(vt_context* ctx, double* traverse_ms, uint32_t* launches, double* union_ms);
 
 /* ---- batches -------------------------------------------------------------
  * Many scenes (contexts on one device, each with its own volume, camera,

@@ -111,7 +111,9 @@ SIGNATURES = {
     "vt_read_sum64": (C.c_int, [C.c_void_p, f64p]),
     "vt_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "vt_set_launch_timing": (C.c_int, [C.c_void_p, C.c_int32]),
-    "vt_get_launch_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+    "vt_set_overlap": (C.c_int, [C.c_void_p, C.c_int32]),
+    "vt_get_launch_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_double)]),
     "vt_trace_first_hit": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, i32p]),
     "vt_trace_rays": (C.c_int, [C.c_void_p, C.c_int32, f32p, f32p, i32p]),
     "vt_voxelize": (C.c_int, [C.c_void_p, f32p, C.c_int64, u32p, C.c_int64, f32p, C.c_int32, C.c_int32,
@@ -382,15 +384,20 @@ class Context:
         _check(lib().vt_get_stats(self._c, C.byref(s)))
         return s.as_dict()
 
+    def set_overlap(self, streams: int):
+        """1 or 2 streams for wavefront renders of many passes (identical images)."""
+        _check(lib().vt_set_overlap(self._c, streams))
+
     def set_launch_timing(self, enabled: bool):
         """Bracket every wavefront traversal launch with HIP events."""
         _check(lib().vt_set_launch_timing(self._c, 1 if enabled else 0))
 
     def launch_timing(self) -> dict:
-        """Summed traversal-kernel time (ms) and launch count of the last render."""
-        ms, n = C.c_double(), C.c_uint32()
-        _check(lib().vt_get_launch_timing(self._c, C.byref(ms), C.byref(n)))
-        return {"traverse_ms": ms.value, "launches": n.value}
+        """Summed traversal-kernel time (ms), launch count and busy time
+        (union of the launches' intervals) of the last render."""
+        ms, n, u = C.c_double(), C.c_uint32(), C.c_double()
+        _check(lib().vt_get_launch_timing(self._c, C.byref(ms), C.byref(n), C.byref(u)))
+        return {"traverse_ms": ms.value, "launches": n.value, "union_ms": u.value}
 
     def first_hit(self, seed: int, frame: int) -> np.ndarray:
         out = np.zeros((self.height, self.width, 8), np.int32)

@@ -33,6 +33,7 @@ int fail(int code, const std::string& msg)
 /* Wavefront defaults (vt_set_render_mode, vt_set_traversal_tuning). */
 #define VT_DEFAULT_MAX_PATHS (16ull << 20)
 #define VT_TIMED_LAUNCHES 256     /* traversal launches timed per render call */
+#define VT_OVERLAP_MIN_PATHS (1ull << 22)   /* paths of a render worth two streams */
 #define VT_DEFAULT_DDA_BUDGET 32
 #define VT_DEFAULT_REFILL_MIN 8
 #define VT_DEFAULT_REFILL_CHUNK 256
@@ -69,6 +70,13 @@ struct vt_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
+    /* second stream: a wavefront render of many passes runs its two halves
+     * on two streams, so each half's launch tails overlap the other's bulk */
+    int overlap = 2;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_half = nullptr, ev_join = nullptr;
+    vt::WfPaths wf2{};
+    uint64_t wf2_alloc = 0;
     /* per-launch traversal timing (wavefront mode) */
     std::vector<hipEvent_t> lt_ev;
     int lt_used = 0;
@@ -403,6 +411,11 @@ void vt_destroy(vt_context* c)
     dfree(c->d_img); dfree(c->d_cdfu); dfree(c->d_cdfv);
     dfree(c->d_accum); dfree(c->d_sum64); dfree(c->d_tile); dfree(c->d_stats);
     free_wavefront(c);
+    free_paths(c->wf2, c->wf2_alloc);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_half) (void)hipEventDestroy(c->ev_half);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->lt_ev) (void)hipEventDestroy(e);
@@ -629,6 +642,19 @@ int vt_set_stats(vt_context* c, int32_t enabled)
     return VT_OK;
 }
 
+int vt_set_overlap(vt_context* c, int32_t streams)
+{
+    if (!c) return fail(VT_ERR_ARG, "null context");
+    if (streams != 1 && streams != 2) return fail(VT_ERR_ARG, "streams must be 1 or 2");
+    c->overlap = streams;
+    if (streams == 1) {
+        VT_HIP(hipSetDevice(c->device));
+        VT_HIP(hipStreamSynchronize(c->stream));
+        free_paths(c->wf2, c->wf2_alloc);
+    }
+    return VT_OK;
+}
+
 int vt_set_launch_timing(vt_context* c, int32_t enabled)
 {
     if (!c) return fail(VT_ERR_ARG, "null context");
@@ -642,19 +668,42 @@ int vt_set_launch_timing(vt_context* c, int32_t enabled)
     return VT_OK;
 }
 
-int vt_get_launch_timing(vt_context* c, double* traverse_ms, uint32_t* launches)
+int vt_get_launch_timing(vt_context* c, double* traverse_ms, uint32_t* launches, double* union_ms)
 {
     if (!c || !traverse_ms || !launches) return fail(VT_ERR_ARG, "null argument");
     VT_HIP(hipSetDevice(c->device));
     VT_HIP(hipStreamSynchronize(c->stream));
     double sum = 0.0;
+    std::vector<std::pair<float, float>> iv;
     for (int k = 0; k < c->lt_used; ++k) {
-        float ms = 0.0f;
+        float ms = 0.0f, t0 = 0.0f, t1 = 0.0f;
         VT_HIP(hipEventElapsedTime(&ms, c->lt_ev[2 * k], c->lt_ev[2 * k + 1]));
         sum += ms;
+        if (union_ms && c->timed) {
+            VT_HIP(hipEventElapsedTime(&t0, c->ev0, c->lt_ev[2 * k]));
+            VT_HIP(hipEventElapsedTime(&t1, c->ev0, c->lt_ev[2 * k + 1]));
+            iv.emplace_back(t0, t1);
+        }
     }
     *traverse_ms = sum;
     *launches = (uint32_t)c->lt_used;
+    if (union_ms) {
+        /* time during which at least one traversal launch ran (launches of
+         * the two streams overlap) */
+        std::sort(iv.begin(), iv.end());
+        double u = 0.0, lo = 0.0, hi = -1.0;
+        for (const auto& p : iv) {
+            if (p.first > hi) {
+                if (hi > lo) u += hi - lo;
+                lo = p.first;
+                hi = p.second;
+            } else if (p.second > hi) {
+                hi = p.second;
+            }
+        }
+        if (hi > lo) u += hi - lo;
+        *union_ms = iv.empty() ? sum : u;
+    }
     return VT_OK;
 }
 
@@ -694,13 +743,43 @@ int vt_render(vt_context* c, uint64_t seed, uint32_t n0, uint32_t count, uint32_
     VT_HIP(vt::launch_reset_counter(c->d_tile, a.stats, c->stream));
     if (c->render_mode == 1) {
         const uint64_t total = (uint64_t)c->W * c->H * count;
-        int r = ensure_wavefront(c, std::min<uint64_t>(total, c->max_paths));
-        if (r != VT_OK) return r;
         c->lt_used = 0;
         const vt::LaunchTimer lt = { c->lt_ev.data(), (int)c->lt_ev.size() / 2, &c->lt_used };
-        VT_HIP(hipEventRecord(c->ev0, c->stream));
-        VT_HIP(vt::launch_render_wavefront(a, c->wf, c->cus, c->stream, c->stats_enabled, c->lt_on ? &lt : nullptr));
-        VT_HIP(hipEventRecord(c->ev1, c->stream));
+        const vt::LaunchTimer* ltp = c->lt_on ? &lt : nullptr;
+        if (c->overlap > 1 && count >= 2 && total >= VT_OVERLAP_MIN_PATHS) {
+            /* passes [n0, n0 + c0) on the context stream, the rest on the
+             * second stream; the second half accumulates after the first */
+            const uint32_t c0 = count / 2, c1 = count - c0;
+            const uint64_t WH = (uint64_t)c->W * c->H, half_cap = std::max<uint64_t>(c->max_paths / 2, WH);
+            if (!c->stream2) {
+                VT_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+                VT_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+                VT_HIP(hipEventCreateWithFlags(&c->ev_half, hipEventDisableTiming));
+                VT_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+            }
+            int r = ensure_wavefront(c, std::min<uint64_t>(WH * c0, half_cap));
+            if (r == VT_OK) r = alloc_paths(c->wf2, c->wf2_alloc, std::min<uint64_t>(WH * c1, half_cap));
+            if (r != VT_OK) return r;
+            vt::TraceArgs a1 = a;
+            a.count = c0;
+            a1.n0 = n0 + c0;
+            a1.count = c1;
+            VT_HIP(hipEventRecord(c->ev0, c->stream));
+            VT_HIP(hipEventRecord(c->ev_fork, c->stream));
+            VT_HIP(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+            VT_HIP(vt::launch_render_wavefront(a, c->wf, c->cus, c->stream, c->stats_enabled, ltp));
+            VT_HIP(hipEventRecord(c->ev_half, c->stream));
+            VT_HIP(vt::launch_render_wavefront(a1, c->wf2, c->cus, c->stream2, c->stats_enabled, ltp, c->ev_half));
+            VT_HIP(hipEventRecord(c->ev_join, c->stream2));
+            VT_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+            VT_HIP(hipEventRecord(c->ev1, c->stream));
+        } else {
+            int r = ensure_wavefront(c, std::min<uint64_t>(total, c->max_paths));
+            if (r != VT_OK) return r;
+            VT_HIP(hipEventRecord(c->ev0, c->stream));
+            VT_HIP(vt::launch_render_wavefront(a, c->wf, c->cus, c->stream, c->stats_enabled, ltp));
+            VT_HIP(hipEventRecord(c->ev1, c->stream));
+        }
     } else {
         const int per_cu = vt::render_blocks_per_cu(a, c->stats_enabled);
         const int grid = std::max(1, c->cus * per_cu);

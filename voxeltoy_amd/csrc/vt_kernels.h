@@ -124,9 +124,11 @@ struct LaunchTimer {
 };
 
 /* Wavefront render of the launch in a: path chunks of at most P.cap slots,
- * 1 + 2*max_bounces traverse/shade rounds per chunk. */
+ * 1 + 2*max_bounces traverse/shade rounds per chunk.  With acc_wait, every
+ * accumulation launch first waits for that event (passes accumulate in
+ * order when two launches render consecutive passes on two streams). */
 hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats,
-                                   const LaunchTimer* timer = nullptr);
+                                   const LaunchTimer* timer = nullptr, hipEvent_t acc_wait = nullptr);
 hipError_t launch_render(const TraceArgs& a, int grid, hipStream_t s, bool stats);
 int render_blocks_per_cu(const TraceArgs& a, bool stats);
 hipError_t launch_first_hit(const TraceArgs& a, int32_t* out, hipStream_t s);

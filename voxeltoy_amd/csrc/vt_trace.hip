@@ -820,7 +820,8 @@ static int blocks_per_cu(K kernel, int threads, size_t lds)
  * the traversal tuning and the LDS sizes of scene 0). */
 template <bool STATS, bool LDSB, bool BATCH>
 static hipError_t wf_run(const TraceArgs& a, const BatchTable& B, const WfPaths& P, uint64_t total,
-                         uint64_t n_pixels, int rounds, int cus, hipStream_t s, const LaunchTimer* timer = nullptr)
+                         uint64_t n_pixels, int rounds, int cus, hipStream_t s, const LaunchTimer* timer = nullptr,
+                         hipEvent_t acc_wait = nullptr)
 {
     const size_t lds_t = BATCH ? 0 : render_lds_bytes(a);
     const size_t lds_s = BATCH ? 0 : 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
@@ -858,23 +859,27 @@ static hipError_t wf_run(const TraceArgs& a, const BatchTable& B, const WfPaths&
         }
         const uint64_t gp = (n_pixels + 255) / 256;
         const unsigned ga = (unsigned)(gp < (uint64_t)cus * 8u ? gp : (uint64_t)cus * 8u);
+        if (acc_wait) {
+            const hipError_t e = hipStreamWaitEvent(s, acc_wait, 0);
+            if (e != hipSuccess) return e;
+        }
         wf_accumulate_kernel<BATCH><<<ga, 256, 0, s>>>(a, B, P, g0, g1, n_pixels);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats,
-                                   const LaunchTimer* timer)
+                                   const LaunchTimer* timer, hipEvent_t acc_wait)
 {
     const BatchTable B = { nullptr, nullptr, nullptr, 0u, nullptr };
     const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
     const uint64_t total = WH * a.count;
     const int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
     if (stats)
-        return a.lds_bits ? wf_run<true, true, false>(a, B, P, total, WH, rounds, cus, s, timer)
-                          : wf_run<true, false, false>(a, B, P, total, WH, rounds, cus, s, timer);
-    return a.lds_bits ? wf_run<false, true, false>(a, B, P, total, WH, rounds, cus, s, timer)
-                      : wf_run<false, false, false>(a, B, P, total, WH, rounds, cus, s, timer);
+        return a.lds_bits ? wf_run<true, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
+                          : wf_run<true, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
+    return a.lds_bits ? wf_run<false, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
+                      : wf_run<false, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
 }
 
 hipError_t launch_render_batch(const TraceArgs& a0, const BatchTable& B, const WfPaths& P, uint64_t total_paths,
