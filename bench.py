@@ -31,11 +31,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s; time-to-RMSE 1e-2 vs FP64 ref, plain vs batched+radiance-cache"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the dominant kernel on the full workload, from
-# rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), each factor being what
-# the counter reads for a copy kernel of known size
+# HBM bytes per launch of the dominant kernel on the full workload (one and
+# two streams), from rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000),
+# each factor being what the counter reads for a copy kernel of known size
 # (profiles/wf_traverse_kernel_full_1024x1024_16spp.md).
-TRAFFIC: dict = {("wf_traverse_kernel", 1): 340897464}
+TRAFFIC: dict = {("wf_traverse_kernel", 1): 340897464, ("wf_traverse_kernel", 2): 171144253}
 
 
 def env_lookup_bytes(cu_w: int, cv_n: int) -> tuple[int, int]:
@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--budget-s", type=float, default=3.0, help="cache workload: render time of each contender")
     ap.add_argument("--cache-bounce", type=int, default=2, help="cache workload: bounce at which paths end")
     ap.add_argument("--train-paths", type=int, default=16384, help="cache workload: training paths per frame")
+    ap.add_argument("--train-extra", type=int, default=2,
+                    help="cache workload: bounces training paths trace past the cache bounce (-1: to the end)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the path shards by "
@@ -335,7 +337,8 @@ def bench_cache(args, api, workloads, target=1e-2):
     cache = api.RadianceCache(cache_ctx, seed=1)
 
     def cache_step(n):
-        st = cache.render(wl.seed, n, 1, cache_bounce=args.cache_bounce, train_paths=args.train_paths, lr=3e-3)
+        st = cache.render(wl.seed, n, 1, cache_bounce=args.cache_bounce, train_paths=args.train_paths, lr=3e-3,
+                          train_extra=args.train_extra)
         stats.append(st)
         return st["ms"]
 
@@ -359,7 +362,7 @@ def bench_cache(args, api, workloads, target=1e-2):
         "config": {"workload": "full-shading 256^3 scene at 1920x1080, 1 spp per frame; FP64 reference of "
                                f"{args.ref_spp} spp; target RMSE {target} (tonemapped mean, pixels x RGB)",
                    "plain": plain, "cache": cached, "cache_bounce": args.cache_bounce,
-                   "train_paths_per_frame": args.train_paths,
+                   "train_paths_per_frame": args.train_paths, "train_extra_bounces": args.train_extra,
                    "cache_last_frame": stats[-1] if stats else None,
                    "cache_mean_samples_per_frame": round(sum(x["samples"] for x in stats) / max(1, len(stats)), 1),
                    "cache_mean_queries_per_frame": round(sum(x["queries"] for x in stats) / max(1, len(stats)), 1),

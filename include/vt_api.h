@@ -215,8 +215,10 @@ int vt_autotune(vt_context* ctx, uint32_t passes, vt_tuning* out);
  * A 288-64-64-64-64-64-3 ReLU MLP over a 12-band sin/cos encoding of a
  * surface query (12 raw floats: position normalised to the volume bounds,
  * (direction+1)/2, (normal+1)/2, albedo) predicting outgoing RGB radiance.
- * Inference runs FP16 matrix-core tiles with FP32 accumulation; training is
- * FP32 (relative L2 loss, Adam).  Parameter vector layout: W1[64][288], b1,
+ * Inference runs FP16 matrix-core tiles with FP32 accumulation; training
+ * keeps FP32 master weights: forward as the inference, backward and weight
+ * gradients on FP32 matrix-core tiles (relative L2 loss, Adam).  Parameter
+ * vector layout: W1[64][288], b1,
  * W2..W5[64][64], b2..b5, W6[3][64], b6.  No counterpart in the reference. */
 typedef struct vt_cache vt_cache;
 int vt_cache_create(vt_context* ctx, uint64_t seed, vt_cache** out);
@@ -233,12 +235,14 @@ int vt_cache_train(vt_cache* cache, int64_t n, const float* raw, const float* ta
 
 /* Render `count` passes (pass n0+k, one sample per pixel each) with the
  * cache: paths end into the cache at their vertex of bounce cache_bounce
- * (-1: never), except about train_paths training paths per pass, which run
- * the plain integrator and give one sample per shaded vertex (target =
- * radiance gathered after the vertex / throughput at the vertex); after each
- * pass the cache takes Adam steps (learning rate lr) over those samples in
- * minibatches of at most `batch`.  Statistics of the last call in *out
- * (may be NULL). */
+ * (-1: never), except about train_paths training paths per pass, which
+ * trace train_extra more bounces (-1: to the end, as the plain integrator)
+ * before ending into the cache themselves, and give one sample per shaded
+ * vertex before that end (target = radiance gathered after the vertex,
+ * including the cache's prediction at the end, / throughput at the vertex);
+ * after each pass the cache takes Adam steps (learning rate lr) over those
+ * samples in minibatches of at most `batch`.  Statistics of the last call
+ * in *out (may be NULL). */
 typedef struct vt_cache_stats {
     uint64_t queries;       /* paths that ended into the cache */
     uint64_t samples;       /* training samples used */
@@ -247,7 +251,8 @@ typedef struct vt_cache_stats {
     double ms;              /* wall time of the call */
 } vt_cache_stats;
 int vt_render_cached(vt_context* ctx, vt_cache* cache, uint64_t seed, uint32_t n0, uint32_t count,
-                     int32_t cache_bounce, uint32_t train_paths, float lr, uint32_t batch, vt_cache_stats* out);
+                     int32_t cache_bounce, uint32_t train_paths, int32_t train_extra, float lr, uint32_t batch,
+                     vt_cache_stats* out);
 
 /* ---- probes -------------------------------------------------------------
  * First hit of every pixel's primary ray of frame `frame`: rec[8] per

@@ -184,6 +184,36 @@ def test_zero_cache_equals_paths_cut_at_the_cache_bounce(vtapi, cb):
     ref.close()
 
 
+def test_zero_cache_training_paths_end_later(vtapi):
+    """With a zero cache, training paths (every 6th pixel of the pass) that
+    end into the cache train_extra = 1 bounce after the others contribute
+    what the plain integrator with max_bounces = cb + 1 does; the other
+    pixels what it does with max_bounces = cb."""
+    from voxeltoy_amd import workloads
+    wl = cached_scene(vtapi, workloads, 5)
+    ctx = vtapi.Context(0)
+    workloads.load(ctx, wl)
+    k = vtapi.RadianceCache(ctx, seed=3)
+    k.set_params(np.zeros(k.n_params, np.float32))
+    wh = wl.width * wl.height
+    st = k.render(wl.seed, 0, 1, cache_bounce=1, train_paths=wh // 6, train_extra=1, lr=1e-3)
+    assert st["queries"] > 0 and st["steps"] == 1 and st["samples"] > 0
+    img = ctx.read_accum().reshape(-1, 4)
+    refs = []
+    for b in (1, 2):
+        r = vtapi.Context(0)
+        wl.bounces = b
+        workloads.load(r, wl)
+        r.render(wl.seed, 0, 1)
+        refs.append(r.read_accum().reshape(-1, 4))
+        r.close()
+    train = (np.arange(wh) % 6) == 0          # pass 0: offset 0
+    want = np.where(train[:, None], refs[1], refs[0])
+    assert np.array_equal(img.view(np.uint32), want.view(np.uint32))
+    k.close()
+    ctx.close()
+
+
 def test_cached_render_trains(vtapi):
     """Lambert-only scene: the plain estimator is finite everywhere there
     (the metal lobe of the full-shading materials yields non-finite samples
@@ -199,7 +229,7 @@ def test_cached_render_trains(vtapi):
     workloads.load(ctx, wl)
     k = vtapi.RadianceCache(ctx, seed=4)
     # 512 training paths of 3072 (every 6th pixel); the rest end into the cache
-    sts = [k.render(wl.seed, n, 1, cache_bounce=1, train_paths=512, lr=3e-3) for n in range(40)]
+    sts = [k.render(wl.seed, n, 1, cache_bounce=1, train_paths=512, lr=3e-3, train_extra=2) for n in range(40)]
     wh = wl.width * wl.height
     for st in sts:
         assert 0 < st["queries"] < wh and st["samples"] > 50 and st["steps"] == 1

@@ -105,7 +105,7 @@ SIGNATURES = {
     "vt_cache_gradient": (C.c_int, [C.c_void_p, C.c_int64, f32p, f32p, f32p, C.POINTER(C.c_float)]),
     "vt_cache_train": (C.c_int, [C.c_void_p, C.c_int64, f32p, f32p, C.c_float, C.POINTER(C.c_float)]),
     "vt_render_cached": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32,
-                                   C.c_float, C.c_uint32, C.POINTER(CacheStats)]),
+                                   C.c_int32, C.c_float, C.c_uint32, C.POINTER(CacheStats)]),
     "vt_render": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
     "vt_read_accum": (C.c_int, [C.c_void_p, f32p]),
     "vt_read_sum64": (C.c_int, [C.c_void_p, f64p]),
@@ -509,12 +509,13 @@ class RadianceCache:
         return loss.value
 
     def render(self, seed: int, n0: int, count: int, cache_bounce: int = 2, train_paths: int = 16384,
-               lr: float = 3e-3, batch: int = 16384) -> dict:
+               lr: float = 3e-3, batch: int = 16384, train_extra: int = -1) -> dict:
         """`count` passes of the context's frame with paths ending into the
-        cache at bounce `cache_bounce`, training after every pass."""
+        cache at bounce `cache_bounce` (training paths `train_extra` bounces
+        later, -1 = never), training after every pass."""
         st = CacheStats()
-        _check(lib().vt_render_cached(self.ctx._c, self._p, seed, n0, count, cache_bounce, train_paths, lr, batch,
-                                      C.byref(st)))
+        _check(lib().vt_render_cached(self.ctx._c, self._p, seed, n0, count, cache_bounce, train_paths, train_extra,
+                                      lr, batch, C.byref(st)))
         return st.as_dict()
 
     def close(self):

@@ -1259,11 +1259,12 @@ int vt_cache_train(vt_cache* c, int64_t n, const float* raw, const float* target
 }
 
 int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uint32_t count, int32_t cache_bounce,
-                     uint32_t train_paths, float lr, uint32_t batch, vt_cache_stats* out)
+                     uint32_t train_paths, int32_t train_extra, float lr, uint32_t batch, vt_cache_stats* out)
 {
     if (!c || !k || k->ctx != c) return fail(VT_ERR_ARG, "cache must belong to the context (and be alive)");
     if (!c->has_volume || !c->d_accum) return fail(VT_ERR_STATE, "no volume or no frame");
     if (cache_bounce < -1 || cache_bounce > 64) return fail(VT_ERR_ARG, "cache_bounce must be in [-1, 64]");
+    if (train_extra < -1 || train_extra > 64) return fail(VT_ERR_ARG, "train_extra must be in [-1, 64]");
     if (train_paths > 0 && !(lr > 0.0f)) return fail(VT_ERR_ARG, "learning rate must be positive");
     if (batch == 0) batch = 16384;
     if (batch > (1u << 20)) return fail(VT_ERR_LIMIT, "training batch above 2^20 samples");
@@ -1323,6 +1324,7 @@ int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uin
         vt::CacheRender C;
         std::memset(&C, 0, sizeof(C));
         C.cache_bounce = cache_bounce;
+        C.train_bounce = (cache_bounce < 0 || train_extra < 0) ? -1 : cache_bounce + train_extra;
         C.train_stride = stride;
         /* the training paths move across the image from pass to pass */
         C.train_offset = (uint32_t)(((uint64_t)n * 2654435761u) % stride);
@@ -1334,6 +1336,7 @@ int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uin
         C.q_id = k->d_qid;
         C.q_out = k->d_qout;
         C.q_count = k->d_counts;
+        C.q_done = k->d_counts + 2;
         C.s_raw = k->d_sraw;
         C.s_tgt = k->d_stgt;
         C.s_count = k->d_counts + 1;

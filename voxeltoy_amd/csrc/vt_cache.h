@@ -154,9 +154,9 @@ struct TrainScratch {
 hipError_t launch_mlp_pack(const float* params, _Float16* pack, float* bias, float* tpack, hipStream_t s);
 hipError_t launch_mlp_infer(const _Float16* pack, const float* bias, const float* raw, float* out, int64_t n,
                             int cus, hipStream_t s);
-/* The same with the query count read on the device (*n_dev, at most n_max). */
+/* The same for queries [*lo_dev, *n_dev) (read on the device; *n_dev at most n_max). */
 hipError_t launch_mlp_infer_dev(const _Float16* pack, const float* bias, const float* raw, float* out,
-                                const uint32_t* n_dev, int64_t n_max, int cus, hipStream_t s);
+                                const uint32_t* lo_dev, const uint32_t* n_dev, int64_t n_max, int cus, hipStream_t s);
 /* Training step of one batch: forward (FP16 MFMA, as inference) and
  * backward (FP32 MFMA) into the scratch, the weight-gradient GEMM into
  * K-split partials, then either the summed gradient into grad (grad != 0)

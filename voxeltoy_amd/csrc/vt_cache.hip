@@ -74,9 +74,11 @@ __global__ __launch_bounds__(INFER_THREADS) void mlp_infer_kernel(const h8* __re
                                                                   const float* __restrict__ bias,
                                                                   const float* __restrict__ raw,
                                                                   float* __restrict__ out, int64_t n,
+                                                                  const uint32_t* __restrict__ lo_dev,
                                                                   const uint32_t* __restrict__ n_dev)
 {
     if (n_dev) n = *n_dev;
+    const int64_t lo = lo_dev ? (int64_t)*lo_dev : 0;
     extern __shared__ __align__(16) uint8_t smem[];
     h8* wp = (h8*)smem;
     float* bs = (float*)(smem + MLP_PACK * 2);
@@ -89,9 +91,9 @@ __global__ __launch_bounds__(INFER_THREADS) void mlp_infer_kernel(const h8* __re
     const h8* w1 = wp;
     const h8* wh = wp + MLP_PACK_L1 / 8;
     const h8* wo = wp + (MLP_PACK_L1 + 4 * MLP_PACK_HID) / 8;
-    for (int64_t tile = wave; tile * 16 < n; tile += waves) {
+    for (int64_t tile = lo / 16 + wave; tile * 16 < n; tile += waves) {
         const int64_t q = tile * 16 + r;
-        const bool valid = q < n;
+        const bool valid = q >= lo && q < n;
         h8 B1[9];
         {
             float enc[72];
@@ -432,12 +434,12 @@ hipError_t launch_mlp_infer(const _Float16* pack, const float* bias, const float
     const int64_t want = (tiles + INFER_THREADS / 64 - 1) / (INFER_THREADS / 64);
     const int64_t cap = (int64_t)cus * 2;
     mlp_infer_kernel<<<(unsigned)(want < cap ? want : cap), INFER_THREADS, lds, s>>>((const h8*)pack, bias, raw, out,
-                                                                                    n, nullptr);
+                                                                                    n, nullptr, nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_mlp_infer_dev(const _Float16* pack, const float* bias, const float* raw, float* out,
-                                const uint32_t* n_dev, int64_t n_max, int cus, hipStream_t s)
+                                const uint32_t* lo_dev, const uint32_t* n_dev, int64_t n_max, int cus, hipStream_t s)
 {
     if (n_max <= 0) return hipSuccess;
     const size_t lds = (size_t)MLP_PACK * 2 + (size_t)MLP_BIAS * 4;
@@ -448,7 +450,7 @@ hipError_t launch_mlp_infer_dev(const _Float16* pack, const float* bias, const f
     const int64_t want = (tiles + INFER_THREADS / 64 - 1) / (INFER_THREADS / 64);
     const int64_t cap = (int64_t)cus * 2;
     mlp_infer_kernel<<<(unsigned)(want < cap ? want : cap), INFER_THREADS, lds, s>>>((const h8*)pack, bias, raw, out,
-                                                                                    n_max, n_dev);
+                                                                                    n_max, lo_dev, n_dev);
     return hipGetLastError();
 }
 

@@ -92,7 +92,8 @@ struct BatchTable {
  * vertex (query, throughput, radiance so far: 18 floats), and when they end
  * append one training sample per vertex: target = (R_end - R_v) / T_v. */
 struct CacheRender {
-    int32_t cache_bounce;
+    int32_t cache_bounce;   /* bounce at which paths end into the cache (-1 never) */
+    int32_t train_bounce;   /* the same for training paths */
     uint32_t train_stride, train_offset, train_slots, max_vertices;
     float* vrec;            /* [train_slots][max_vertices][18] */
     uint32_t* vrec_n;       /* [train_slots] */
@@ -100,6 +101,7 @@ struct CacheRender {
     uint32_t* q_id;
     float* q_out;           /* [paths][3] */
     uint32_t* q_count;
+    uint32_t* q_done;       /* queries already answered */
     float* s_raw;           /* [s_cap][12] */
     float* s_tgt;           /* [s_cap][3] */
     uint32_t* s_count;

@@ -12,6 +12,8 @@
  * (when they fit) and the environment's marginal CDF.
  */
 #include "vt_kernels.h"
+
+#include <algorithm>
 #include "vt_cache.h"
 
 namespace vt {
@@ -717,8 +719,10 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0
             const uint32_t slot1 = CACHE ? P.meta[id].w : 0u;   /* training slot + 1 */
             bool vrec = false;
             const int st = shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, kind, cnt,
-                                                    slot1 ? -1 : C.cache_bounce, vraw, &vrec);
-            if (CACHE && vrec && slot1) {
+                                                    slot1 ? C.train_bounce : C.cache_bounce, vraw, &vrec);
+            /* a training path records its vertices, except the one where it
+             * ends into the cache (whose target would be the prediction) */
+            if (CACHE && vrec && slot1 && st != 2) {
                 const uint32_t slot = slot1 - 1u, nv = C.vrec_n[slot];
                 if (nv < C.max_vertices) {
                     float* e = C.vrec + ((size_t)slot * C.max_vertices + nv) * 18u;
@@ -890,11 +894,13 @@ hipError_t launch_render_batch(const TraceArgs& a0, const BatchTable& B, const W
     return wf_run<false, false, true>(a0, B, P, total_paths, total_pixels, rounds, cus, s);
 }
 
-/* Paths that ended into the cache: radiance += throughput * max(0, cache). */
+/* Paths that ended into the cache since the last apply (queries
+ * [*q_done, *q_count)): radiance += throughput * max(0, cache); a training
+ * path that ended so emits its samples with that radiance as the end. */
 __global__ __launch_bounds__(256) void cache_apply_kernel(TraceArgs a, WfPaths P, CacheRender C)
 {
-    const uint32_t n = *C.q_count;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t lo = *C.q_done, n = *C.q_count;
+    for (uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t id = C.q_id[i];
         const float4 r = P.rad[id], t = P.thr[id];
         const float y0 = C.q_out[(size_t)i * 3u], y1 = C.q_out[(size_t)i * 3u + 1], y2 = C.q_out[(size_t)i * 3u + 2];
@@ -903,7 +909,14 @@ __global__ __launch_bounds__(256) void cache_apply_kernel(TraceArgs a, WfPaths P
                           y2 > 0.0f ? r.z + t.z * y2 : r.z);
         const f3 c = tonemap(a.s, rad);
         P.smp[id] = make_float4(c.x, c.y, c.z, 1.0f);
+        const uint32_t slot1 = P.meta[id].w;
+        if (slot1) emit_training(C, slot1 - 1u, rad);
     }
+}
+
+__global__ void cache_mark_kernel(CacheRender C)
+{
+    if (threadIdx.x == 0) *C.q_done = *C.q_count;
 }
 
 template <bool LDSB>
@@ -920,8 +933,12 @@ static hipError_t wf_run_cached(const TraceArgs& a, const WfPaths& P, const Cach
     const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
     if (WH > P.cap) return hipErrorInvalidValue;
     const uint32_t n = (uint32_t)WH;
-    const int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
+    int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
     const int qround = C.cache_bounce >= 0 ? 2 * C.cache_bounce : -1;
+    const int tround = C.train_bounce >= 0 ? 2 * C.train_bounce : -1;
+    /* every path still alive at round 2b is at bounce b, so once the plain
+     * and the training paths have ended into the cache no round is left */
+    if (qround >= 0 && tround >= 0) rounds = std::min(rounds, 1 + std::max(qround, tround));
     wf_reset_kernel<<<1, 64, 0, s>>>(P.cnt);
     const uint32_t gn = (n + WF_GENERATE_THREADS - 1) / WF_GENERATE_THREADS;
     const unsigned gg = (unsigned)(gn < (uint32_t)cus * 2u ? gn : (uint32_t)cus * 2u);
@@ -935,10 +952,12 @@ static hipError_t wf_run_cached(const TraceArgs& a, const WfPaths& P, const Cach
             wf_shade_kernel<false, PH_SHADOW, false, true><<<cus * bs1, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel, C);
         else
             wf_shade_kernel<false, PH_CONT, false, true><<<cus * bs2, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel, C);
-        if (r == qround) {
-            hipError_t e = launch_mlp_infer_dev((const _Float16*)pack, bias, C.q_raw, C.q_out, C.q_count, n, cus, s);
+        if (r == qround || r == tround) {
+            hipError_t e = launch_mlp_infer_dev((const _Float16*)pack, bias, C.q_raw, C.q_out, C.q_done, C.q_count, n,
+                                                cus, s);
             if (e != hipSuccess) return e;
             cache_apply_kernel<<<(unsigned)cus * 4u, 256, 0, s>>>(a, P, C);
+            cache_mark_kernel<<<1, 64, 0, s>>>(C);
         }
         sel ^= 1;
     }
