@@ -96,10 +96,11 @@ def main():
                          "graph launch against 64 sequential renders; cache: time to RMSE 1e-2 against an FP64 "
                          "reference, plain tracer against the radiance cache")
     ap.add_argument("--ref-spp", type=int, default=4096, help="cache workload: passes of the FP64 reference")
-    ap.add_argument("--budget-s", type=float, default=3.0, help="cache workload: render time of each contender")
+    ap.add_argument("--budget-s", type=float, default=2.0, help="cache workload: render time of each contender")
     ap.add_argument("--cache-bounce", type=int, default=2, help="cache workload: bounce at which paths end")
-    ap.add_argument("--train-paths", type=int, default=16384, help="cache workload: training paths per frame")
-    ap.add_argument("--train-extra", type=int, default=2,
+    ap.add_argument("--train-paths", type=int, default=65536,
+                    help="cache workload: training paths per frame (65536 give about 58k samples per frame)")
+    ap.add_argument("--train-extra", type=int, default=1,
                     help="cache workload: bounces training paths trace past the cache bounce (-1: to the end)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -279,8 +280,10 @@ def bench_cache(args, api, workloads, target=1e-2):
     """Time to RMSE `target` (over pixels x RGB of the tonemapped mean)
     against an FP64-accumulated reference of the same scene: the plain
     tracer (8 bounces) against radiance-cache rendering (paths end into the
-    cache after 2 bounces, 16k training paths per frame), one sample per
-    pixel per frame, each given the same render-time budget."""
+    cache at bounce --cache-bounce; --train-paths training paths per frame
+    trace --train-extra more bounces and train the cache), one sample per
+    pixel per frame, each given the same render-time budget; a third run
+    ends the paths at the same bounce into an untrained zero cache."""
     import numpy as np
     wl = workloads.full(width=1920, height=1080, spp=1, bounces=8)
     ref_ctx = api.Context(0)
@@ -298,6 +301,18 @@ def bench_cache(args, api, workloads, target=1e-2):
         d = img[..., :3].astype(np.float64) - ref
         d = d[finite & np.isfinite(img[..., :3]).all(axis=-1)]
         return float(np.sqrt(np.mean(d * d)))
+
+    def crossing(curve, level):
+        """Time (s) at which the RMSE curve first reaches `level`,
+        interpolated between checkpoints."""
+        prev = None
+        for t, e in curve:
+            if e <= level:
+                if prev is not None and prev[1] > level:
+                    return round((prev[0] + (prev[1] - level) / (prev[1] - e) * (t - prev[0])) / 1000.0, 3)
+                return round(t / 1000.0, 3)
+            prev = (t, e)
+        return None
 
     def run(step_fn, read_fn):
         """Frames until the time budget is spent; the RMSE is read at frames
@@ -322,7 +337,7 @@ def bench_cache(args, api, workloads, target=1e-2):
         curve.append((round(t, 2), round(e, 5)))
         return {"frames": n, "ms": round(t, 1), "final_rmse": curve[-1][1],
                 "time_to_target_s": None if first is None else round(first / 1000.0, 3),
-                "curve": curve}
+                "time_to_3e-3_s": crossing(curve, 3e-3), "curve": curve}
 
     plain_ctx = api.Context(0)
     workloads.load(plain_ctx, wl)
@@ -346,6 +361,18 @@ def bench_cache(args, api, workloads, target=1e-2):
 
     cached = run(cache_step, cache_ctx.read_accum)
     tp, tc = plain["time_to_target_s"], cached["time_to_target_s"]
+
+    # the same paths ending into a cache that predicts 0 and never trains:
+    # what the learned network adds over plain path truncation
+    cut_ctx = api.Context(0)
+    workloads.load(cut_ctx, wl)
+    zero = api.RadianceCache(cut_ctx, seed=1)
+    zero.set_params(np.zeros(zero.n_params, np.float32))
+
+    def cut_step(n):
+        return zero.render(wl.seed, n, 1, cache_bounce=args.cache_bounce, train_paths=0)["ms"]
+
+    truncated = run(cut_step, cut_ctx.read_accum)
     out = {
         "metric": METRIC,
         "value": tc,
@@ -361,7 +388,8 @@ def bench_cache(args, api, workloads, target=1e-2):
         "data": "synthetic",
         "config": {"workload": "full-shading 256^3 scene at 1920x1080, 1 spp per frame; FP64 reference of "
                                f"{args.ref_spp} spp; target RMSE {target} (tonemapped mean, pixels x RGB)",
-                   "plain": plain, "cache": cached, "cache_bounce": args.cache_bounce,
+                   "plain": plain, "cache": cached, "truncated_untrained": truncated,
+                   "cache_bounce": args.cache_bounce,
                    "train_paths_per_frame": args.train_paths, "train_extra_bounces": args.train_extra,
                    "cache_last_frame": stats[-1] if stats else None,
                    "cache_mean_samples_per_frame": round(sum(x["samples"] for x in stats) / max(1, len(stats)), 1),
@@ -372,8 +400,10 @@ def bench_cache(args, api, workloads, target=1e-2):
     }
     print(json.dumps(out))
     cache.close()
+    zero.close()
     plain_ctx.close()
     cache_ctx.close()
+    cut_ctx.close()
 
 
 if __name__ == "__main__":

@@ -243,6 +243,44 @@ def test_cached_render_trains(vtapi):
     ctx.close()
 
 
+def test_trained_cache_beats_untrained_truncation(vtapi):
+    """What the learned network adds: with the same paths ending into the
+    cache at bounce 1 (same seeds, so the same primary and light samples),
+    the trained cache's image is closer to a converged plain render than the
+    image of an untrained zero cache (plain path truncation)."""
+    import math as _m
+    from voxeltoy_amd import scenes, workloads
+    sc = scenes.small_scene(6, n=48)
+    bmin, bmax = workloads.reference_bounds(sc.shape)
+    cam = scenes.orbit_camera(bmin, bmax, _m.radians(125), _m.radians(35), 96, 72)
+    wl = workloads.Workload("g", sc, cam, 96, 72, spp=1, bounces=6, seed=5, bmin=bmin, bmax=bmax,
+                            env_top=(1.2, 1.3, 1.5), env_bottom=(0.6, 0.55, 0.5))
+    ref = vtapi.Context(0)
+    workloads.load(ref, wl, fp64=True)
+    ref.render(wl.seed + 1, 0, 1024)
+    # mean of the tonemapped samples, as the accumulation holds
+    truth = ref.read_sum64() / 1024.0
+    ref.close()
+    frames, wh = 64, 96 * 72
+    errs = {}
+    for name in ("trained", "zero"):
+        c = vtapi.Context(0)
+        workloads.load(c, wl)
+        k = vtapi.RadianceCache(c, seed=8)
+        if name == "zero":
+            k.set_params(np.zeros(k.n_params, np.float32))
+        for n in range(frames):
+            if name == "trained":
+                k.render(wl.seed, n, 1, cache_bounce=1, train_paths=wh // 4, train_extra=1, lr=3e-3)
+            else:
+                k.render(wl.seed, n, 1, cache_bounce=1, train_paths=0)
+        img = c.read_accum()[..., :3].astype(np.float64)
+        errs[name] = float(np.sqrt(np.mean((img - truth) ** 2)))
+        k.close()
+        c.close()
+    assert errs["trained"] < 0.8 * errs["zero"], errs
+
+
 def test_context_destroyed_before_its_cache(vtapi):
     """Destroying a context releases its caches: their calls then fail
     cleanly, and the next context works (no stale runtime error)."""

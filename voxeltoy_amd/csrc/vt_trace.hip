@@ -935,7 +935,8 @@ static hipError_t wf_run_cached(const TraceArgs& a, const WfPaths& P, const Cach
     const uint32_t n = (uint32_t)WH;
     int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
     const int qround = C.cache_bounce >= 0 ? 2 * C.cache_bounce : -1;
-    const int tround = C.train_bounce >= 0 ? 2 * C.train_bounce : -1;
+    /* without training paths nothing goes past the cache bounce */
+    const int tround = C.train_slots == 0 ? qround : (C.train_bounce >= 0 ? 2 * C.train_bounce : -1);
     /* every path still alive at round 2b is at bounce b, so once the plain
      * and the training paths have ended into the cache no round is left */
     if (qround >= 0 && tround >= 0) rounds = std::min(rounds, 1 + std::max(qround, tround));
