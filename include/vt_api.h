@@ -272,7 +272,7 @@ int vt_trace_rays(vt_context* ctx, int32_t n, const float* origins, const float*
  * (vt_set_volume); covered voxels are set to material offset
  * `material_offset` (must be a known offset).  If occupancy != NULL the
  * dense byte occupancy (1 = covered) of this call alone is also written
- * there. */
+ * there.  vt_get_stats then reports the voxelizer kernel's time. */
 int vt_voxelize(vt_context* ctx, const float* vertices, int64_t n_vertices, const uint32_t* indices,
                 int64_t n_triangles, const float* model_transform, int32_t mode, int32_t material_offset,
                 uint8_t* occupancy);

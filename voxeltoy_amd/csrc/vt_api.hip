@@ -1572,7 +1572,11 @@ int vt_voxelize(vt_context* c, const float* verts, int64_t nv, const uint32_t* i
         VT_HIP(vt::launch_fill_u8(d_occ, (uint64_t)nvox, 0, c->stream));
     }
     a.verts = d_v; a.idx = d_i; a.occ = d_occ;
+    VT_HIP(hipEventRecord(c->ev0, c->stream));
     VT_HIP(vt::launch_voxelize(a, c->stream));
+    VT_HIP(hipEventRecord(c->ev1, c->stream));
+    c->timed = true;           /* vt_get_stats: kernel_ms of the voxelizer launch */
+    c->stats_valid = false;
     {
         int r = refresh_occupancy(c);
         if (r != VT_OK) return r;
