@@ -180,6 +180,49 @@ __device__ __forceinline__ uint32_t block_key(int32_t x, int32_t y, int32_t z)
  * point lies outside the grid (steps = -1, nothing else written); advance()
  * runs at most `budget` steps and returns 0 while the march goes on, 1 on a
  * hit, 2 when it left the grid or ran out of steps. */
+/* The grid-independent part of raymarch's set-up (dda.h:7-35): origin
+ * moved by sign(d)*0.001, direction clamped by mix(d, 1e-5, step(|d|, 1e-5)),
+ * its reciprocal and sign.  Computed where a ray is made, so a traversal
+ * that takes a new ray skips three divisions. */
+struct PreRay {
+    f3 o, inv;
+    int32_t sx, sy, sz;
+};
+
+__device__ __forceinline__ PreRay prepare_ray(f3 o, f3 d)
+{
+    PreRay r;
+    r.o = o + mk(vt_signf(d.x), vt_signf(d.y), vt_signf(d.z)) * 0.001f;
+    /* mix(d, 1e-5, step(|d|, 1e-5)) as an exact select */
+    const float dx = vt_stepf(vt_absf(d.x), 1e-5f) != 0.0f ? 1e-5f : d.x;
+    const float dy = vt_stepf(vt_absf(d.y), 1e-5f) != 0.0f ? 1e-5f : d.y;
+    const float dz = vt_stepf(vt_absf(d.z), 1e-5f) != 0.0f ? 1e-5f : d.z;
+    r.inv = mk(1.0f / dx, 1.0f / dy, 1.0f / dz);
+    r.sx = dx > 0.0f ? 1 : (dx < 0.0f ? -1 : 0);
+    r.sy = dy > 0.0f ? 1 : (dy < 0.0f ? -1 : 0);
+    r.sz = dz > 0.0f ? 1 : (dz < 0.0f ? -1 : 0);
+    return r;
+}
+
+/* Queue encoding of a prepared ray: (origin, path id) and (1/d, tag with
+ * the three signs + 1 in bits 26..31; tags stay below 2^26). */
+__device__ __forceinline__ uint32_t pack_signs(const PreRay& r, uint32_t tag)
+{
+    return tag | ((uint32_t)(r.sx + 1) << 26) | ((uint32_t)(r.sy + 1) << 28) | ((uint32_t)(r.sz + 1) << 30);
+}
+__device__ __forceinline__ PreRay unpack_ray(float4 qa, float4 qb)
+{
+    PreRay r;
+    r.o = mk(qa.x, qa.y, qa.z);
+    r.inv = mk(qb.x, qb.y, qb.z);
+    const uint32_t w = __float_as_uint(qb.w);
+    r.sx = (int32_t)((w >> 26) & 3u) - 1;
+    r.sy = (int32_t)((w >> 28) & 3u) - 1;
+    r.sz = (int32_t)(w >> 30) - 1;
+    return r;
+}
+__device__ __forceinline__ uint32_t queue_tag(float4 qb) { return __float_as_uint(qb.w) & 0x03ffffffu; }
+
 struct DdaState {
     int32_t x, y, z, steps;
     float disx, disy, disz, ax, ay, az;
@@ -187,29 +230,22 @@ struct DdaState {
     uint32_t key;               /* block key of `mask` */
     uint64_t mask;
 
-    __device__ __forceinline__ bool init(const GridView& g, f3 o, f3 d)
+    __device__ __forceinline__ bool init(const GridView& g, f3 o, f3 d) { return init(g, prepare_ray(o, d)); }
+
+    __device__ __forceinline__ bool init(const GridView& g, const PreRay& r)
     {
-        o = o + mk(vt_signf(d.x), vt_signf(d.y), vt_signf(d.z)) * 0.001f;
-        const float vox = ((o.x - g.bmin[0]) * g.ext[0]) * g.rx;
-        const float voy = ((o.y - g.bmin[1]) * g.ext[1]) * g.ry;
-        const float voz = ((o.z - g.bmin[2]) * g.ext[2]) * g.rz;
+        const float vox = ((r.o.x - g.bmin[0]) * g.ext[0]) * g.rx;
+        const float voy = ((r.o.y - g.bmin[1]) * g.ext[1]) * g.ry;
+        const float voz = ((r.o.z - g.bmin[2]) * g.ext[2]) * g.rz;
         const float fx = floorf(vox), fy = floorf(voy), fz = floorf(voz);
         steps = -1;
         if (fx < 0.0f || fy < 0.0f || fz < 0.0f || fx >= g.rx || fy >= g.ry || fz >= g.rz) return false;
-        /* mix(d, 1e-5, step(|d|, 1e-5)) as an exact select */
-        const float dx = vt_stepf(vt_absf(d.x), 1e-5f) != 0.0f ? 1e-5f : d.x;
-        const float dy = vt_stepf(vt_absf(d.y), 1e-5f) != 0.0f ? 1e-5f : d.y;
-        const float dz = vt_stepf(vt_absf(d.z), 1e-5f) != 0.0f ? 1e-5f : d.z;
-        const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
-        const float sx = vt_signf(dx), sy = vt_signf(dy), sz = vt_signf(dz);
-        disx = (((fx - vox) + 0.5f) + sx * 0.5f) * ix;
-        disy = (((fy - voy) + 0.5f) + sy * 0.5f) * iy;
-        disz = (((fz - voz) + 0.5f) + sz * 0.5f) * iz;
+        disx = (((fx - vox) + 0.5f) + (float)r.sx * 0.5f) * r.inv.x;
+        disy = (((fy - voy) + 0.5f) + (float)r.sy * 0.5f) * r.inv.y;
+        disz = (((fz - voz) + 0.5f) + (float)r.sz * 0.5f) * r.inv.z;
         /* (mask*sign)*inc with mask in {0,1} is |inc| when the axis steps */
-        ax = vt_absf(ix); ay = vt_absf(iy); az = vt_absf(iz);
-        stx = sx > 0.0f ? 1 : (sx < 0.0f ? -1 : 0);
-        sty = sy > 0.0f ? 1 : (sy < 0.0f ? -1 : 0);
-        stz = sz > 0.0f ? 1 : (sz < 0.0f ? -1 : 0);
+        ax = vt_absf(r.inv.x); ay = vt_absf(r.inv.y); az = vt_absf(r.inv.z);
+        stx = r.sx; sty = r.sy; stz = r.sz;
         x = (int32_t)fx; y = (int32_t)fy; z = (int32_t)fz;
         steps = 0;
         key = 0xffffffffu;

@@ -66,8 +66,8 @@ struct WfPaths {
     uint64_t* rng;    /* PCG32 state (increment = 2*pixel+1) */
     int4* hit;        /* traversal result per queue position: x, y, z, kind << 8 | slot */
     float4* smp;      /* finished tonemapped sample of each path */
-    float4* qa0;      /* ray queue 0: origin.xyz, path slot id (as bits) */
-    float4* qb0;      /*              direction.xyz */
+    float4* qa0;      /* ray queue 0: origin.xyz moved by sign(d)*0.001, path slot id (as bits) */
+    float4* qb0;      /*              1/clamped direction, tag | direction signs << 26 (bits) */
     float4* qa1;      /* ray queue 1 */
     float4* qb1;
     uint32_t* cnt;    /* queue lengths: cnt[0] for q0, cnt[1] for q1 */

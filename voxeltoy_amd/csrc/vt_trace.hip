@@ -422,9 +422,10 @@ __device__ __forceinline__ void flush_counters(unsigned long long* stats, const 
 }
 
 /* Append the ray (o, d) of path `id` for the threads with `keep` to a ray
- * queue (length counter *n); the queue stores the rays themselves (and the
- * path's scene index `tag` in a batch) so the traversal reads them in queue
- * order.  One atomic per workgroup of NT
+ * queue (length counter *n); the queue stores the rays themselves, already
+ * prepared for the march (prepare_ray), and the path's scene index `tag` in
+ * a batch, so the traversal reads them in queue order.  One atomic per
+ * workgroup of NT
  * threads: every queue counter is a single device address that all XCDs
  * contend for.  Every thread of the workgroup must call it. */
 template <int NT>
@@ -445,8 +446,9 @@ __device__ __forceinline__ void block_append(bool keep, uint32_t id, uint32_t ta
     __syncthreads();
     if (keep) {
         const uint32_t j = bbase + wsum[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-        qa[j] = make_float4(o.x, o.y, o.z, __uint_as_float(id));
-        qb[j] = make_float4(d.x, d.y, d.z, __uint_as_float(tag));
+        const PreRay r = prepare_ray(o, d);
+        qa[j] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(id));
+        qb[j] = make_float4(r.inv.x, r.inv.y, r.inv.z, __uint_as_float(pack_signs(r, tag)));
     }
     __syncthreads();
 }
@@ -605,8 +607,8 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
                 if (j != 0xffffffffu) {
                     const float4 o = qa[j], d = qb[j];
                     jq = j;
-                    if (BATCH) gl = B.tab[__float_as_uint(d.w)].g;
-                    if (D.init(G, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z))) {
+                    if (BATCH) gl = B.tab[queue_tag(d)].g;
+                    if (D.init(G, unpack_ray(o, d))) {
                         has = true;
                     } else {
                         P.hit[j] = make_int4(0, 0, 0, KIND_MISS << 8);
