@@ -89,8 +89,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--mode", type=int, default=1, help="0 megakernel, 1 wavefront (default)")
-    ap.add_argument("--streams", type=int, default=2,
-                    help="wavefront: 2 (default) runs the step's pass halves on two streams, 1 on one")
+    ap.add_argument("--streams", type=int, default=1,
+                    help="wavefront: 1 (default) runs the step on one stream, 2 runs its pass halves on two streams")
     ap.add_argument("--workload", default="full", choices=["full", "batch", "cache", "voxelize"],
                     help="full: the 256^3 full-shading scene (default); batch: 64 mixed scenes, one batched "
                          "graph launch against 64 sequential renders; cache: time to RMSE 1e-2 against an FP64 "

@@ -164,7 +164,7 @@ int vt_get_stats(vt_context* ctx, vt_stats* out);
  * second half of the passes on two streams of the device (1 = one stream).
  * Images are identical (the halves accumulate in pass order); the second
  * half's paths take their own path buffers, so the in-flight capacity
- * (vt_set_render_mode max_paths) is split between the halves.  Default 2;
+ * (vt_set_render_mode max_paths) is split between the halves.  Default 1;
  * no counterpart in the reference. */
 int vt_set_overlap(vt_context* ctx, int32_t streams);
 /* Per-launch timing of the traversal kernel: when enabled, each wavefront

@@ -72,7 +72,7 @@ struct vt_context {
     bool timed = false;
     /* second stream: a wavefront render of many passes runs its two halves
      * on two streams, so each half's launch tails overlap the other's bulk */
-    int overlap = 2;
+    int overlap = 1;
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_half = nullptr, ev_join = nullptr;
     vt::WfPaths wf2{};
@@ -327,8 +327,11 @@ vt::TraceArgs trace_args(const vt_context* c)
 
 void free_paths(vt::WfPaths& P, uint64_t& alloc)
 {
-    dfree(P.o); dfree(P.d); dfree(P.bo); dfree(P.thr); dfree(P.rad); dfree(P.dir); dfree(P.fac); dfree(P.nd);
-    dfree(P.tgt); dfree(P.meta); dfree(P.rng); dfree(P.hit); dfree(P.smp);
+    for (vt::PathBuf& B : P.s) {
+        dfree(B.o); dfree(B.nd); dfree(B.thr); dfree(B.rad); dfree(B.dir); dfree(B.fac);
+        dfree(B.tgt); dfree(B.meta); dfree(B.rng);
+    }
+    dfree(P.hit); dfree(P.smp);
     dfree(P.qa0); dfree(P.qb0); dfree(P.qa1); dfree(P.qb1); dfree(P.cnt);
     P.cap = 0;
     alloc = 0;
@@ -342,10 +345,12 @@ int alloc_paths(vt::WfPaths& P, uint64_t& alloc, uint64_t slots)
     if (alloc >= slots) { P.cap = slots; return VT_OK; }
     free_paths(P, alloc);
     const size_t n = (size_t)slots;
-    VT_HIP(hipMalloc(&P.o, n * 16)); VT_HIP(hipMalloc(&P.d, n * 16)); VT_HIP(hipMalloc(&P.bo, n * 16));
-    VT_HIP(hipMalloc(&P.thr, n * 16)); VT_HIP(hipMalloc(&P.rad, n * 16)); VT_HIP(hipMalloc(&P.dir, n * 16));
-    VT_HIP(hipMalloc(&P.fac, n * 16)); VT_HIP(hipMalloc(&P.nd, n * 16)); VT_HIP(hipMalloc(&P.tgt, n * 16));
-    VT_HIP(hipMalloc(&P.meta, n * 16)); VT_HIP(hipMalloc(&P.rng, n * 8)); VT_HIP(hipMalloc(&P.hit, n * 16));
+    for (vt::PathBuf& B : P.s) {
+        VT_HIP(hipMalloc(&B.o, n * 16)); VT_HIP(hipMalloc(&B.nd, n * 16)); VT_HIP(hipMalloc(&B.thr, n * 16));
+        VT_HIP(hipMalloc(&B.rad, n * 16)); VT_HIP(hipMalloc(&B.dir, n * 16)); VT_HIP(hipMalloc(&B.fac, n * 16));
+        VT_HIP(hipMalloc(&B.tgt, n * 16)); VT_HIP(hipMalloc(&B.meta, n * 16)); VT_HIP(hipMalloc(&B.rng, n * 8));
+    }
+    VT_HIP(hipMalloc(&P.hit, n * 16));
     VT_HIP(hipMalloc(&P.smp, n * 16));
     VT_HIP(hipMalloc(&P.qa0, n * 16)); VT_HIP(hipMalloc(&P.qb0, n * 16));
     VT_HIP(hipMalloc(&P.qa1, n * 16)); VT_HIP(hipMalloc(&P.qb1, n * 16));
@@ -913,7 +918,7 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
     if (graph) {
         const std::vector<uint64_t> key = { (uint64_t)n, total, pixels, b->wf.cap, (uint64_t)max_bounces,
                                             (uint64_t)stats, (uint64_t)a0.dda_budget, (uint64_t)a0.refill_min,
-                                            (uint64_t)a0.refill_chunk, (uint64_t)(uintptr_t)b->wf.o };
+                                            (uint64_t)a0.refill_chunk, (uint64_t)(uintptr_t)b->wf.s[0].o };
         if (!b->exec || key != b->graph_key) {
             if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
             hipGraph_t g = nullptr;
@@ -1000,7 +1005,8 @@ struct vt_cache {
     /* cache rendering buffers */
     uint64_t qcap = 0;
     float *d_qraw = nullptr, *d_qout = nullptr;
-    uint32_t* d_qid = nullptr;
+    uint4* d_qmeta = nullptr;
+    float4 *d_qrad = nullptr, *d_qthr = nullptr;
     uint32_t* d_counts = nullptr;      /* [0] queries, [1] samples */
     uint64_t vcap = 0;                 /* train slots x vertices */
     float* d_vrec = nullptr;
@@ -1177,7 +1183,7 @@ static void cache_release(vt_cache* c)
     dfree(c->d_tpack); dfree(c->d_part); dfree(c->d_loss); dfree(c->d_steps); dfree(c->d_nsamp); dfree(c->d_tally);
     dfree(c->d_X); dfree(c->d_H); dfree(c->d_DZ); dfree(c->d_DY); dfree(c->d_raw);
     dfree(c->d_tgt); dfree(c->d_iraw); dfree(c->d_iout);
-    dfree(c->d_qraw); dfree(c->d_qout); dfree(c->d_qid); dfree(c->d_counts);
+    dfree(c->d_qraw); dfree(c->d_qout); dfree(c->d_qmeta); dfree(c->d_qrad); dfree(c->d_qthr); dfree(c->d_counts);
     dfree(c->d_vrec); dfree(c->d_vrec_n); dfree(c->d_sraw); dfree(c->d_stgt);
     (void)hipGetLastError();
     auto& v = c->ctx->caches;
@@ -1277,11 +1283,13 @@ int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uin
     if (r != VT_OK) return r;
     /* buffers */
     if (k->qcap < WH) {
-        dfree(k->d_qraw); dfree(k->d_qout); dfree(k->d_qid);
+        dfree(k->d_qraw); dfree(k->d_qout); dfree(k->d_qmeta); dfree(k->d_qrad); dfree(k->d_qthr);
         k->qcap = 0;
         VT_HIP(hipMalloc(&k->d_qraw, WH * 12 * 4));
         VT_HIP(hipMalloc(&k->d_qout, WH * 3 * 4));
-        VT_HIP(hipMalloc(&k->d_qid, WH * 4));
+        VT_HIP(hipMalloc(&k->d_qmeta, WH * 16));
+        VT_HIP(hipMalloc(&k->d_qrad, WH * 16));
+        VT_HIP(hipMalloc(&k->d_qthr, WH * 16));
         k->qcap = WH;
     }
     if (!k->d_counts) VT_HIP(hipMalloc(&k->d_counts, 16));
@@ -1333,7 +1341,9 @@ int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uin
         C.vrec = k->d_vrec;
         C.vrec_n = k->d_vrec_n;
         C.q_raw = k->d_qraw;
-        C.q_id = k->d_qid;
+        C.q_meta = k->d_qmeta;
+        C.q_rad = k->d_qrad;
+        C.q_thr = k->d_qthr;
         C.q_out = k->d_qout;
         C.q_count = k->d_counts;
         C.q_done = k->d_counts + 2;

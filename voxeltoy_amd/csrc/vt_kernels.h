@@ -52,21 +52,27 @@ struct PathState {
 
 /* Wavefront path storage: one slot per path of a chunk of the launch's
  * (pass, pixel) space, structure-of-arrays in HBM. */
-struct WfPaths {
-    float4* o;        /* ray_o.xyz, bsdf_pdf */
-    float4* d;        /* ray_d.xyz */
-    float4* bo;       /* basis_o.xyz */
+/* Path state by queue position: the fields of the path whose ray sits at
+ * position j of a ray queue are at index j of that queue's buffer, so shade
+ * kernels read and write them contiguously (a surviving path moves to its
+ * position in the next queue). */
+struct PathBuf {
+    float4* o;        /* ray_o (or basis origin after generate / shadow).xyz, bsdf_pdf */
+    float4* nd;       /* continuation direction (camera direction after generate) */
     float4* thr;      /* throughput */
     float4* rad;      /* radiance */
     float4* dir;      /* pending light-sample contribution */
     float4* fac;      /* continuation factor */
-    float4* nd;       /* continuation direction */
     int4* tgt;        /* emissive target voxel, light_is_voxel */
-    uint4* meta;      /* pixel, pass, phase | bounces << 8 */
+    uint4* meta;      /* pixel, sample slot of the chunk, bounces, scene or training slot + 1 */
     uint64_t* rng;    /* PCG32 state (increment = 2*pixel+1) */
+};
+
+struct WfPaths {
+    PathBuf s[2];     /* state of the paths in queue 0 / queue 1 */
     int4* hit;        /* traversal result per queue position: x, y, z, kind << 8 | slot */
-    float4* smp;      /* finished tonemapped sample of each path */
-    float4* qa0;      /* ray queue 0: origin.xyz moved by sign(d)*0.001, path slot id (as bits) */
+    float4* smp;      /* finished tonemapped sample of each path of the chunk */
+    float4* qa0;      /* ray queue 0: origin.xyz moved by sign(d)*0.001, sample slot (as bits) */
     float4* qb0;      /*              1/clamped direction, tag | direction signs << 26 (bits) */
     float4* qa1;      /* ray queue 1 */
     float4* qb1;
@@ -98,7 +104,9 @@ struct CacheRender {
     float* vrec;            /* [train_slots][max_vertices][18] */
     uint32_t* vrec_n;       /* [train_slots] */
     float* q_raw;           /* [paths][12] */
-    uint32_t* q_id;
+    uint4* q_meta;          /* [paths] metadata of the path that ended into the cache */
+    float4* q_rad;          /* [paths] its radiance and throughput at that point */
+    float4* q_thr;
     float* q_out;           /* [paths][3] */
     uint32_t* q_count;
     uint32_t* q_done;       /* queries already answered */

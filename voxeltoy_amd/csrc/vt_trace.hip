@@ -316,34 +316,41 @@ __global__ __launch_bounds__(256) void render_kernel(TraceArgs a)
  * registers and keeps many waves resident; shading consumes the hits and
  * appends surviving paths to the next queue with one atomic per wave. */
 
-/* Path state in HBM, moved per phase.  Every path of a traversal round is in
- * the same phase (round 0 PRIMARY, odd rounds SHADOW, even rounds CONT), so
- * each shade kernel moves only the fields its phase reads and writes:
- *   generate -> bo, d, meta, rng        (throughput 1 and radiance 0 implied)
- *   PRIMARY  <- bo, d, meta, rng        -> o, dir, fac, nd, tgt, rng, rad, thr
- *   SHADOW   <- o, nd, tgt, rad, thr, dir, fac   -> rad, thr
- *   CONT     <- o, nd, thr, rad, meta, rng       -> o, dir, fac, nd, tgt, rng, bounces
+/* Path state in HBM, by queue position (PathBuf).  Every path of a
+ * traversal round is in the same phase (round 0 PRIMARY, odd rounds SHADOW,
+ * even rounds CONT); a shade kernel reads the fields its phase needs at the
+ * path's position j in the current buffer and writes the fields the next
+ * phase needs at its position k in the next queue's buffer:
+ *   generate -> o (= basis origin), nd (= camera direction), meta, rng
+ *   PRIMARY  <- o, nd, meta, rng           -> o, nd, thr, rad, dir, fac, tgt, meta, rng
+ *   SHADOW   <- o, nd, thr, rad, dir, fac, tgt, meta, rng -> o, nd, thr, rad, meta, rng
+ *   CONT     <- o, nd, thr, rad, meta, rng -> o, nd, thr, rad, dir, fac, tgt, meta, rng
  * After SHADOW, basis_o = ray_o and ray_d = next_d, so CONT reads them from
  * o and nd. */
-__device__ __forceinline__ void store_generated(const WfPaths& P, uint32_t id, const PathState& S, uint32_t pix,
-                                                uint32_t pass, uint32_t scene)
+__device__ __forceinline__ void store_generated(const PathBuf& N, uint32_t k, const PathState& S, uint32_t pix,
+                                                uint32_t slot, uint32_t tag)
 {
-    P.bo[id] = make_float4(S.basis_o.x, S.basis_o.y, S.basis_o.z, 0.0f);
-    P.d[id] = make_float4(S.ray_d.x, S.ray_d.y, S.ray_d.z, 0.0f);
-    P.meta[id] = make_uint4(pix, pass, (uint32_t)S.bounces, scene);
-    P.rng[id] = S.rng.state;
+    N.o[k] = make_float4(S.basis_o.x, S.basis_o.y, S.basis_o.z, 0.0f);
+    N.nd[k] = make_float4(S.ray_d.x, S.ray_d.y, S.ray_d.z, 0.0f);
+    N.meta[k] = make_uint4(pix, slot, (uint32_t)S.bounces, tag);
+    N.rng[k] = S.rng.state;
 }
 
 template <int PHASE>
-__device__ __forceinline__ void load_phase(const WfPaths& P, uint32_t id, PathState& S)
+__device__ __forceinline__ void load_phase(const PathBuf& B, uint32_t j, PathState& S, uint4& me)
 {
     S.phase = PHASE;
     S.direct = S.factor = S.next_d = mk(0.0f, 0.0f, 0.0f);
     S.ex = S.ey = S.ez = S.light_is_voxel = 0;
     S.bsdf_pdf = 0.0f;
+    me = B.meta[j];
+    S.bounces = (int32_t)me.z;
+    S.rng.state = B.rng[j];
+    S.rng.inc = ((uint64_t)me.x << 1u) | 1u;
+    const float4 o = B.o[j], nd = B.nd[j];
     if (PHASE == PH_SHADOW) {
-        const float4 o = P.o[id], nd = P.nd[id], th = P.thr[id], rd = P.rad[id], di = P.dir[id], fa = P.fac[id];
-        const int4 tg = P.tgt[id];
+        const float4 th = B.thr[j], rd = B.rad[j], di = B.dir[j], fa = B.fac[j];
+        const int4 tg = B.tgt[j];
         S.ray_o = mk(o.x, o.y, o.z);
         S.bsdf_pdf = o.w;
         S.next_d = mk(nd.x, nd.y, nd.z);
@@ -354,23 +361,16 @@ __device__ __forceinline__ void load_phase(const WfPaths& P, uint32_t id, PathSt
         S.direct = mk(di.x, di.y, di.z);
         S.factor = mk(fa.x, fa.y, fa.z);
         S.ex = tg.x; S.ey = tg.y; S.ez = tg.z; S.light_is_voxel = tg.w;
-        S.bounces = 0;
-        S.rng.state = 0; S.rng.inc = 1;
         return;
     }
-    const uint4 me = P.meta[id];
-    S.bounces = (int32_t)me.z;
-    S.rng.state = P.rng[id];
-    S.rng.inc = ((uint64_t)me.x << 1u) | 1u;
     if (PHASE == PH_PRIMARY) {
-        const float4 bo = P.bo[id], d = P.d[id];
-        S.basis_o = mk(bo.x, bo.y, bo.z);
-        S.ray_d = mk(d.x, d.y, d.z);
+        S.basis_o = mk(o.x, o.y, o.z);
+        S.ray_d = mk(nd.x, nd.y, nd.z);
         S.ray_o = S.basis_o;
         S.throughput = mk(1.0f, 1.0f, 1.0f);
         S.radiance = mk(0.0f, 0.0f, 0.0f);
     } else {
-        const float4 o = P.o[id], nd = P.nd[id], th = P.thr[id], rd = P.rad[id];
+        const float4 th = B.thr[j], rd = B.rad[j];
         S.basis_o = mk(o.x, o.y, o.z);
         S.ray_o = S.basis_o;
         S.bsdf_pdf = o.w;
@@ -381,27 +381,27 @@ __device__ __forceinline__ void load_phase(const WfPaths& P, uint32_t id, PathSt
 }
 
 /* Store after a shade step that continues the path (the next phase is
- * SHADOW after PRIMARY/CONT, CONT after SHADOW). */
+ * SHADOW after PRIMARY/CONT, CONT after SHADOW) at position k of the next
+ * queue's buffer. */
 template <int PHASE>
-__device__ __forceinline__ void store_phase(const WfPaths& P, uint32_t id, const PathState& S)
+__device__ __forceinline__ void store_phase(const PathBuf& N, uint32_t k, const PathState& S, uint4 me)
 {
+    N.rad[k] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
+    N.thr[k] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
+    N.rng[k] = S.rng.state;
+    me.z = (uint32_t)S.bounces;
+    N.meta[k] = me;
     if (PHASE == PH_SHADOW) {
-        P.rad[id] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
-        P.thr[id] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
+        /* next: CONT from basis_o (= ray_o) along ray_d (= next_d) */
+        N.o[k] = make_float4(S.basis_o.x, S.basis_o.y, S.basis_o.z, S.bsdf_pdf);
+        N.nd[k] = make_float4(S.ray_d.x, S.ray_d.y, S.ray_d.z, 0.0f);
         return;
     }
-    P.o[id] = make_float4(S.ray_o.x, S.ray_o.y, S.ray_o.z, S.bsdf_pdf);
-    P.dir[id] = make_float4(S.direct.x, S.direct.y, S.direct.z, 0.0f);
-    P.fac[id] = make_float4(S.factor.x, S.factor.y, S.factor.z, 0.0f);
-    P.nd[id] = make_float4(S.next_d.x, S.next_d.y, S.next_d.z, 0.0f);
-    P.tgt[id] = make_int4(S.ex, S.ey, S.ez, S.light_is_voxel);
-    P.rng[id] = S.rng.state;
-    if (PHASE == PH_PRIMARY) {
-        P.rad[id] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
-        P.thr[id] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
-    } else {
-        P.meta[id].z = (uint32_t)S.bounces;
-    }
+    N.o[k] = make_float4(S.ray_o.x, S.ray_o.y, S.ray_o.z, S.bsdf_pdf);
+    N.nd[k] = make_float4(S.next_d.x, S.next_d.y, S.next_d.z, 0.0f);
+    N.dir[k] = make_float4(S.direct.x, S.direct.y, S.direct.z, 0.0f);
+    N.fac[k] = make_float4(S.factor.x, S.factor.y, S.factor.z, 0.0f);
+    N.tgt[k] = make_int4(S.ex, S.ey, S.ez, S.light_is_voxel);
 }
 
 template <bool STATS>
@@ -421,16 +421,16 @@ __device__ __forceinline__ void flush_counters(unsigned long long* stats, const 
     if (threadIdx.x < 6 && red[threadIdx.x]) atomicAdd(&stats[threadIdx.x], red[threadIdx.x]);
 }
 
-/* Append the ray (o, d) of path `id` for the threads with `keep` to a ray
- * queue (length counter *n); the queue stores the rays themselves, already
- * prepared for the march (prepare_ray), and the path's scene index `tag` in
- * a batch, so the traversal reads them in queue order.  One atomic per
- * workgroup of NT
+/* Append the ray (o, d) for the threads with `keep` to a ray queue (length
+ * counter *n) and return each such thread's position; the queue stores the
+ * rays themselves, already prepared for the march (prepare_ray), the path's
+ * sample slot `id` and its scene index `tag` in a batch, so the traversal
+ * reads them in queue order.  One atomic per workgroup of NT
  * threads: every queue counter is a single device address that all XCDs
  * contend for.  Every thread of the workgroup must call it. */
 template <int NT>
-__device__ __forceinline__ void block_append(bool keep, uint32_t id, uint32_t tag, f3 o, f3 d, float4* qa, float4* qb,
-                                             uint32_t* n)
+__device__ __forceinline__ uint32_t block_append(bool keep, uint32_t id, uint32_t tag, f3 o, f3 d, float4* qa,
+                                                 float4* qb, uint32_t* n)
 {
     __shared__ uint32_t wsum[NT / 64];
     __shared__ uint32_t bbase;
@@ -444,13 +444,14 @@ __device__ __forceinline__ void block_append(bool keep, uint32_t id, uint32_t ta
         bbase = t ? atomicAdd(n, t) : 0u;
     }
     __syncthreads();
+    const uint32_t j = bbase + wsum[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
     if (keep) {
-        const uint32_t j = bbase + wsum[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
         const PreRay r = prepare_ray(o, d);
         qa[j] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(id));
         qb[j] = make_float4(r.inv.x, r.inv.y, r.inv.z, __uint_as_float(pack_signs(r, tag)));
     }
     __syncthreads();
+    return j;
 }
 
 /* Workgroup-aggregated reservation of one entry of a counter for every
@@ -503,8 +504,9 @@ __global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceA
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         bool keep = false;
-        uint32_t sid = 0;
+        uint32_t sid = 0, gpix = 0, gtag = 0;
         f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
+        PathState G;
         if (i < n) {
             uint64_t gi = g0 + i;
             if (BATCH) {
@@ -524,7 +526,6 @@ __global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceA
                 C.vrec_n[tag - 1u] = 0u;
             }
             if (start_path<STATS>(a, px, py, rf, S, cnt)) {
-                store_generated(P, i, S, pix, k, tag);
                 keep = true;
                 ro = S.ray_o;
                 rd = S.ray_d;
@@ -532,8 +533,12 @@ __global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceA
                 const f3 c = tonemap(a.s, S.radiance);
                 P.smp[i] = make_float4(c.x, c.y, c.z, 1.0f);
             }
+            G = S;
+            gpix = pix;
+            gtag = tag;
         }
-        block_append<WF_GENERATE_THREADS>(keep, i, sid, ro, rd, P.qa0, P.qb0, &P.cnt[0]);
+        const uint32_t q = block_append<WF_GENERATE_THREADS>(keep, i, sid, ro, rd, P.qa0, P.qb0, &P.cnt[0]);
+        if (keep) store_generated(P.s[0], q, G, gpix, i, gtag);
     }
     flush_counters<STATS>(BATCH ? B.stats : a0.stats, cnt);
 }
@@ -692,33 +697,34 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0
     float* cdfv0 = nullptr;
     const uint32_t* bits0 = nullptr;
     if (!BATCH) stage_lds<false>(a0, smem, recs0, cdfv0, bits0);
-    const float4* qa = sel ? P.qa1 : P.qa0;
     float4* qao = sel ? P.qa0 : P.qa1;
     float4* qbo = sel ? P.qb0 : P.qb1;
+    const PathBuf& Bc = P.s[sel];
+    const PathBuf& Bn = P.s[sel ^ 1];
     const uint32_t n = P.cnt[sel];
     if (blockIdx.x == 0 && threadIdx.x == 0) P.cnt[2 + (sel ^ 1)] = 0u;
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t j = base + threadIdx.x;
         bool keep = false, query = false;
-        uint32_t id = 0, sid = 0;
+        uint32_t sid = 0;
         f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
         float vraw[12];
+        PathState S;
+        uint4 me = make_uint4(0u, 0u, 0u, 0u);
         if (j < n) {
-            id = __float_as_uint(qa[j].w);
-            if (BATCH) sid = P.meta[id].w;
+            load_phase<PHASE>(Bc, j, S, me);
+            if (BATCH) sid = me.w;
             const TraceArgs& a = BATCH ? B.tab[sid] : a0;
             const MatRec* recs = BATCH ? a.recs : recs0;
             const float* cdfv = BATCH ? a.e.cdfv : cdfv0;
             const uint32_t* bits = BATCH ? a.g.bits : bits0;
-            PathState S;
-            load_phase<PHASE>(P, id, S);
             const int4 hv = P.hit[j];
             Hit h;
             h.x = hv.x; h.y = hv.y; h.z = hv.z; h.slot = (uint32_t)hv.w & 0xffu;
             h.steps = 0;
             const int kind = hv.w >> 8;
-            const uint32_t slot1 = CACHE ? P.meta[id].w : 0u;   /* training slot + 1 */
+            const uint32_t slot1 = CACHE ? me.w : 0u;   /* training slot + 1 */
             bool vrec = false;
             const int st = shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, kind, cnt,
                                                     slot1 ? C.train_bounce : C.cache_bounce, vraw, &vrec);
@@ -737,15 +743,11 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0
             keep = st == 1;
             query = CACHE && st == 2;
             if (keep) {
-                store_phase<PHASE>(P, id, S);
                 ro = S.ray_o;
                 rd = S.ray_d;
-            } else if (query) {
-                P.rad[id] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
-                P.thr[id] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
-            } else {
+            } else if (!query) {
                 const f3 c = tonemap(a.s, S.radiance);
-                P.smp[id] = make_float4(c.x, c.y, c.z, 1.0f);
+                P.smp[me.y] = make_float4(c.x, c.y, c.z, 1.0f);
                 if (CACHE && slot1) emit_training(C, slot1 - 1u, S.radiance);
             }
         }
@@ -753,10 +755,13 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0
             const uint32_t qi = block_reserve<WF_SHADE_THREADS>(query, C.q_count);
             if (query) {
                 for (int k = 0; k < 12; ++k) C.q_raw[(size_t)qi * 12u + k] = vraw[k];
-                C.q_id[qi] = id;
+                C.q_meta[qi] = me;
+                C.q_rad[qi] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
+                C.q_thr[qi] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
             }
         }
-        block_append<WF_SHADE_THREADS>(keep, id, sid, ro, rd, qao, qbo, &P.cnt[sel ^ 1]);
+        const uint32_t k = block_append<WF_SHADE_THREADS>(keep, me.y, sid, ro, rd, qao, qbo, &P.cnt[sel ^ 1]);
+        if (keep) store_phase<PHASE>(Bn, k, S, me);
     }
     flush_counters<STATS>(BATCH ? B.stats : a0.stats, cnt);
 }
@@ -903,16 +908,15 @@ __global__ __launch_bounds__(256) void cache_apply_kernel(TraceArgs a, WfPaths P
 {
     const uint32_t lo = *C.q_done, n = *C.q_count;
     for (uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t id = C.q_id[i];
-        const float4 r = P.rad[id], t = P.thr[id];
+        const uint4 me = C.q_meta[i];
+        const float4 r = C.q_rad[i], t = C.q_thr[i];
         const float y0 = C.q_out[(size_t)i * 3u], y1 = C.q_out[(size_t)i * 3u + 1], y2 = C.q_out[(size_t)i * 3u + 2];
         /* a channel the cache predicts as 0 (or less) adds nothing */
         const f3 rad = mk(y0 > 0.0f ? r.x + t.x * y0 : r.x, y1 > 0.0f ? r.y + t.y * y1 : r.y,
                           y2 > 0.0f ? r.z + t.z * y2 : r.z);
         const f3 c = tonemap(a.s, rad);
-        P.smp[id] = make_float4(c.x, c.y, c.z, 1.0f);
-        const uint32_t slot1 = P.meta[id].w;
-        if (slot1) emit_training(C, slot1 - 1u, rad);
+        P.smp[me.y] = make_float4(c.x, c.y, c.z, 1.0f);
+        if (me.w) emit_training(C, me.w - 1u, rad);
     }
 }
 
