@@ -276,8 +276,13 @@ struct DdaState {
                 const uint32_t bk = block_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
                 mask = ((bits[bk >> 5] >> (bk & 31u)) & 1u) ? g.occ64[bk] : 0ull;
             }
-            const uint32_t bit = ((uint32_t)x & 3u) | (((uint32_t)y & 3u) << 2) | (((uint32_t)z & 3u) << 4);
-            const bool hit = inside && ((mask >> bit) & 1ull);
+            /* the voxel test only when some lane's block has an occupied voxel
+             * (a wave-uniform branch: rays in empty space skip it) */
+            bool hit = false;
+            if (__ballot(!done && inside && mask != 0ull) != 0ull) {
+                const uint32_t bit = ((uint32_t)x & 3u) | (((uint32_t)y & 3u) << 2) | (((uint32_t)z & 3u) << 4);
+                hit = inside && ((mask >> bit) & 1ull);
+            }
             if (COUNT && !done && more && inside) visits++;
             done = done || !more || !inside || hit;
             /* stopped lanes step no axis, so the step can run before the
