@@ -130,9 +130,18 @@ __device__ __forceinline__ uint32_t local_morton(uint32_t x, uint32_t y, uint32_
 }
 /* Linear index of the 4x4x4 block holding in-grid voxel (x, y, z); fits
  * 24-bit multiplies since block extents are at most 1024. */
+/* a * b + c on 24-bit operands as one full-rate instruction (the compiler
+ * otherwise may pick a 64-bit multiply-add) */
+__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 __device__ __forceinline__ uint32_t block_index(const GridView& g, uint32_t x, uint32_t y, uint32_t z)
 {
-    return (x >> 2) + __umul24(g.bx4, (y >> 2) + __umul24(g.by4, z >> 2));
+    return mad24(mad24(z >> 2, g.by4, y >> 2), g.bx4, x >> 2);
 }
 /* Bit of voxel (x, y, z) in its 4x4x4 block mask. */
 __device__ __forceinline__ uint32_t block_bit(uint32_t x, uint32_t y, uint32_t z)
@@ -175,11 +184,6 @@ __device__ __forceinline__ uint32_t block_key(int32_t x, int32_t y, int32_t z)
     return ((uint32_t)x >> 2) | (((uint32_t)y >> 2) << 10) | (((uint32_t)z >> 2) << 20);
 }
 
-/* The march as resumable state, for traversal kernels whose lanes take a
- * new ray as soon as theirs finishes.  init() returns false when the start
- * point lies outside the grid (steps = -1, nothing else written); advance()
- * runs at most `budget` steps and returns 0 while the march goes on, 1 on a
- * hit, 2 when it left the grid or ran out of steps. */
 /* The grid-independent part of raymarch's set-up (dda.h:7-35): origin
  * moved by sign(d)*0.001, direction clamped by mix(d, 1e-5, step(|d|, 1e-5)),
  * its reciprocal and sign.  Computed where a ray is made, so a traversal
@@ -223,11 +227,16 @@ __device__ __forceinline__ PreRay unpack_ray(float4 qa, float4 qb)
 }
 __device__ __forceinline__ uint32_t queue_tag(float4 qb) { return __float_as_uint(qb.w) & 0x03ffffffu; }
 
+/* The march as resumable state, for traversal kernels whose lanes take a
+ * new ray as soon as theirs finishes.  init() returns false when the start
+ * point lies outside the grid (steps = -1, nothing else written); advance()
+ * runs at most `budget` steps and returns 0 while the march goes on, 1 on a
+ * hit, 2 when it left the grid or ran out of steps. */
 struct DdaState {
     int32_t x, y, z, steps;
     float disx, disy, disz, ax, ay, az;
     int32_t stx, sty, stz;
-    uint32_t key;               /* block key of `mask` */
+    uint32_t key;               /* linear index of the block of `mask` */
     uint64_t mask;
 
     __device__ __forceinline__ bool init(const GridView& g, f3 o, f3 d) { return init(g, prepare_ray(o, d)); }
@@ -266,20 +275,24 @@ struct DdaState {
     {
         const int32_t limit = budget < g.max_steps - steps ? steps + budget : g.max_steps;
         bool done = false;
+        /* block-grid strides held in vector registers across the loop */
+        uint32_t bx4 = g.bx4, by4 = g.by4;
+        asm volatile("" : "+v"(bx4), "+v"(by4));
         for (;;) {
             const bool inside = (uint32_t)x < (uint32_t)g.nx && (uint32_t)y < (uint32_t)g.ny &&
                                 (uint32_t)z < (uint32_t)g.nz;
             const bool more = steps < limit;
-            const uint32_t k = block_key(x, y, z);
-            if (inside && k != key) {
-                key = k;
-                const uint32_t bk = block_index(g, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+            /* the block's linear index doubles as its key (meaningful for
+             * in-grid voxels only, the only ones that reload) */
+            const uint32_t bk = mad24(mad24((uint32_t)z >> 2, by4, (uint32_t)y >> 2), bx4, (uint32_t)x >> 2);
+            if (inside && bk != key) {
+                key = bk;
                 mask = ((bits[bk >> 5] >> (bk & 31u)) & 1u) ? g.occ64[bk] : 0ull;
             }
             /* the voxel test only when some lane's block has an occupied voxel
              * (a wave-uniform branch: rays in empty space skip it) */
             bool hit = false;
-            if (__ballot(!done && inside && mask != 0ull) != 0ull) {
+            if (__builtin_amdgcn_ballot_w64(!done && inside && mask != 0ull) != 0ull) {
                 const uint32_t bit = ((uint32_t)x & 3u) | (((uint32_t)y & 3u) << 2) | (((uint32_t)z & 3u) << 4);
                 hit = inside && ((mask >> bit) & 1ull);
             }
@@ -298,7 +311,7 @@ struct DdaState {
             y += my ? sty : 0;
             z += mz ? stz : 0;
             steps += done ? 0 : 1;
-            if (__ballot(!done) == 0ull) break;
+            if (__builtin_amdgcn_ballot_w64(!done) == 0ull) break;
         }
         /* every lane holds the state it stopped in */
         if (steps >= limit) return steps >= g.max_steps ? 2 : 0;
