@@ -478,7 +478,7 @@ __device__ __forceinline__ uint32_t block_reserve(bool want, uint32_t* counter)
 }
 
 #define WF_GENERATE_THREADS 1024
-#define WF_SHADE_THREADS 512
+#define WF_SHADE_THREADS 256
 
 /* Scene of global path index gi in a batch: base[s] <= gi < base[s + 1]. */
 __device__ __forceinline__ uint32_t scene_of(const uint64_t* __restrict__ base, uint32_t n, uint64_t gi)
@@ -689,7 +689,7 @@ __device__ __forceinline__ void emit_training(const CacheRender& C, uint32_t slo
 }
 
 template <bool STATS, int PHASE, bool BATCH, bool CACHE = false>
-__global__ __launch_bounds__(WF_SHADE_THREADS) void wf_shade_kernel(TraceArgs a0, BatchTable B, WfPaths P, int sel,
+__global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void wf_shade_kernel(TraceArgs a0, BatchTable B, WfPaths P, int sel,
                                                                     CacheRender C = CacheRender())
 {
     extern __shared__ __align__(16) uint8_t smem[];
