@@ -350,10 +350,11 @@ int alloc_paths(vt::WfPaths& P, uint64_t& alloc, uint64_t slots)
         VT_HIP(hipMalloc(&B.rad, n * 16)); VT_HIP(hipMalloc(&B.dir, n * 16)); VT_HIP(hipMalloc(&B.fac, n * 16));
         VT_HIP(hipMalloc(&B.tgt, n * 16)); VT_HIP(hipMalloc(&B.meta, n * 16)); VT_HIP(hipMalloc(&B.rng, n * 8));
     }
-    VT_HIP(hipMalloc(&P.hit, n * 16));
+    /* queues and hits: two entries (shadow and continuation ray) per path */
+    VT_HIP(hipMalloc(&P.hit, 2 * n * 16));
     VT_HIP(hipMalloc(&P.smp, n * 16));
-    VT_HIP(hipMalloc(&P.qa0, n * 16)); VT_HIP(hipMalloc(&P.qb0, n * 16));
-    VT_HIP(hipMalloc(&P.qa1, n * 16)); VT_HIP(hipMalloc(&P.qb1, n * 16));
+    VT_HIP(hipMalloc(&P.qa0, 2 * n * 16)); VT_HIP(hipMalloc(&P.qb0, 2 * n * 16));
+    VT_HIP(hipMalloc(&P.qa1, 2 * n * 16)); VT_HIP(hipMalloc(&P.qb1, 2 * n * 16));
     VT_HIP(hipMalloc(&P.cnt, 16));
     P.cap = slots;
     alloc = slots;
@@ -716,7 +717,7 @@ int vt_set_render_mode(vt_context* c, int32_t mode, uint64_t max_paths)
 {
     if (!c) return fail(VT_ERR_ARG, "null context");
     if (mode != 0 && mode != 1) return fail(VT_ERR_ARG, "mode must be 0 (megakernel) or 1 (wavefront)");
-    if (max_paths > (1ull << 31)) return fail(VT_ERR_LIMIT, "max_paths above 2^31");
+    if (max_paths > (1ull << 30)) return fail(VT_ERR_LIMIT, "max_paths above 2^30");
     c->render_mode = mode;
     c->max_paths = max_paths ? max_paths : VT_DEFAULT_MAX_PATHS;
     return VT_OK;

@@ -33,7 +33,7 @@ struct Counters {
     uint32_t rays, visits, interactions, samples, env_samples, env_evals;
 };
 
-enum { PH_PRIMARY = 0, PH_SHADOW = 1, PH_CONT = 2 };
+enum { PH_PRIMARY = 0, PH_SHADOW = 1, PH_CONT = 2, PH_COMB = 3 /* shade round: shadow result, then CONT */ };
 
 /* Registers of one path between traversals. */
 struct PathState {
@@ -70,9 +70,10 @@ struct PathBuf {
 
 struct WfPaths {
     PathBuf s[2];     /* state of the paths in queue 0 / queue 1 */
-    int4* hit;        /* traversal result per queue position: x, y, z, kind << 8 | slot */
+    int4* hit;        /* traversal result per queue entry: x, y, z, kind << 8 | slot */
     float4* smp;      /* finished tonemapped sample of each path of the chunk */
-    float4* qa0;      /* ray queue 0: origin.xyz moved by sign(d)*0.001, sample slot (as bits) */
+    float4* qa0;      /* ray queue 0 (2 x cap entries: a path's shadow and continuation rays at 2k, 2k + 1):
+                         origin.xyz moved by sign(d)*0.001, sample slot (as bits) */
     float4* qb0;      /*              1/clamped direction, tag | direction signs << 26 (bits) */
     float4* qa1;      /* ray queue 1 */
     float4* qb1;
@@ -134,7 +135,8 @@ struct LaunchTimer {
 };
 
 /* Wavefront render of the launch in a: path chunks of at most P.cap slots,
- * 1 + 2*max_bounces traverse/shade rounds per chunk.  With acc_wait, every
+ * 1 + max_bounces traverse/shade rounds per chunk (each round after the
+ * first traces a shadow and a continuation ray per path).  With acc_wait, every
  * accumulation launch first waits for that event (passes accumulate in
  * order when two launches render consecutive passes on two streams). */
 hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats,

@@ -311,8 +311,9 @@ __global__ __launch_bounds__(256) void render_kernel(TraceArgs a)
 }
 
 /* ---- wavefront form ---------------------------------------------------------------
- * generate -> (traverse -> shade) x (1 + 2*max_bounces) -> accumulate, per
- * chunk of paths.  The traversal kernel only runs the DDA, so it needs few
+ * generate -> (traverse -> shade) x (1 + max_bounces) -> accumulate, per
+ * chunk of paths; after the camera rays, each round traces every path's
+ * shadow ray and continuation ray together.  The traversal kernel only runs the DDA, so it needs few
  * registers and keeps many waves resident; shading consumes the hits and
  * appends surviving paths to the next queue with one atomic per wave. */
 
@@ -452,6 +453,37 @@ __device__ __forceinline__ uint32_t block_append(bool keep, uint32_t id, uint32_
     }
     __syncthreads();
     return j;
+}
+
+/* The same with two rays per thread with `keep`, sharing the origin o (a
+ * shadow ray along d0 and a continuation ray along d1): they take queue
+ * entries 2k and 2k + 1, k being the returned position. */
+template <int NT>
+__device__ __forceinline__ uint32_t block_append2(bool keep, uint32_t id, uint32_t tag, f3 o, f3 d0, f3 d1,
+                                                  float4* qa, float4* qb, uint32_t* n)
+{
+    __shared__ uint32_t wsum2[NT / 64];
+    __shared__ uint32_t bbase2;
+    const unsigned long long mask = __ballot(keep);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) wsum2[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < NT / 64; ++w) { const uint32_t c = wsum2[w]; wsum2[w] = t; t += c; }
+        bbase2 = t ? atomicAdd(n, 2u * t) / 2u : 0u;
+    }
+    __syncthreads();
+    const uint32_t k = bbase2 + wsum2[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (keep) {
+        const PreRay r0 = prepare_ray(o, d0), r1 = prepare_ray(o, d1);
+        qa[2 * k] = make_float4(r0.o.x, r0.o.y, r0.o.z, __uint_as_float(id));
+        qb[2 * k] = make_float4(r0.inv.x, r0.inv.y, r0.inv.z, __uint_as_float(pack_signs(r0, tag)));
+        qa[2 * k + 1] = make_float4(r1.o.x, r1.o.y, r1.o.z, __uint_as_float(id));
+        qb[2 * k + 1] = make_float4(r1.inv.x, r1.inv.y, r1.inv.z, __uint_as_float(pack_signs(r1, tag)));
+    }
+    __syncthreads();
+    return k;
 }
 
 /* Workgroup-aggregated reservation of one entry of a counter for every
@@ -688,9 +720,15 @@ __device__ __forceinline__ void emit_training(const CacheRender& C, uint32_t slo
     }
 }
 
+/* Shading of one round.  PHASE PH_PRIMARY: the camera rays' hits (one
+ * queue entry per path).  PH_COMB: every path's shadow ray (entry 2j) and
+ * continuation ray (entry 2j + 1), traced in the same traversal launch:
+ * the shadow result is applied first, then the continuation hit is shaded,
+ * exactly the order of the reference's loop.  A path that goes on appends
+ * its next shadow and continuation rays, which leave from the same point. */
 template <bool STATS, int PHASE, bool BATCH, bool CACHE = false>
-__global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void wf_shade_kernel(TraceArgs a0, BatchTable B, WfPaths P, int sel,
-                                                                    CacheRender C = CacheRender())
+__global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void wf_shade_kernel(
+    TraceArgs a0, BatchTable B, WfPaths P, int sel, CacheRender C = CacheRender())
 {
     extern __shared__ __align__(16) uint8_t smem[];
     MatRec* recs0 = nullptr;
@@ -701,30 +739,36 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
     float4* qbo = sel ? P.qb0 : P.qb1;
     const PathBuf& Bc = P.s[sel];
     const PathBuf& Bn = P.s[sel ^ 1];
-    const uint32_t n = P.cnt[sel];
+    const uint32_t n = PHASE == PH_PRIMARY ? P.cnt[sel] : P.cnt[sel] / 2u;
     if (blockIdx.x == 0 && threadIdx.x == 0) P.cnt[2 + (sel ^ 1)] = 0u;
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t j = base + threadIdx.x;
         bool keep = false, query = false;
         uint32_t sid = 0;
-        f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
+        f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, rn = ro;
         float vraw[12];
         PathState S;
         uint4 me = make_uint4(0u, 0u, 0u, 0u);
         if (j < n) {
-            load_phase<PHASE>(Bc, j, S, me);
+            load_phase<PHASE == PH_PRIMARY ? PH_PRIMARY : PH_SHADOW>(Bc, j, S, me);
             if (BATCH) sid = me.w;
             const TraceArgs& a = BATCH ? B.tab[sid] : a0;
             const MatRec* recs = BATCH ? a.recs : recs0;
             const float* cdfv = BATCH ? a.e.cdfv : cdfv0;
             const uint32_t* bits = BATCH ? a.g.bits : bits0;
-            const int4 hv = P.hit[j];
-            Hit h;
-            h.x = hv.x; h.y = hv.y; h.z = hv.z; h.slot = (uint32_t)hv.w & 0xffu;
-            h.steps = 0;
-            const int kind = hv.w >> 8;
             const uint32_t slot1 = CACHE ? me.w : 0u;   /* training slot + 1 */
+            Hit h;
+            h.steps = 0;
+            if (PHASE != PH_PRIMARY) {
+                /* the shadow ray's result (never ends the path) */
+                const int4 hs = P.hit[2 * j];
+                h.x = hs.x; h.y = hs.y; h.z = hs.z; h.slot = (uint32_t)hs.w & 0xffu;
+                shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, hs.w >> 8, cnt, -1, vraw, nullptr);
+            }
+            const int4 hv = P.hit[PHASE == PH_PRIMARY ? j : 2 * j + 1];
+            h.x = hv.x; h.y = hv.y; h.z = hv.z; h.slot = (uint32_t)hv.w & 0xffu;
+            const int kind = hv.w >> 8;
             bool vrec = false;
             const int st = shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, kind, cnt,
                                                     slot1 ? C.train_bounce : C.cache_bounce, vraw, &vrec);
@@ -744,7 +788,8 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
             query = CACHE && st == 2;
             if (keep) {
                 ro = S.ray_o;
-                rd = S.ray_d;
+                rd = S.ray_d;       /* shadow ray */
+                rn = S.next_d;      /* continuation ray */
             } else if (!query) {
                 const f3 c = tonemap(a.s, S.radiance);
                 P.smp[me.y] = make_float4(c.x, c.y, c.z, 1.0f);
@@ -760,8 +805,8 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
                 C.q_thr[qi] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
             }
         }
-        const uint32_t k = block_append<WF_SHADE_THREADS>(keep, me.y, sid, ro, rd, qao, qbo, &P.cnt[sel ^ 1]);
-        if (keep) store_phase<PHASE>(Bn, k, S, me);
+        const uint32_t k = block_append2<WF_SHADE_THREADS>(keep, me.y, sid, ro, rd, rn, qao, qbo, &P.cnt[sel ^ 1]);
+        if (keep) store_phase<PH_CONT>(Bn, k, S, me);
     }
     flush_counters<STATS>(BATCH ? B.stats : a0.stats, cnt);
 }
@@ -838,8 +883,7 @@ static hipError_t wf_run(const TraceArgs& a, const BatchTable& B, const WfPaths&
     const size_t lds_s = BATCH ? 0 : 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
     const int bt = blocks_per_cu(wf_traverse_kernel<STATS, LDSB, BATCH>, WF_TRAVERSE_THREADS, lds_t);
     const int bs0 = blocks_per_cu(wf_shade_kernel<STATS, PH_PRIMARY, BATCH>, WF_SHADE_THREADS, lds_s);
-    const int bs1 = blocks_per_cu(wf_shade_kernel<STATS, PH_SHADOW, BATCH>, WF_SHADE_THREADS, lds_s);
-    const int bs2 = blocks_per_cu(wf_shade_kernel<STATS, PH_CONT, BATCH>, WF_SHADE_THREADS, lds_s);
+    const int bs1 = blocks_per_cu(wf_shade_kernel<STATS, PH_COMB, BATCH>, WF_SHADE_THREADS, lds_s);
     for (uint64_t g0 = 0; g0 < total; g0 += P.cap) {
         const uint64_t g1 = g0 + P.cap < total ? g0 + P.cap : total;
         const uint32_t n = (uint32_t)(g1 - g0);
@@ -862,10 +906,8 @@ static hipError_t wf_run(const TraceArgs& a, const BatchTable& B, const WfPaths&
             }
             if (r == 0)
                 wf_shade_kernel<STATS, PH_PRIMARY, BATCH><<<cus * bs0, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel);
-            else if (r & 1)
-                wf_shade_kernel<STATS, PH_SHADOW, BATCH><<<cus * bs1, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel);
             else
-                wf_shade_kernel<STATS, PH_CONT, BATCH><<<cus * bs2, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel);
+                wf_shade_kernel<STATS, PH_COMB, BATCH><<<cus * bs1, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel);
             sel ^= 1;
         }
         const uint64_t gp = (n_pixels + 255) / 256;
@@ -885,7 +927,7 @@ hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus
     const BatchTable B = { nullptr, nullptr, nullptr, 0u, nullptr };
     const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
     const uint64_t total = WH * a.count;
-    const int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
+    const int rounds = 1 + (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
     if (stats)
         return a.lds_bits ? wf_run<true, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
                           : wf_run<true, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
@@ -896,7 +938,7 @@ hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus
 hipError_t launch_render_batch(const TraceArgs& a0, const BatchTable& B, const WfPaths& P, uint64_t total_paths,
                                uint64_t total_pixels, int max_bounces, int cus, hipStream_t s, bool stats)
 {
-    const int rounds = 1 + 2 * (max_bounces > 0 ? max_bounces : 0);
+    const int rounds = 1 + (max_bounces > 0 ? max_bounces : 0);
     if (stats) return wf_run<true, false, true>(a0, B, P, total_paths, total_pixels, rounds, cus, s);
     return wf_run<false, false, true>(a0, B, P, total_paths, total_pixels, rounds, cus, s);
 }
@@ -934,17 +976,16 @@ static hipError_t wf_run_cached(const TraceArgs& a, const WfPaths& P, const Cach
     const size_t lds_s = 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
     const int bt = blocks_per_cu(wf_traverse_kernel<false, LDSB, false>, WF_TRAVERSE_THREADS, lds_t);
     const int bs0 = blocks_per_cu(wf_shade_kernel<false, PH_PRIMARY, false, true>, WF_SHADE_THREADS, lds_s);
-    const int bs1 = blocks_per_cu(wf_shade_kernel<false, PH_SHADOW, false, true>, WF_SHADE_THREADS, lds_s);
-    const int bs2 = blocks_per_cu(wf_shade_kernel<false, PH_CONT, false, true>, WF_SHADE_THREADS, lds_s);
+    const int bs1 = blocks_per_cu(wf_shade_kernel<false, PH_COMB, false, true>, WF_SHADE_THREADS, lds_s);
     const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
     if (WH > P.cap) return hipErrorInvalidValue;
     const uint32_t n = (uint32_t)WH;
-    int rounds = 1 + 2 * (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
-    const int qround = C.cache_bounce >= 0 ? 2 * C.cache_bounce : -1;
+    int rounds = 1 + (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
+    const int qround = C.cache_bounce >= 0 ? C.cache_bounce : -1;
     /* without training paths nothing goes past the cache bounce */
-    const int tround = C.train_slots == 0 ? qround : (C.train_bounce >= 0 ? 2 * C.train_bounce : -1);
-    /* every path still alive at round 2b is at bounce b, so once the plain
-     * and the training paths have ended into the cache no round is left */
+    const int tround = C.train_slots == 0 ? qround : (C.train_bounce >= 0 ? C.train_bounce : -1);
+    /* round b shades the vertices of bounce b, so once the plain and the
+     * training paths have ended into the cache no round is left */
     if (qround >= 0 && tround >= 0) rounds = std::min(rounds, 1 + std::max(qround, tround));
     wf_reset_kernel<<<1, 64, 0, s>>>(P.cnt);
     const uint32_t gn = (n + WF_GENERATE_THREADS - 1) / WF_GENERATE_THREADS;
@@ -955,10 +996,8 @@ static hipError_t wf_run_cached(const TraceArgs& a, const WfPaths& P, const Cach
         wf_traverse_kernel<false, LDSB, false><<<cus * bt, WF_TRAVERSE_THREADS, lds_t, s>>>(a, B, P, sel);
         if (r == 0)
             wf_shade_kernel<false, PH_PRIMARY, false, true><<<cus * bs0, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel, C);
-        else if (r & 1)
-            wf_shade_kernel<false, PH_SHADOW, false, true><<<cus * bs1, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel, C);
         else
-            wf_shade_kernel<false, PH_CONT, false, true><<<cus * bs2, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel, C);
+            wf_shade_kernel<false, PH_COMB, false, true><<<cus * bs1, WF_SHADE_THREADS, lds_s, s>>>(a, B, P, sel, C);
         if (r == qround || r == tround) {
             hipError_t e = launch_mlp_infer_dev((const _Float16*)pack, bias, C.q_raw, C.q_out, C.q_done, C.q_count, n,
                                                 cus, s);
