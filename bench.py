@@ -31,11 +31,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s; time-to-RMSE 1e-2 vs FP64 ref, plain vs batched+radiance-cache"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the dominant kernel on the full workload (one and
-# two streams), from rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000),
-# each factor being what the counter reads for a copy kernel of known size
+# HBM bytes per launch of the dominant kernel on the full workload (one
+# stream), from rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), each
+# factor being what the counter reads for a copy kernel of known size
 # (profiles/wf_traverse_kernel_full_1024x1024_16spp.md).
-TRAFFIC: dict = {("wf_traverse_kernel", 1): 340897464, ("wf_traverse_kernel", 2): 171144253}
+TRAFFIC: dict = {("wf_traverse_kernel", 1): 656747789}
 
 
 def env_lookup_bytes(cu_w: int, cv_n: int) -> tuple[int, int]:
