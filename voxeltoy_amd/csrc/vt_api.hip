@@ -100,6 +100,7 @@ struct vt_context {
     /* environment */
     vt_environment env{};
     float* d_img = nullptr;
+    float4* d_quad = nullptr;       /* bilinear quads of the image */
     float* d_cdfu = nullptr;
     float* d_cdfv = nullptr;
 
@@ -289,7 +290,7 @@ vt::TraceArgs trace_args(const vt_context* c)
     a.g = grid_view(c);
     a.e.use_image = c->env.use_image;
     for (int k = 0; k < 3; ++k) { a.e.top[k] = c->env.top[k]; a.e.bottom[k] = c->env.bottom[k]; }
-    a.e.w = c->env.image_w; a.e.h = c->env.image_h; a.e.img = c->d_img;
+    a.e.w = c->env.image_w; a.e.h = c->env.image_h; a.e.img = c->d_img; a.e.quad = c->d_quad;
     a.e.cu_w = c->env.cdf_u_w; a.e.cu_h = c->env.cdf_u_h; a.e.cdfu = c->d_cdfu;
     a.e.cv_n = c->env.cdf_v_n; a.e.cdfv = c->d_cdfv;
     a.e.integral = c->env.integral; a.e.rotation = c->env.rotation_radians;
@@ -414,7 +415,7 @@ void vt_destroy(vt_context* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_bricks); dfree(c->d_occ64); dfree(c->d_bits); dfree(c->d_emissive); dfree(c->d_recs);
-    dfree(c->d_img); dfree(c->d_cdfu); dfree(c->d_cdfv);
+    dfree(c->d_img); dfree(c->d_quad); dfree(c->d_cdfu); dfree(c->d_cdfv);
     dfree(c->d_accum); dfree(c->d_sum64); dfree(c->d_tile); dfree(c->d_stats);
     free_wavefront(c);
     free_paths(c->wf2, c->wf2_alloc);
@@ -589,7 +590,7 @@ int vt_set_environment(vt_context* c, const vt_environment* env)
     if (!c || !env) return fail(VT_ERR_ARG, "null argument");
     VT_HIP(hipSetDevice(c->device));
     vt_environment e = *env;
-    dfree(c->d_img); dfree(c->d_cdfu); dfree(c->d_cdfv);
+    dfree(c->d_img); dfree(c->d_quad); dfree(c->d_cdfu); dfree(c->d_cdfv);
     if (e.use_image) {
         if (e.image_w <= 0 || e.image_h <= 0 || !e.image_rgb || e.cdf_u_w < 2 || e.cdf_u_h < 1 || !e.cdf_u ||
             e.cdf_v_n < 2 || !e.cdf_v)
@@ -603,6 +604,8 @@ int vt_set_environment(vt_context* c, const vt_environment* env)
         VT_HIP(hipMemcpyAsync(c->d_img, e.image_rgb, ni * 4, hipMemcpyHostToDevice, c->stream));
         VT_HIP(hipMemcpyAsync(c->d_cdfu, e.cdf_u, nu * 4, hipMemcpyHostToDevice, c->stream));
         VT_HIP(hipMemcpyAsync(c->d_cdfv, e.cdf_v, (size_t)e.cdf_v_n * 4, hipMemcpyHostToDevice, c->stream));
+        VT_HIP(hipMalloc(&c->d_quad, (size_t)(e.image_w + 1) * (size_t)(e.image_h + 1) * 64));
+        VT_HIP(vt::launch_env_quads(c->d_img, e.image_w, e.image_h, c->d_quad, c->stream));
         VT_HIP(hipStreamSynchronize(c->stream));
     }
     e.image_rgb = nullptr; e.cdf_u = nullptr; e.cdf_v = nullptr;

@@ -84,6 +84,10 @@ struct EnvView {
     float top[3], bottom[3];
     int32_t w, h;
     const float* img;           /* RGB */
+    /* bilinear quads: entry (j0 + 1) * (w + 1) + (i0 + 1) holds texels
+     * (i0, j0), (i0 + 1, j0), (i0, j0 + 1), (i0 + 1, j0 + 1) as 12 floats
+     * (zero outside the image, the GL_CLAMP border) in one 64-byte line */
+    const float4* quad;
     int32_t cu_w, cu_h;
     const float* cdfu;
     int32_t cv_n;
@@ -463,17 +467,14 @@ __device__ __forceinline__ f3 tex_bilinear(const EnvView& e, float u, float v)
     const int32_t i0 = (int32_t)fs, j0 = (int32_t)ft;
     const float a = s - fs, b = t - ft;
     const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
-    const bool i0ok = i0 >= 0 && i0 < e.w, i1ok = i0 + 1 >= 0 && i0 + 1 < e.w;
-    const bool j0ok = j0 >= 0 && j0 < e.h, j1ok = j0 + 1 >= 0 && j0 + 1 < e.h;
+    /* i0, j0 lie in [-1, w - 1] x [-1, h - 1]: one quad holds the four texels */
+    const float4* q = e.quad + ((size_t)(j0 + 1) * (size_t)(e.w + 1) + (size_t)(i0 + 1)) * 4u;
+    const float4 q0 = q[0], q1 = q[1], q2 = q[2];
+    const float t00[3] = { q0.x, q0.y, q0.z }, t10[3] = { q0.w, q1.x, q1.y };
+    const float t01[3] = { q1.z, q1.w, q2.x }, t11[3] = { q2.y, q2.z, q2.w };
     float out[3];
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        const float t00 = (i0ok && j0ok) ? e.img[((size_t)j0 * e.w + i0) * 3 + ch] : 0.0f;
-        const float t10 = (i1ok && j0ok) ? e.img[((size_t)j0 * e.w + i0 + 1) * 3 + ch] : 0.0f;
-        const float t01 = (i0ok && j1ok) ? e.img[((size_t)(j0 + 1) * e.w + i0) * 3 + ch] : 0.0f;
-        const float t11 = (i1ok && j1ok) ? e.img[((size_t)(j0 + 1) * e.w + i0 + 1) * 3 + ch] : 0.0f;
-        out[ch] = ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11;
-    }
+    for (int ch = 0; ch < 3; ++ch) out[ch] = ((w00 * t00[ch] + w10 * t10[ch]) + w01 * t01[ch]) + w11 * t11[ch];
     return mk(out[0], out[1], out[2]);
 }
 

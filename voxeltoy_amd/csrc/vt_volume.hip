@@ -319,6 +319,38 @@ __global__ __launch_bounds__(256) void voxelize_kernel(VoxArgs a)
     }
 }
 
+/* Bilinear quad table of an RGB image (EnvView::quad): one 64-byte entry per
+ * (i0 + 1, j0 + 1) in [0, w] x [0, h], texels outside the image zero. */
+__global__ __launch_bounds__(256) void env_quads_kernel(const float* __restrict__ img, int32_t w, int32_t h,
+                                                        float4* __restrict__ quads)
+{
+    const uint64_t n = (uint64_t)(w + 1) * (uint64_t)(h + 1);
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+        const int32_t i0 = (int32_t)(e % (uint64_t)(w + 1)) - 1, j0 = (int32_t)(e / (uint64_t)(w + 1)) - 1;
+        float t[12];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int32_t i = i0 + (k & 1), j = j0 + (k >> 1);
+            const bool ok = i >= 0 && i < w && j >= 0 && j < h;
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) t[3 * k + ch] = ok ? img[((size_t)j * w + i) * 3 + ch] : 0.0f;
+        }
+        float4* q = quads + e * 4u;
+        q[0] = make_float4(t[0], t[1], t[2], t[3]);
+        q[1] = make_float4(t[4], t[5], t[6], t[7]);
+        q[2] = make_float4(t[8], t[9], t[10], t[11]);
+        q[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+hipError_t launch_env_quads(const float* img, int32_t w, int32_t h, float4* quads, hipStream_t s)
+{
+    const uint64_t n = (uint64_t)(w + 1) * (uint64_t)(h + 1);
+    const uint64_t g = (n + 255) / 256;
+    env_quads_kernel<<<(unsigned)(g < 65536 ? g : 65536), 256, 0, s>>>(img, w, h, quads);
+    return hipGetLastError();
+}
+
 hipError_t launch_voxelize(const VoxArgs& a, hipStream_t s)
 {
     if (a.ntri <= 0) return hipSuccess;
