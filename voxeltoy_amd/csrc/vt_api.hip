@@ -92,6 +92,7 @@ struct vt_context {
     uint32_t bit_words = 0;
     int32_t* d_emissive = nullptr;
     int32_t n_emissive = 0;
+    int4* d_lights = nullptr;       /* emissive entries decoded: x, y, z, slot */
     std::vector<float> mat;
     int32_t slot_off[256];
     int32_t n_slots = 0;            /* slots 1..n_slots used */
@@ -160,9 +161,19 @@ int upload_records(vt_context* c)
 }
 
 /* Rebuild both occupancy levels from the slot bytes. */
+vt::TraceArgs trace_args(const vt_context* c);
+
+/* After any change of the bricks or the emissive list: block occupancy and
+ * the emissive light table. */
 int refresh_occupancy(vt_context* c)
 {
     VT_HIP(vt::launch_block_masks(c->dims, c->d_bricks, c->d_occ64, c->d_bits, c->stream));
+    dfree(c->d_lights);
+    if (c->n_emissive > 0 && c->d_emissive) {
+        VT_HIP(hipMalloc(&c->d_lights, (size_t)c->n_emissive * 16u));
+        const vt::TraceArgs a = trace_args(c);
+        VT_HIP(vt::launch_light_table(a.g, c->d_lights, c->stream));
+    }
     return VT_OK;
 }
 
@@ -280,6 +291,7 @@ vt::GridView grid_view(const vt_context* c)
     g.bit_words = c->bit_words;
     g.n_emissive = c->n_emissive;
     g.emissive = c->d_emissive;
+    g.lights = c->d_lights;
     return g;
 }
 
@@ -414,7 +426,7 @@ void vt_destroy(vt_context* c)
     while (!c->batches.empty()) batch_release(c->batches.back());
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    dfree(c->d_bricks); dfree(c->d_occ64); dfree(c->d_bits); dfree(c->d_emissive); dfree(c->d_recs);
+    dfree(c->d_bricks); dfree(c->d_occ64); dfree(c->d_bits); dfree(c->d_emissive); dfree(c->d_lights); dfree(c->d_recs);
     dfree(c->d_img); dfree(c->d_quad); dfree(c->d_cdfu); dfree(c->d_cdfv);
     dfree(c->d_accum); dfree(c->d_sum64); dfree(c->d_tile); dfree(c->d_stats);
     free_wavefront(c);

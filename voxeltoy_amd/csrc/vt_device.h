@@ -77,6 +77,7 @@ struct GridView {
     uint32_t bit_words;
     int32_t n_emissive;
     const int32_t* emissive;    /* linear voxel indices */
+    const int4* lights;         /* per emissive entry: voxel x, y, z and its material slot */
 };
 
 struct EnvView {

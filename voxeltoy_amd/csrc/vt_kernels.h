@@ -176,6 +176,8 @@ hipError_t launch_dense_from_bricks(const BrickDims& d, const uint8_t* bricks, c
                                     hipStream_t s);
 /* Fill a byte buffer (graph-safe replacement for hipMemsetAsync). */
 hipError_t launch_fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s);
+/* Emissive list -> (x, y, z, slot) per entry, from the current grid. */
+hipError_t launch_light_table(const GridView& g, int4* out, hipStream_t s);
 hipError_t launch_env_quads(const float* img, int32_t w, int32_t h, float4* quads, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 /* Streaming 16-byte-vector copy (bytes must be a multiple of 16). */

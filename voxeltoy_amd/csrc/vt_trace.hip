@@ -138,12 +138,10 @@ __device__ __forceinline__ int shade_step(const TraceArgs& a, const MatRec* recs
     f3 L, wl;
     float lpdf;
     if (S.light_is_voxel) {
-        int32_t idx = g.emissive[li];
-        const int32_t dz = g.nx * g.ny, dy = g.nx;
-        S.ez = idx / dz; idx -= S.ez * dz;
-        S.ey = idx / dy; idx -= S.ey * dy;
-        S.ex = idx;
-        const MatRec& me = recs[fetch_slot(g, bits, S.ex, S.ey, S.ez)];
+        /* the emissive list entry decoded once per grid (light_table_kernel) */
+        const int4 lt = g.lights[li];
+        S.ex = lt.x; S.ey = lt.y; S.ez = lt.z;
+        const MatRec& me = recs[lt.w];
         L = mk(me.e[0], me.e[1], me.e[2]) * 10.0f;
         const f3 vsE = mk((float)S.ex, (float)S.ey, (float)S.ez);
         const f3 bmin = mk(g.bmin[0], g.bmin[1], g.bmin[2]), bmax = mk(g.bmax[0], g.bmax[1], g.bmax[2]);
@@ -1018,6 +1016,30 @@ hipError_t launch_render_cached(const TraceArgs& a, const WfPaths& P, const Cach
 {
     return a.lds_bits ? wf_run_cached<true>(a, P, C, pack, bias, cus, s)
                       : wf_run_cached<false>(a, P, C, pack, bias, cus, s);
+}
+
+/* Emissive list entry i -> (x, y, z, material slot) of its voxel: the
+ * linear index decoded as coordinates.h:118-128 builds it, the slot read
+ * with texelFetch semantics (offset 0 outside the grid). */
+__global__ __launch_bounds__(256) void light_table_kernel(GridView g, int4* __restrict__ out)
+{
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n_emissive; i += gridDim.x * blockDim.x) {
+        int32_t idx = g.emissive[i];
+        const int32_t dz = g.nx * g.ny, dy = g.nx;
+        const int32_t z = idx / dz;
+        idx -= z * dz;
+        const int32_t y = idx / dy;
+        idx -= y * dy;
+        out[i] = make_int4(idx, y, z, (int32_t)fetch_slot(g, g.bits, idx, y, z));
+    }
+}
+
+hipError_t launch_light_table(const GridView& g, int4* out, hipStream_t s)
+{
+    if (g.n_emissive <= 0) return hipSuccess;
+    const int32_t blocks = (g.n_emissive + 255) / 256;
+    light_table_kernel<<<blocks < 4096 ? blocks : 4096, 256, 0, s>>>(g, out);
+    return hipGetLastError();
 }
 
 /* First hit of the primary ray of every pixel (the parity probe). */
