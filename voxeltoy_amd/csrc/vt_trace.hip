@@ -663,7 +663,8 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
             continue;
         }
         if (has) {
-            const int status = D.advance<STATS>(G, BATCH ? G.bits : bits, budget, cnt.visits);
+            const int status = D.advance_wave<STATS, LDSB && !BATCH, !BATCH>(G, BATCH ? G.bits : bits, budget,
+                                                                             cnt.visits);
             if (status) {
                 Hit h;
                 const int kind = D.finish(G, status, h);
