@@ -185,6 +185,31 @@ def test_render_full_shading_bit_exact(ctx, orc, lens_model, mode):
     ctx.set_render_mode(1, 0)
 
 
+@pytest.mark.parametrize("budget,refill,chunk", [(1, 1, -1), (5, 64, 64), (4096, 8, 4096), (77, 3, 0)])
+def test_traversal_tunings_bit_exact(ctx, orc, budget, refill, chunk):
+    """The wavefront march split at every step (budget 1), in long marches
+    that run into the per-lane step limit (budget 4096 > max_steps), with
+    single-lane and whole-wave refills: images and counters equal the oracle."""
+    from voxeltoy_amd import workloads
+    wl = small_full_workload(1, w=80, h=60)
+    workloads.load(ctx, wl)
+    ctx.set_render_mode(1, 0)
+    ctx.set_traversal_tuning(budget, refill, chunk)
+    ctx.set_stats(True)
+    try:
+        ctx.render(wl.seed, 0, wl.spp)
+        got = ctx.read_accum()
+        stats = ctx.stats()
+    finally:
+        ctx.set_stats(False)
+        ctx.set_traversal_tuning(0, 0, 0)
+    ref, cnt = oracle_for(orc, wl).render(wl.width, wl.height, wl.seed, 0, wl.spp)
+    same = (got.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), f"{(~same).sum()} values differ"
+    for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
+        assert stats[k] == cnt[k], k
+
+
 def test_render_settings_variants_bit_exact(ctx, orc):
     """Wireframe overlay, selected voxel, exposure/gamma and rng frame clamp."""
     from voxeltoy_amd import workloads

@@ -70,7 +70,7 @@ struct PathBuf {
 
 struct WfPaths {
     PathBuf s[2];     /* state of the paths in queue 0 / queue 1 */
-    int4* hit;        /* traversal result per queue entry: x, y, z, kind << 8 | slot */
+    int4* hit;        /* traversal result per queue entry: x, y, z, kind << 8 */
     float4* smp;      /* finished tonemapped sample of each path of the chunk */
     float4* qa0;      /* ray queue 0 (2 x cap entries: a path's shadow and continuation rays at 2k, 2k + 1):
                          origin.xyz moved by sign(d)*0.001, sample slot (as bits) */

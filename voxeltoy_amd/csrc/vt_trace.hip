@@ -666,9 +666,9 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
             const int status = D.advance_wave<STATS, LDSB && !BATCH, !BATCH>(G, BATCH ? G.bits : bits, budget,
                                                                              cnt.visits);
             if (status) {
-                Hit h;
-                const int kind = D.finish(G, status, h);
-                P.hit[jq] = make_int4(h.x, h.y, h.z, (kind << 8) | (int32_t)h.slot);
+                /* kind and voxel only: the shade kernel reads the material slot */
+                const int kind = status == 1 ? KIND_VOXEL : (D.y < 0 ? KIND_GROUND : KIND_MISS);
+                P.hit[jq] = make_int4(D.x, D.y, D.z, kind << 8);
                 has = false;
             }
         }
@@ -762,12 +762,15 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
             if (PHASE != PH_PRIMARY) {
                 /* the shadow ray's result (never ends the path) */
                 const int4 hs = P.hit[2 * j];
-                h.x = hs.x; h.y = hs.y; h.z = hs.z; h.slot = (uint32_t)hs.w & 0xffu;
+                h.x = hs.x; h.y = hs.y; h.z = hs.z; h.slot = 0u;   /* visibility needs no material */
                 shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, hs.w >> 8, cnt, -1, vraw, nullptr);
             }
             const int4 hv = P.hit[PHASE == PH_PRIMARY ? j : 2 * j + 1];
-            h.x = hv.x; h.y = hv.y; h.z = hv.z; h.slot = (uint32_t)hv.w & 0xffu;
             const int kind = hv.w >> 8;
+            h.x = hv.x; h.y = hv.y; h.z = hv.z;
+            /* the hit voxel's material slot is read here, where occupancy
+             * hides the load, not by the traversal (ground: slot 0) */
+            h.slot = kind == KIND_VOXEL ? fetch_slot(a.g, bits, hv.x, hv.y, hv.z) : 0u;
             bool vrec = false;
             const int st = shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, kind, cnt,
                                                     slot1 ? C.train_bounce : C.cache_bounce, vraw, &vrec);
