@@ -210,6 +210,31 @@ def test_traversal_tunings_bit_exact(ctx, orc, budget, refill, chunk):
         assert stats[k] == cnt[k], k
 
 
+@pytest.mark.parametrize("shape,angle", [((41, 30, 53), 1), ((9, 6, 13), 3), ((3, 1, 2), 0), ((64, 64, 61), 6)])
+def test_render_ragged_grid_bit_exact(ctx, orc, shape, angle):
+    """Grids whose extents are not multiples of the 4-voxel block: rays leave
+    the grid inside partly covered blocks."""
+    from voxeltoy_amd import scenes, workloads
+    sc = scenes.small_scene(12, n=64)
+    sc.ids = np.ascontiguousarray(sc.ids[:shape[0], :shape[1], :shape[2]])
+    bmin, bmax = workloads.reference_bounds(sc.shape)
+    cam = scenes.orbit_camera(bmin, bmax, math.radians(60), math.radians(45.0 * angle), 72, 56)
+    wl = workloads.Workload("ragged", sc, cam, 72, 56, spp=2, bounces=3, bmin=bmin, bmax=bmax,
+                            env_top=(1.5, 1.4, 1.2), env_bottom=(0.3, 0.3, 0.35))
+    workloads.load(ctx, wl)
+    ctx.set_render_mode(1, 0)
+    ctx.set_stats(True)
+    ctx.render(wl.seed, 0, wl.spp)
+    got = ctx.read_accum()
+    stats = ctx.stats()
+    ctx.set_stats(False)
+    ref, cnt = oracle_for(orc, wl).render(wl.width, wl.height, wl.seed, 0, wl.spp)
+    same = (got.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), f"{(~same).sum()} values differ"
+    for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
+        assert stats[k] == cnt[k], k
+
+
 def test_render_settings_variants_bit_exact(ctx, orc):
     """Wireframe overlay, selected voxel, exposure/gamma and rng frame clamp."""
     from voxeltoy_amd import workloads

@@ -90,6 +90,9 @@ struct vt_context {
     uint64_t* d_occ64 = nullptr;
     uint32_t* d_bits = nullptr;
     uint32_t bit_words = 0;
+    uint64_t* d_occp = nullptr;     /* padded levels of the wave march */
+    uint32_t* d_bitsp = nullptr;
+    uint32_t bit_words_p = 0;
     int32_t* d_emissive = nullptr;
     int32_t n_emissive = 0;
     int4* d_lights = nullptr;       /* emissive entries decoded: x, y, z, slot */
@@ -167,7 +170,7 @@ vt::TraceArgs trace_args(const vt_context* c);
  * the emissive light table. */
 int refresh_occupancy(vt_context* c)
 {
-    VT_HIP(vt::launch_block_masks(c->dims, c->d_bricks, c->d_occ64, c->d_bits, c->stream));
+    VT_HIP(vt::launch_block_masks(c->dims, c->d_bricks, c->d_occ64, c->d_bits, c->d_occp, c->d_bitsp, c->stream));
     dfree(c->d_lights);
     if (c->n_emissive > 0 && c->d_emissive) {
         VT_HIP(hipMalloc(&c->d_lights, (size_t)c->n_emissive * 16u));
@@ -215,6 +218,8 @@ int common_volume_setup(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const
     dfree(c->d_bricks);
     dfree(c->d_occ64);
     dfree(c->d_bits);
+    dfree(c->d_occp);
+    dfree(c->d_bitsp);
     dfree(c->d_emissive);
     c->dims = d;
     if (bmin && bmax) {
@@ -231,6 +236,10 @@ int common_volume_setup(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const
     VT_HIP(hipMalloc(&c->d_occ64, d.n_blocks * 8u));
     c->bit_words = (uint32_t)((d.n_blocks + 31) / 32);
     VT_HIP(hipMalloc(&c->d_bits, (size_t)c->bit_words * 4u));
+    const uint64_t np = (uint64_t)(d.kx + 2) * (uint64_t)(d.ky + 2) * (uint64_t)(d.kz + 2);
+    VT_HIP(hipMalloc(&c->d_occp, np * 8u));
+    c->bit_words_p = (uint32_t)((np + 31) / 32);
+    VT_HIP(hipMalloc(&c->d_bitsp, (size_t)c->bit_words_p * 4u));
     VT_HIP(vt::launch_fill_u8(c->d_bricks, d.n_bricks * 512u, 0, c->stream));
     return VT_OK;
 }
@@ -289,6 +298,12 @@ vt::GridView grid_view(const vt_context* c)
     g.occ64 = c->d_occ64;
     g.bits = c->d_bits;
     g.bit_words = c->bit_words;
+    g.bxp = (uint32_t)c->dims.kx + 2u;
+    g.byp = (uint32_t)c->dims.ky + 2u;
+    g.occp = c->d_occp;
+    g.bitsp = c->d_bitsp;
+    g.bit_words_p = c->bit_words_p;
+    g.wide = ((uint64_t)(c->dims.kx + 2) * (uint64_t)(c->dims.ky + 2) * (uint64_t)(c->dims.kz + 2)) > (1ull << 29);
     g.n_emissive = c->n_emissive;
     g.emissive = c->d_emissive;
     g.lights = c->d_lights;
@@ -332,6 +347,7 @@ vt::TraceArgs trace_args(const vt_context* c)
     a.cdfv_lds_bytes = c->env.use_image ? ((c->env.cdf_v_n * 4 + 15) / 16) * 16 : 0;
     const size_t base = 256 * sizeof(vt::MatRec) + (size_t)a.cdfv_lds_bytes;
     a.lds_bits = (base + (size_t)c->bit_words * 4u <= c->lds_limit) ? 1 : 0;
+    a.lds_bits_p = ((size_t)c->bit_words_p * 4u <= c->lds_limit) ? 1 : 0;
     a.dda_budget = c->dda_budget;
     a.refill_min = c->refill_min;
     a.refill_chunk = c->refill_chunk;
@@ -426,7 +442,7 @@ void vt_destroy(vt_context* c)
     while (!c->batches.empty()) batch_release(c->batches.back());
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    dfree(c->d_bricks); dfree(c->d_occ64); dfree(c->d_bits); dfree(c->d_emissive); dfree(c->d_lights); dfree(c->d_recs);
+    dfree(c->d_bricks); dfree(c->d_occ64); dfree(c->d_bits); dfree(c->d_occp); dfree(c->d_bitsp); dfree(c->d_emissive); dfree(c->d_lights); dfree(c->d_recs);
     dfree(c->d_img); dfree(c->d_quad); dfree(c->d_cdfu); dfree(c->d_cdfv);
     dfree(c->d_accum); dfree(c->d_sum64); dfree(c->d_tile); dfree(c->d_stats);
     free_wavefront(c);

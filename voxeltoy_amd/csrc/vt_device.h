@@ -75,6 +75,15 @@ struct GridView {
     const uint64_t* occ64;      /* per-block voxel masks, linear block order */
     const uint32_t* bits;       /* per-block non-empty bits (global copy) */
     uint32_t bit_words;
+    /* the same two levels over the block grid padded by one block on every
+     * side (grid block (bx, by, bz) is padded block (bx+1, by+1, bz+1)), in
+     * which every voxel outside the grid reads as occupied: the wave march
+     * stops at the first voxel outside the grid without a bounds test */
+    uint32_t bxp, byp;          /* padded block extents in x and y */
+    const uint64_t* occp;
+    const uint32_t* bitsp;
+    uint32_t bit_words_p;
+    int32_t wide;               /* padded masks span 4 GiB or more (byte offsets need 64 bits) */
     int32_t n_emissive;
     const int32_t* emissive;    /* linear voxel indices */
     const int4* lights;         /* per emissive entry: voxel x, y, z and its material slot */
@@ -332,9 +341,12 @@ struct DdaState {
      * instead of as selects, the step count follows a scalar iteration
      * counter, and which lanes sit in a non-empty block is a scalar lane
      * mask maintained at block changes.  The wave runs at most `budget`
-     * iterations and no lane passes its max_steps.  LDSB: `bits` points
-     * into LDS; UNI: every lane marches the same grid (else grid, bits and
-     * masks are per lane, as in batch launches). */
+     * iterations and no lane passes its max_steps.  It reads the padded
+     * block levels of the grid (bxp, byp, occp; `bits` = bitsp or its LDS
+     * copy), where the voxels outside the grid are set, so leaving the grid
+     * needs no test of its own.  LDSB: `bits` points into LDS; UNI: every
+     * lane marches the same grid and its padded masks span less than 4 GiB
+     * (else grid, bits and masks are per lane, as in batch launches). */
     template <bool COUNT, bool LDSB, bool UNI>
     __device__ __forceinline__ int advance_wave(const GridView& g, const uint32_t* __restrict__ bits, int budget,
                                                 uint32_t& visits);
@@ -353,40 +365,38 @@ struct DdaState {
 };
 
 /* ---- wave march: the loop of DdaState::advance_wave ---------------------------------
- * Per iteration, for the marching lanes (exec):
- *   in   = voxel inside the grid
- *   key  = linear index of its 4x4x4 block; lanes inside whose key changed
- *          reload: block bit from `bits`, then the block's voxel mask (0 for
- *          an empty block); ne = lanes whose block is non-empty
- *   hit  = in & ne & voxel bit set (tested only when some lane is in & ne)
- *   lanes with !in or hit stop: steps += iteration, they leave exec
+ * Coordinates inside the loop are offset by +4 (one block), so the voxel at
+ * x = -1 has a non-negative block index in the padded block grid.  Per
+ * iteration, for the marching lanes (exec):
+ *   key  = linear index of the voxel's padded 4x4x4 block; lanes whose key
+ *          changed reload: block bit from `bits`, then the block's voxel
+ *          mask; ne = lanes whose block is non-empty (kept in bit 31 of key
+ *          between calls)
+ *   hit  = ne & voxel bit set (tested only when some lane is in a non-empty
+ *          block); voxels outside the grid are set in the padded masks, so
+ *          a hit is also how a ray leaves the grid
+ *   lanes that hit stop: steps += iteration, they leave exec
  *   step: m = min3(dis); each axis whose dis equals m adds its increment
  *   to dis and its sign to the coordinate, under that axis's lane mask.
  * At the end lanes still marching add the iteration count to steps, the
- * entry mask is restored and `flag` is 1 on the lanes still marching. */
+ * entry mask is restored and t1 is 1 on the lanes still marching. */
 #define VT_WAVE_DDA_HEAD                                                                  \
     "s_mov_b64 %[live], exec\n\t"                                                         \
-    "v_cmp_ne_u64_e64 %[ne], 0, %[mask]\n\t"                                              \
+    "v_cmp_gt_i32_e64 %[ne], 0, %[key]\n\t"                                               \
+    "v_bfe_u32 %[key], %[key], 0, 31\n\t"                                                 \
     "s_mov_b32 %[it], 0\n"                                                                \
     ".Lvt_top%=:\n\t"                                                                     \
     "s_cmp_ge_u32 %[it], %[n]\n\t"                                                        \
     "s_cbranch_scc1 .Lvt_end%=\n\t"                                                       \
-    "v_cmp_gt_u32_e64 %[in], %[nx], %[x]\n\t"                                             \
-    "v_cmp_gt_u32_e64 %[s0], %[ny], %[y]\n\t"                                             \
-    "s_and_b64 %[in], %[in], %[s0]\n\t"                                                   \
-    "v_cmp_gt_u32_e64 %[s0], %[nz], %[z]\n\t"                                             \
-    "s_and_b64 %[in], %[in], %[s0]\n\t"                                                   \
     "v_lshrrev_b32 %[t0], 2, %[z]\n\t"                                                    \
     "v_lshrrev_b32 %[t1], 2, %[y]\n\t"                                                    \
     "v_mad_u32_u24 %[t0], %[t0], %[by4], %[t1]\n\t"                                       \
     "v_lshrrev_b32 %[t1], 2, %[x]\n\t"                                                    \
     "v_mad_u32_u24 %[t0], %[t0], %[bx4], %[t1]\n\t"                                       \
     "v_cmp_ne_u32_e64 %[s0], %[t0], %[key]\n\t"                                           \
-    "s_and_b64 %[s0], %[s0], %[in]\n\t"                                                   \
-    "s_cbranch_scc0 .Lvt_norl%=\n\t"                                                      \
     "s_and_saveexec_b64 %[s1], %[s0]\n\t"                                                 \
+    "s_cbranch_scc0 .Lvt_norl%=\n\t"                                                      \
     "v_mov_b32 %[key], %[t0]\n\t"                                                         \
-    "v_mov_b64 %[mask], 0\n\t"                                                            \
     "v_lshrrev_b32 %[t1], 5, %[t0]\n\t"
 /* between HEAD and MID: t1 = 32-bit word of the block bits holding bit t0 */
 #define VT_WAVE_DDA_BITS_LDS                                                              \
@@ -407,7 +417,7 @@ struct DdaState {
     "s_andn2_b64 %[ne], %[ne], exec\n\t"                                                  \
     "s_or_b64 %[ne], %[ne], %[s0]\n\t"                                                    \
     "s_and_b64 exec, exec, %[s0]\n\t"                                                     \
-    "s_cbranch_execz .Lvt_nold%=\n\t"
+    "s_cbranch_execz .Lvt_norl%=\n\t"
 /* between MID and TAIL: mask = voxel mask of block t0 */
 #define VT_WAVE_DDA_MASK_GLOBAL                                                           \
     "v_lshlrev_b32 %[t1], 3, %[t0]\n\t"                                                   \
@@ -418,11 +428,10 @@ struct DdaState {
     "global_load_dwordx2 %[mask], %[t64], off\n\t"                                        \
     "s_waitcnt vmcnt(0)\n"
 #define VT_WAVE_DDA_TAIL                                                                  \
-    ".Lvt_nold%=:\n\t"                                                                    \
-    "s_mov_b64 exec, %[s1]\n"                                                             \
     ".Lvt_norl%=:\n\t"                                                                    \
-    "s_and_b64 %[s0], %[ne], %[in]\n\t"                                                   \
-    "s_cbranch_scc0 .Lvt_nohit%=\n\t"                                                     \
+    "s_mov_b64 exec, %[s1]\n\t"                                                           \
+    "s_and_b64 %[s0], %[ne], exec\n\t"                                                    \
+    "s_cbranch_scc0 .Lvt_go%=\n\t"                                                        \
     "s_and_saveexec_b64 %[s1], %[s0]\n\t"                                                 \
     "v_lshlrev_b32 %[t1], 2, %[y]\n\t"                                                    \
     "v_bfi_b32 %[t0], 3, %[x], %[t1]\n\t"                                                 \
@@ -431,26 +440,24 @@ struct DdaState {
     "v_not_b32 %[t0], %[t0]\n\t"                                                          \
     "v_lshlrev_b64 %[t64], %[t0], %[mask]\n\t"                                            \
     "v_cmp_gt_i64_e64 %[s0], 0, %[t64]\n\t"                                               \
-    "s_mov_b64 exec, %[s1]\n"                                                             \
-    ".Lvt_nohit%=:\n\t"                                                                   \
-    "s_andn2_b64 %[s1], exec, %[in]\n\t"                                                  \
-    "s_or_b64 %[s1], %[s1], %[s0]\n\t"                                                    \
-    "s_cbranch_scc0 .Lvt_go%=\n\t"                                                        \
-    "s_andn2_b64 %[s0], exec, %[s1]\n\t"                                                  \
     "s_mov_b64 exec, %[s1]\n\t"                                                           \
-    "v_add_u32 %[steps], %[it], %[steps]\n\t"                                             \
+    "s_cmp_eq_u64 %[s0], 0\n\t"                                                           \
+    "s_cbranch_scc1 .Lvt_go%=\n\t"                                                        \
+    "s_andn2_b64 %[s1], exec, %[s0]\n\t"                                                  \
     "s_mov_b64 exec, %[s0]\n\t"                                                           \
+    "v_add_u32 %[steps], %[it], %[steps]\n\t"                                             \
+    "s_mov_b64 exec, %[s1]\n\t"                                                           \
     "s_cbranch_execz .Lvt_out%=\n"                                                        \
     ".Lvt_go%=:\n\t"                                                                      \
     "v_min3_f32 %[t0], %[dx], %[dy], %[dz]\n\t"                                           \
     "s_mov_b64 %[s1], exec\n\t"                                                           \
     "v_cmp_eq_f32_e64 %[s0], %[dx], %[t0]\n\t"                                            \
-    "v_cmp_eq_f32_e64 %[in], %[dy], %[t0]\n\t"                                            \
+    "v_cmp_eq_f32_e64 %[s2], %[dy], %[t0]\n\t"                                            \
     "v_cmp_eq_f32_e64 vcc, %[dz], %[t0]\n\t"                                              \
     "s_mov_b64 exec, %[s0]\n\t"                                                           \
     "v_add_f32 %[dx], %[dx], %[ax]\n\t"                                                   \
     "v_add_u32 %[x], %[x], %[sx]\n\t"                                                     \
-    "s_mov_b64 exec, %[in]\n\t"                                                           \
+    "s_mov_b64 exec, %[s2]\n\t"                                                           \
     "v_add_f32 %[dy], %[dy], %[ay]\n\t"                                                   \
     "v_add_u32 %[y], %[y], %[sy]\n\t"                                                     \
     "s_mov_b64 exec, vcc\n\t"                                                             \
@@ -463,14 +470,16 @@ struct DdaState {
     "v_add_u32 %[steps], %[it], %[steps]\n"                                               \
     ".Lvt_out%=:\n\t"                                                                     \
     "s_mov_b64 %[s0], exec\n\t"                                                           \
+    "s_and_b64 exec, %[live], %[ne]\n\t"                                                  \
+    "v_or_b32 %[key], 0x80000000, %[key]\n\t"                                             \
     "s_mov_b64 exec, %[live]\n\t"                                                         \
     "v_cndmask_b32_e64 %[t1], 0, 1, %[s0]\n\t"
 
 #define VT_WAVE_DDA_OUTS                                                                  \
-    [x] "+v"(x), [y] "+v"(y), [z] "+v"(z), [dx] "+v"(disx), [dy] "+v"(disy), [dz] "+v"(disz),            \
+    [x] "+v"(xo), [y] "+v"(yo), [z] "+v"(zo), [dx] "+v"(disx), [dy] "+v"(disy), [dz] "+v"(disz),         \
         [steps] "+v"(steps), [key] "+v"(key), [mask] "+v"(mask), [t0] "=&v"(t0), [t1] "=&v"(t1),         \
-        [t64] "=&v"(t64), [live] "=&s"(s_live), [ne] "=&s"(s_ne), [in] "=&s"(s_in), [s0] "=&s"(s_0),     \
-        [s1] "=&s"(s_1), [it] "=&s"(it)
+        [t64] "=&v"(t64), [live] "=&s"(s_live), [ne] "=&s"(s_ne), [s0] "=&s"(s_0), [s1] "=&s"(s_1),      \
+        [s2] "=&s"(s_2), [it] "=&s"(it)
 #define VT_WAVE_DDA_INS                                                                   \
     [ax] "v"(ax), [ay] "v"(ay), [az] "v"(az), [sx] "v"(stx), [sy] "v"(sty), [sz] "v"(stz), [n] "s"(n)
 
@@ -488,30 +497,29 @@ __device__ __forceinline__ int DdaState::advance_wave(const GridView& g, const u
     }
     n = n > 0 ? n : 0;
     const int32_t steps0 = steps;
+    int32_t xo = x + 4, yo = y + 4, zo = z + 4;
     uint32_t t0, t1;
-    uint64_t t64, s_live, s_ne, s_in, s_0, s_1;
+    uint64_t t64, s_live, s_ne, s_0, s_1, s_2;
     uint32_t it;
     if constexpr (LDSB) {
         const uint32_t lds = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const uint32_t*)bits;
         asm volatile(VT_WAVE_DDA_HEAD VT_WAVE_DDA_BITS_LDS VT_WAVE_DDA_MID VT_WAVE_DDA_MASK_GLOBAL VT_WAVE_DDA_TAIL
                      : VT_WAVE_DDA_OUTS
-                     : VT_WAVE_DDA_INS, [nx] "s"(g.nx), [ny] "s"(g.ny), [nz] "s"(g.nz), [bx4] "s"(g.bx4),
-                       [by4] "s"(g.by4), [bits] "s"(lds), [occ] "s"(g.occ64)
+                     : VT_WAVE_DDA_INS, [bx4] "s"(g.bxp), [by4] "s"(g.byp), [bits] "s"(lds), [occ] "s"(g.occp)
                      : "vcc", "scc", "memory");
     } else if constexpr (UNI) {
         asm volatile(VT_WAVE_DDA_HEAD VT_WAVE_DDA_BITS_GLOBAL VT_WAVE_DDA_MID VT_WAVE_DDA_MASK_GLOBAL
                          VT_WAVE_DDA_TAIL
                      : VT_WAVE_DDA_OUTS
-                     : VT_WAVE_DDA_INS, [nx] "s"(g.nx), [ny] "s"(g.ny), [nz] "s"(g.nz), [bx4] "s"(g.bx4),
-                       [by4] "s"(g.by4), [bits] "s"(bits), [occ] "s"(g.occ64)
+                     : VT_WAVE_DDA_INS, [bx4] "s"(g.bxp), [by4] "s"(g.byp), [bits] "s"(bits), [occ] "s"(g.occp)
                      : "vcc", "scc", "memory");
     } else {
         asm volatile(VT_WAVE_DDA_HEAD VT_WAVE_DDA_BITS_LANE VT_WAVE_DDA_MID VT_WAVE_DDA_MASK_LANE VT_WAVE_DDA_TAIL
                      : VT_WAVE_DDA_OUTS
-                     : VT_WAVE_DDA_INS, [nx] "v"(g.nx), [ny] "v"(g.ny), [nz] "v"(g.nz), [bx4] "v"(g.bx4),
-                       [by4] "v"(g.by4), [bits] "v"(bits), [occ] "v"(g.occ64)
+                     : VT_WAVE_DDA_INS, [bx4] "v"(g.bxp), [by4] "v"(g.byp), [bits] "v"(bits), [occ] "v"(g.occp)
                      : "vcc", "scc", "memory");
     }
+    x = xo - 4; y = yo - 4; z = zo - 4;
     const bool marching = t1 != 0u;
     const bool inside = (uint32_t)x < (uint32_t)g.nx && (uint32_t)y < (uint32_t)g.ny && (uint32_t)z < (uint32_t)g.nz;
     /* a visit per iteration spent inside the grid under the limit: every

@@ -23,6 +23,7 @@ struct TraceArgs {
     uint32_t* tile_counter;
     int32_t tiles_x, n_tiles;
     int32_t lds_bits;            /* stage occupancy bits in LDS */
+    int32_t lds_bits_p;          /* wavefront traversal: stage the padded block bits in LDS */
     int32_t cdfv_lds_bytes;
     int32_t dda_budget;          /* wavefront traversal: DDA steps between refill checks */
     int32_t refill_min;          /* wavefront traversal: idle lanes that trigger a refill */
@@ -168,9 +169,12 @@ hipError_t launch_brick_from_offsets(const BrickDims& d, const int32_t* dense, c
 hipError_t launch_brick_from_ids(const BrickDims& d, const uint8_t* dense, const uint8_t* remap, uint8_t* bricks,
                                  hipStream_t s);
 /* Per 4x4x4 block voxel masks in linear block order (occ64 holds n_blocks
- * words) and the per-block non-empty bits ((n_blocks+31)/32 words). */
+ * words) and the per-block non-empty bits ((n_blocks+31)/32 words); then
+ * the same over the block grid padded by one block on every side
+ * ((kx+2)(ky+2)(kz+2) words in occp), where voxels outside the grid are
+ * set. */
 hipError_t launch_block_masks(const BrickDims& d, const uint8_t* bricks, uint64_t* occ64, uint32_t* bits,
-                              hipStream_t s);
+                              uint64_t* occp, uint32_t* bitsp, hipStream_t s);
 /* Bricked slots -> dense int32 offsets via slot_off[256]. */
 hipError_t launch_dense_from_bricks(const BrickDims& d, const uint8_t* bricks, const int32_t* slot_off, int32_t* dense,
                                     hipStream_t s);

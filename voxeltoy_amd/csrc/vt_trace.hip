@@ -588,10 +588,14 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
                                                                          int sel)
 {
     extern __shared__ __align__(16) uint8_t smem[];
-    MatRec* recs;
-    float* cdfv;
-    const uint32_t* bits;
-    stage_lds<LDSB>(a, smem, recs, cdfv, bits);
+    /* the march reads the padded block bits only */
+    const uint32_t* bits = a.g.bitsp;
+    if (LDSB) {
+        uint32_t* sb = (uint32_t*)smem;
+        for (uint32_t i = threadIdx.x; i < a.g.bit_words_p; i += blockDim.x) sb[i] = a.g.bitsp[i];
+        bits = sb;
+        __syncthreads();
+    }
     const float4* qa = sel ? P.qa1 : P.qa0;
     const float4* qb = sel ? P.qb1 : P.qb0;
     const uint32_t n = P.cnt[sel];
@@ -663,8 +667,11 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
             continue;
         }
         if (has) {
-            const int status = D.advance_wave<STATS, LDSB && !BATCH, !BATCH>(G, BATCH ? G.bits : bits, budget,
-                                                                             cnt.visits);
+            /* per-lane grids (batch) and grids whose padded masks span 4 GiB
+             * address the masks with 64-bit per-lane arithmetic */
+            const int status = (BATCH || G.wide)
+                                   ? D.advance_wave<STATS, false, false>(G, G.bitsp, budget, cnt.visits)
+                                   : D.advance_wave<STATS, LDSB, true>(G, bits, budget, cnt.visits);
             if (status) {
                 /* kind and voxel only: the shade kernel reads the material slot */
                 const int kind = status == 1 ? KIND_VOXEL : (D.y < 0 ? KIND_GROUND : KIND_MISS);
@@ -881,7 +888,7 @@ static hipError_t wf_run(const TraceArgs& a, const BatchTable& B, const WfPaths&
                          uint64_t n_pixels, int rounds, int cus, hipStream_t s, const LaunchTimer* timer = nullptr,
                          hipEvent_t acc_wait = nullptr)
 {
-    const size_t lds_t = BATCH ? 0 : render_lds_bytes(a);
+    const size_t lds_t = (!BATCH && LDSB) ? (size_t)a.g.bit_words_p * 4u : 0u;
     const size_t lds_s = BATCH ? 0 : 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
     const int bt = blocks_per_cu(wf_traverse_kernel<STATS, LDSB, BATCH>, WF_TRAVERSE_THREADS, lds_t);
     const int bs0 = blocks_per_cu(wf_shade_kernel<STATS, PH_PRIMARY, BATCH>, WF_SHADE_THREADS, lds_s);
@@ -931,10 +938,10 @@ hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus
     const uint64_t total = WH * a.count;
     const int rounds = 1 + (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
     if (stats)
-        return a.lds_bits ? wf_run<true, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
-                          : wf_run<true, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
-    return a.lds_bits ? wf_run<false, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
-                      : wf_run<false, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
+        return a.lds_bits_p ? wf_run<true, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
+                            : wf_run<true, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
+    return a.lds_bits_p ? wf_run<false, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
+                        : wf_run<false, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
 }
 
 hipError_t launch_render_batch(const TraceArgs& a0, const BatchTable& B, const WfPaths& P, uint64_t total_paths,
@@ -974,7 +981,7 @@ static hipError_t wf_run_cached(const TraceArgs& a, const WfPaths& P, const Cach
                                 const float* bias, int cus, hipStream_t s)
 {
     const BatchTable B = { nullptr, nullptr, nullptr, 0u, nullptr };
-    const size_t lds_t = render_lds_bytes(a);
+    const size_t lds_t = LDSB ? (size_t)a.g.bit_words_p * 4u : 0u;
     const size_t lds_s = 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
     const int bt = blocks_per_cu(wf_traverse_kernel<false, LDSB, false>, WF_TRAVERSE_THREADS, lds_t);
     const int bs0 = blocks_per_cu(wf_shade_kernel<false, PH_PRIMARY, false, true>, WF_SHADE_THREADS, lds_s);
@@ -1018,8 +1025,8 @@ static hipError_t wf_run_cached(const TraceArgs& a, const WfPaths& P, const Cach
 hipError_t launch_render_cached(const TraceArgs& a, const WfPaths& P, const CacheRender& C, const void* pack,
                                 const float* bias, int cus, hipStream_t s)
 {
-    return a.lds_bits ? wf_run_cached<true>(a, P, C, pack, bias, cus, s)
-                      : wf_run_cached<false>(a, P, C, pack, bias, cus, s);
+    return a.lds_bits_p ? wf_run_cached<true>(a, P, C, pack, bias, cus, s)
+                        : wf_run_cached<false>(a, P, C, pack, bias, cus, s);
 }
 
 /* Emissive list entry i -> (x, y, z, material slot) of its voxel: the

@@ -97,6 +97,34 @@ __global__ __launch_bounds__(256) void block_bits_kernel(uint64_t n_blocks, cons
     }
 }
 
+/* occp over the padded block grid (kx+2) x (ky+2) x (kz+2): border blocks
+ * all set; interior block (bx, by, bz) = grid block (bx-1, by-1, bz-1)
+ * with the bits of its voxels outside the grid set (grids whose extents
+ * are not multiples of 4). */
+__global__ __launch_bounds__(256) void block_masks_padded_kernel(BrickDims d, const uint64_t* __restrict__ occ64,
+                                                                 uint64_t* __restrict__ occp)
+{
+    const uint32_t px = (uint32_t)d.kx + 2u, py = (uint32_t)d.ky + 2u, pz = (uint32_t)d.kz + 2u;
+    const uint64_t n = (uint64_t)px * py * pz;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t bx = (uint32_t)(w % px);
+        const uint64_t r = w / px;
+        const uint32_t by = (uint32_t)(r % py), bz = (uint32_t)(r / py);
+        if (bx == 0 || by == 0 || bz == 0 || bx == px - 1u || by == py - 1u || bz == pz - 1u) {
+            occp[w] = ~0ull;
+            continue;
+        }
+        const uint32_t gx = bx - 1u, gy = by - 1u, gz = bz - 1u;
+        uint64_t m = occ64[gx + (uint64_t)d.kx * (gy + (uint64_t)d.ky * gz)];
+        for (uint32_t l = 0; l < 64u; ++l) {
+            const int32_t x = (int32_t)(4u * gx + (l & 3u)), y = (int32_t)(4u * gy + ((l >> 2) & 3u)),
+                          z = (int32_t)(4u * gz + (l >> 4));
+            if (x >= d.nx || y >= d.ny || z >= d.nz) m |= 1ull << l;
+        }
+        occp[w] = m;
+    }
+}
+
 __global__ __launch_bounds__(256) void dense_from_bricks_kernel(BrickDims d, const uint8_t* __restrict__ bricks,
                                                                 const int32_t* __restrict__ slot_off,
                                                                 int32_t* __restrict__ dense)
@@ -159,10 +187,13 @@ hipError_t launch_brick_from_ids(const BrickDims& d, const uint8_t* dense, const
 }
 
 hipError_t launch_block_masks(const BrickDims& d, const uint8_t* bricks, uint64_t* occ64, uint32_t* bits,
-                              hipStream_t s)
+                              uint64_t* occp, uint32_t* bitsp, hipStream_t s)
 {
     block_masks_kernel<<<grid_for(d.n_blocks), 256, 0, s>>>(d, bricks, occ64);
     block_bits_kernel<<<grid_for((d.n_blocks + 31) / 32), 256, 0, s>>>(d.n_blocks, occ64, bits);
+    const uint64_t np = (uint64_t)(d.kx + 2) * (uint64_t)(d.ky + 2) * (uint64_t)(d.kz + 2);
+    block_masks_padded_kernel<<<grid_for(np), 256, 0, s>>>(d, occ64, occp);
+    block_bits_kernel<<<grid_for((np + 31) / 32), 256, 0, s>>>(np, occp, bitsp);
     return hipGetLastError();
 }
 
