@@ -644,7 +644,10 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
                 if (p < avail) j = c_pos + p;
                 else if (grab && nb + (p - avail) < n) j = nb + (p - avail);
                 if (j != 0xffffffffu) {
-                    const float4 o = qa[j], d = qb[j];
+                    float4 o = qa[j], d = qb[j];
+                    /* both halves of the entry in flight together: one
+                     * memory latency per refill instead of two */
+                    asm volatile("" : "+v"(o.x), "+v"(o.y), "+v"(o.z), "+v"(d.x), "+v"(d.y), "+v"(d.z), "+v"(d.w));
                     jq = j;
                     if (BATCH) gl = B.tab[queue_tag(d)].g;
                     if (D.init(G, unpack_ray(o, d))) {
