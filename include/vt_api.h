@@ -179,13 +179,16 @@ int vt_get_launch_timing(vt_context* ctx, double* traverse_ms, uint32_t* launche
 
 /* ---- batches -------------------------------------------------------------
  * Many scenes (contexts on one device, each with its own volume, camera,
- * environment, settings and frame) rendered by one wavefront launch whose
- * queues hold the paths of every scene; optionally recorded once as a HIP
- * graph and replayed.  Each context's accumulation afterwards equals what
+ * environment, settings and frame) rendered together: scenes of at least
+ * 2^22 paths (width x height x count) get their own wavefront launches on
+ * their own context streams, concurrently; the others share one wavefront
+ * launch whose queues hold the paths of all of them, optionally recorded
+ * once as a HIP graph and replayed.  Each context's accumulation afterwards equals what
  * vt_render(ctx, seeds[i], n0[i], counts[i], rng_frame_max) gives.  The
- * traversal tuning and path capacity (vt_set_render_mode max_paths) of
- * ctxs[0] apply.  No counterpart in the reference (one scene per GL
- * context); the contexts must outlive the batch. */
+ * traversal tuning of ctxs[0] applies to the shared launch, each scene's
+ * own tuning to its own launches; the path capacity (vt_set_render_mode
+ * max_paths) of ctxs[0] applies to both.  No counterpart in the reference
+ * (one scene per GL context); the contexts must outlive the batch. */
 typedef struct vt_batch vt_batch;
 int vt_batch_create(vt_context* const* ctxs, int32_t n, vt_batch** out);
 /* flags: bit 0 = launch through a HIP graph (recorded on first use and

@@ -101,6 +101,47 @@ def test_batch_chunked_and_stats():
         c.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_with_scenes_of_their_own_launches(graph):
+    """Scenes of 2^22 paths or more take their own launches (on their own
+    streams), the rest one shared launch; both kinds equal their single
+    renders, also on a graph replay with new seeds, and the counters add up
+    over both kinds."""
+    from voxeltoy_amd import api, scenes, workloads
+    wls = mixed_scenes()[:3]
+    big = []
+    for i, (n, W, H, spp) in enumerate([(32, 1024, 1024, 4), (40, 1024, 768, 6)]):
+        sc = scenes.small_scene(30 + i, n=n)
+        bmin, bmax = workloads.reference_bounds(sc.shape)
+        cam = scenes.orbit_camera(bmin, bmax, math.radians(120), math.radians(40 + 50 * i), W, H)
+        big.append(workloads.Workload(f"b{i}", sc, cam, W, H, spp=spp, bounces=2, seed=300 + i, bmin=bmin, bmax=bmax,
+                                      env_top=(1.5, 1.4, 1.2), env_bottom=(0.3, 0.3, 0.35)))
+    wls = [wls[0], big[0], wls[1], big[1], wls[2]]
+    assert sum(wl.width * wl.height * wl.spp >= (1 << 22) for wl in wls) == 2
+    seeds = [wl.seed for wl in wls]
+    ref = singles(api, workloads, wls, seeds, [0] * 5)
+    ctxs = []
+    for wl in wls:
+        c = api.Context(0)
+        workloads.load(c, wl)
+        ctxs.append(c)
+    b = api.Batch(ctxs)
+    b.render(seeds, [wl.spp for wl in wls], graph=graph, stats=True)
+    for i, c in enumerate(ctxs):
+        assert np.array_equal(c.read_accum().view(np.uint32), ref[i].view(np.uint32)), f"scene {i} differs"
+    assert b.stats()["samples"] == sum(wl.width * wl.height * wl.spp for wl in wls)
+    seeds2 = [s + 11 for s in seeds]
+    for c in ctxs:
+        c.reset_accum()
+    b.render(seeds2, [wl.spp for wl in wls], graph=graph)
+    ref2 = singles(api, workloads, wls, seeds2, [0] * 5)
+    for i, c in enumerate(ctxs):
+        assert np.array_equal(c.read_accum().view(np.uint32), ref2[i].view(np.uint32)), f"second render scene {i}"
+    b.close()
+    for c in ctxs:
+        c.close()
+
+
 def test_batch_rejects_bad_input():
     from voxeltoy_amd import api
     c = api.Context(0)

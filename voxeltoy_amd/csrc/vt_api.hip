@@ -37,6 +37,10 @@ int fail(int code, const std::string& msg)
 #define VT_DEFAULT_DDA_BUDGET 32
 #define VT_DEFAULT_REFILL_MIN 8
 #define VT_DEFAULT_REFILL_CHUNK 256
+/* paths from which a scene of a batch gets its own launches */
+#ifndef VT_BATCH_SOLO_PATHS
+#define VT_BATCH_SOLO_PATHS (1ull << 22)
+#endif
 
 /* On failure the runtime's last-error state is cleared too, so a failed
  * call is not reported again by the next launch check. */
@@ -843,6 +847,7 @@ struct vt_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<uint64_t> graph_key;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> join;         /* per scene: end of its own launches on its context stream */
     bool timed = false, stats_valid = false;
 };
 
@@ -886,6 +891,9 @@ static void batch_release(vt_batch* b)
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     b->ev0 = b->ev1 = nullptr;
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    b->join.clear();
     (void)hipGetLastError();
     for (vt_context* c : b->ctxs) c->batches.erase(std::remove(c->batches.begin(), c->batches.end(), b), c->batches.end());
     b->ctxs.clear();
@@ -908,10 +916,13 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
     vt_context* L = b->lead;
     VT_HIP(hipSetDevice(L->device));
     const bool stats = (flags & 2) != 0, graph = (flags & 1) != 0;
-    uint64_t* pb = b->h_base.data();
-    uint64_t* xb = pb + n + 1;
-    pb[0] = xb[0] = 0;
-    int max_bounces = 0;
+    /* Scenes with at least VT_BATCH_SOLO_PATHS paths fill the GPU by
+     * themselves: they get the single-scene launches (block bits in LDS,
+     * the grid as scalars) on their own context streams, concurrently with
+     * each other and with one batched launch of all other scenes on the
+     * lead stream, where merging saves their many small launches. */
+    std::vector<std::pair<size_t, vt::TraceArgs>> solo;
+    std::vector<size_t> merged;
     for (size_t i = 0; i < n; ++i) {
         vt_context* c = b->ctxs[i];
         if (!c->has_volume) return fail(VT_ERR_STATE, "batch scene without a volume");
@@ -922,35 +933,74 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
         a.count = counts[i];
         a.rng_frame_max = rng_frame_max;
         a.stats = stats ? b->d_stats : nullptr;
-        b->h_tab[i] = a;
+        const uint64_t paths = (uint64_t)c->W * (uint64_t)c->H * counts[i];
+        if (paths == 0) continue;
+        if (paths >= VT_BATCH_SOLO_PATHS) {
+            solo.emplace_back(i, a);
+        } else {
+            b->h_tab[merged.size()] = a;
+            merged.push_back(i);
+        }
+    }
+    const size_t m = merged.size();
+    uint64_t* pb = b->h_base.data();
+    uint64_t* xb = pb + m + 1;
+    pb[0] = xb[0] = 0;
+    int max_bounces = 0;
+    for (size_t k = 0; k < m; ++k) {
+        vt_context* c = b->ctxs[merged[k]];
         const uint64_t WH = (uint64_t)c->W * (uint64_t)c->H;
-        pb[i + 1] = pb[i] + WH * counts[i];
-        xb[i + 1] = xb[i] + WH;
+        pb[k + 1] = pb[k] + WH * counts[merged[k]];
+        xb[k + 1] = xb[k] + WH;
         max_bounces = std::max(max_bounces, c->st.max_bounces);
     }
-    const uint64_t total = pb[n], pixels = xb[n];
-    if (total == 0) return VT_OK;
+    const uint64_t total = pb[m], pixels = xb[m];
+    if (total == 0 && solo.empty()) return VT_OK;
     /* uploads and clears still queued on the scenes' own streams */
     for (vt_context* c : b->ctxs) VT_HIP(hipStreamSynchronize(c->stream));
     if (total > (1ull << 40)) return fail(VT_ERR_LIMIT, "batch too large");
     /* the previous launch may still read the table */
     VT_HIP(hipStreamSynchronize(L->stream));
-    VT_HIP(hipMemcpy(b->d_tab, b->h_tab.data(), sizeof(vt::TraceArgs) * n, hipMemcpyHostToDevice));
-    VT_HIP(hipMemcpy(b->d_base, b->h_base.data(), sizeof(uint64_t) * b->h_base.size(), hipMemcpyHostToDevice));
-    int r = alloc_paths(b->wf, b->wf_alloc, std::min<uint64_t>(total, L->max_paths));
-    if (r != VT_OK) return r;
+    if (m) {
+        VT_HIP(hipMemcpy(b->d_tab, b->h_tab.data(), sizeof(vt::TraceArgs) * m, hipMemcpyHostToDevice));
+        VT_HIP(hipMemcpy(b->d_base, b->h_base.data(), sizeof(uint64_t) * 2 * (m + 1), hipMemcpyHostToDevice));
+    }
+    if (m) {
+        const int r = alloc_paths(b->wf, b->wf_alloc, std::min<uint64_t>(total, L->max_paths));
+        if (r != VT_OK) return r;
+    }
+    if (b->join.size() != n) {
+        for (hipEvent_t e : b->join)
+            if (e) (void)hipEventDestroy(e);
+        b->join.assign(n, nullptr);
+        for (hipEvent_t& e : b->join) VT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    for (const auto& sa : solo) {
+        vt_context* c = b->ctxs[sa.first];
+        const uint64_t paths = (uint64_t)sa.second.W * (uint64_t)sa.second.H * sa.second.count;
+        const int r = ensure_wavefront(c, std::min<uint64_t>(paths, c->max_paths));
+        if (r != VT_OK) return r;
+    }
     vt::BatchTable B;
     B.tab = b->d_tab;
     B.path_base = b->d_base;
-    B.pix_base = b->d_base + n + 1;
-    B.n = (uint32_t)n;
+    B.pix_base = b->d_base + m + 1;
+    B.n = (uint32_t)m;
     B.stats = stats ? b->d_stats : nullptr;
     vt::TraceArgs a0 = trace_args(L);
     if (stats) VT_HIP(vt::launch_reset_counter(L->d_tile, b->d_stats, L->stream));
-    if (graph) {
-        const std::vector<uint64_t> key = { (uint64_t)n, total, pixels, b->wf.cap, (uint64_t)max_bounces,
-                                            (uint64_t)stats, (uint64_t)a0.dda_budget, (uint64_t)a0.refill_min,
-                                            (uint64_t)a0.refill_chunk, (uint64_t)(uintptr_t)b->wf.s[0].o };
+    VT_HIP(hipEventRecord(b->ev0, L->stream));
+    for (const auto& sa : solo) {
+        vt_context* c = b->ctxs[sa.first];
+        if (c->stream != L->stream) VT_HIP(hipStreamWaitEvent(c->stream, b->ev0, 0));
+        VT_HIP(vt::launch_render_wavefront(sa.second, c->wf, c->cus, c->stream, stats));
+        VT_HIP(hipEventRecord(b->join[sa.first], c->stream));
+    }
+    if (m && graph) {
+        std::vector<uint64_t> key = { (uint64_t)m, total, pixels, b->wf.cap, (uint64_t)max_bounces,
+                                      (uint64_t)stats, (uint64_t)a0.dda_budget, (uint64_t)a0.refill_min,
+                                      (uint64_t)a0.refill_chunk, (uint64_t)(uintptr_t)b->wf.s[0].o };
+        key.insert(key.end(), merged.begin(), merged.end());
         if (!b->exec || key != b->graph_key) {
             if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
             hipGraph_t g = nullptr;
@@ -971,14 +1021,12 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
             }
             b->graph_key = key;
         }
-        VT_HIP(hipEventRecord(b->ev0, L->stream));
         VT_HIP(hipGraphLaunch(b->exec, L->stream));
-        VT_HIP(hipEventRecord(b->ev1, L->stream));
-    } else {
-        VT_HIP(hipEventRecord(b->ev0, L->stream));
+    } else if (m) {
         VT_HIP(vt::launch_render_batch(a0, B, b->wf, total, pixels, max_bounces, L->cus, L->stream, stats));
-        VT_HIP(hipEventRecord(b->ev1, L->stream));
     }
+    for (const auto& sa : solo) VT_HIP(hipStreamWaitEvent(L->stream, b->join[sa.first], 0));
+    VT_HIP(hipEventRecord(b->ev1, L->stream));
     /* each context's later reads and renders run on its own stream */
     VT_HIP(hipStreamSynchronize(L->stream));
     b->timed = true;
