@@ -235,6 +235,49 @@ def test_render_ragged_grid_bit_exact(ctx, orc, shape, angle):
         assert stats[k] == cnt[k], k
 
 
+def test_render_large_grid_bit_exact(ctx, orc):
+    """A grid whose padded block bits exceed the LDS a workgroup may stage
+    (the march then reads them from global memory): ragged 452 x 437 x 458,
+    terrain, boxes, metal and emissive materials."""
+    from voxeltoy_amd import api, scenes, workloads
+    nz, ny, nx = 458, 437, 452
+    rng = np.random.default_rng(21)
+    h = scenes.value_noise_2d(rng, max(nx, nz), 7)[:nz, :nx] * 0.3 * ny + 0.04 * ny
+    occ = np.arange(ny, dtype=np.float32)[None, :, None] < h[:, None, :].astype(np.float32)
+    for k in range(6):
+        s = rng.integers(20, 60, size=3)
+        p = [int(rng.integers(0, nz - s[2])), int(rng.integers(0, ny // 2)), int(rng.integers(0, nx - s[0]))]
+        occ[p[0]:p[0] + s[2], p[1]:p[1] + s[1], p[2]:p[2] + s[0]] = True
+    recs = [scenes.lambert((0, 0, 0), (0.6, 0.5, 0.4)), scenes.metal((0, 0, 0), (0.8, 0.8, 0.9), 0.3),
+            scenes.lambert((4.0, 3.5, 3.0), (0.5, 0.5, 0.5)), scenes.lambert((0, 0, 0), (0.3, 0.6, 0.3))]
+    mat, offs = scenes.build_materials(recs)
+    id_offset = np.full(256, -1, np.int32)
+    id_offset[1:5] = offs
+    zz = np.arange(nz, dtype=np.uint16)[:, None, None]
+    xx = np.arange(nx, dtype=np.uint16)[None, None, :]
+    base = np.array([1, 2, 4], np.uint8)[(zz * 7 + xx * 3) % 3]           # lambert, metal, lambert
+    ids = np.where(occ, np.where((zz * 5 + xx * 11) % 97 == 0, np.uint8(3), base), np.uint8(0))
+    sc = scenes.Scene(ids.astype(np.uint8), id_offset, mat)
+    sc.emissive = api.prune_emissive(sc.offsets(), scenes.emissive_list(sc.ids, id_offset, mat))
+    assert len(sc.emissive) > 0
+    bmin, bmax = workloads.reference_bounds(sc.shape)
+    cam = scenes.orbit_camera(bmin, bmax, math.radians(105), math.radians(35), 96, 80)
+    wl = workloads.Workload("large", sc, cam, 96, 80, spp=2, bounces=3, bmin=bmin, bmax=bmax,
+                            env_top=(1.5, 1.4, 1.2), env_bottom=(0.3, 0.3, 0.35))
+    workloads.load(ctx, wl)
+    ctx.set_render_mode(1, 0)
+    ctx.set_stats(True)
+    ctx.render(wl.seed, 0, wl.spp)
+    got = ctx.read_accum()
+    stats = ctx.stats()
+    ctx.set_stats(False)
+    ref, cnt = oracle_for(orc, wl).render(wl.width, wl.height, wl.seed, 0, wl.spp)
+    same = (got.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), f"{(~same).sum()} values differ"
+    for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
+        assert stats[k] == cnt[k], k
+
+
 def test_render_settings_variants_bit_exact(ctx, orc):
     """Wireframe overlay, selected voxel, exposure/gamma and rng frame clamp."""
     from voxeltoy_amd import workloads
