@@ -362,7 +362,7 @@ void free_paths(vt::WfPaths& P, uint64_t& alloc)
 {
     for (vt::PathBuf& B : P.s) {
         dfree(B.o); dfree(B.nd); dfree(B.thr); dfree(B.rad); dfree(B.dir); dfree(B.fac);
-        dfree(B.tgt); dfree(B.meta); dfree(B.rng);
+        dfree(B.meta); dfree(B.rng);
     }
     dfree(P.hit); dfree(P.smp);
     dfree(P.qa0); dfree(P.qb0); dfree(P.qa1); dfree(P.qb1); dfree(P.cnt);
@@ -381,7 +381,7 @@ int alloc_paths(vt::WfPaths& P, uint64_t& alloc, uint64_t slots)
     for (vt::PathBuf& B : P.s) {
         VT_HIP(hipMalloc(&B.o, n * 16)); VT_HIP(hipMalloc(&B.nd, n * 16)); VT_HIP(hipMalloc(&B.thr, n * 16));
         VT_HIP(hipMalloc(&B.rad, n * 16)); VT_HIP(hipMalloc(&B.dir, n * 16)); VT_HIP(hipMalloc(&B.fac, n * 16));
-        VT_HIP(hipMalloc(&B.tgt, n * 16)); VT_HIP(hipMalloc(&B.meta, n * 16)); VT_HIP(hipMalloc(&B.rng, n * 8));
+        VT_HIP(hipMalloc(&B.meta, n * 16)); VT_HIP(hipMalloc(&B.rng, n * 8));
     }
     /* queues and hits: two entries (shadow and continuation ray) per path */
     VT_HIP(hipMalloc(&P.hit, 2 * n * 16));

@@ -58,14 +58,13 @@ struct PathState {
  * kernels read and write them contiguously (a surviving path moves to its
  * position in the next queue). */
 struct PathBuf {
-    float4* o;        /* ray_o (or basis origin after generate / shadow).xyz, bsdf_pdf */
-    float4* nd;       /* continuation direction (camera direction after generate) */
-    float4* thr;      /* throughput */
-    float4* rad;      /* radiance */
-    float4* dir;      /* pending light-sample contribution */
-    float4* fac;      /* continuation factor */
-    int4* tgt;        /* emissive target voxel, light_is_voxel */
-    uint4* meta;      /* pixel, sample slot of the chunk, bounces, scene or training slot + 1 */
+    float4* o;        /* ray_o (or basis origin after generate).xyz, bsdf_pdf */
+    float4* nd;       /* continuation direction (camera direction after generate), pixel (bits) */
+    float4* thr;      /* throughput, sample slot of the chunk (bits) */
+    float4* rad;      /* radiance, scene or training slot + 1 (bits) */
+    float4* dir;      /* pending light-sample contribution, emissive target x | y << 16 (bits) */
+    float4* fac;      /* continuation factor, target z | light_is_voxel << 15 | bounces << 16 (bits) */
+    uint4* meta;      /* after generate only: pixel, sample slot, bounces (0), scene or training slot + 1 */
     uint64_t* rng;    /* PCG32 state (increment = 2*pixel+1) */
 };
 

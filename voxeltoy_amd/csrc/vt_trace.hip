@@ -342,14 +342,16 @@ __device__ __forceinline__ void load_phase(const PathBuf& B, uint32_t j, PathSta
     S.direct = S.factor = S.next_d = mk(0.0f, 0.0f, 0.0f);
     S.ex = S.ey = S.ez = S.light_is_voxel = 0;
     S.bsdf_pdf = 0.0f;
-    me = B.meta[j];
-    S.bounces = (int32_t)me.z;
     S.rng.state = B.rng[j];
-    S.rng.inc = ((uint64_t)me.x << 1u) | 1u;
     const float4 o = B.o[j], nd = B.nd[j];
     if (PHASE == PH_SHADOW) {
+        /* the state store_phase packs: pixel, slot and tag in the w of nd,
+         * thr, rad; light target and bounce count in the w of dir, fac */
         const float4 th = B.thr[j], rd = B.rad[j], di = B.dir[j], fa = B.fac[j];
-        const int4 tg = B.tgt[j];
+        const uint32_t exy = __float_as_uint(di.w), ezb = __float_as_uint(fa.w);
+        me = make_uint4(__float_as_uint(nd.w), __float_as_uint(th.w), ezb >> 16, __float_as_uint(rd.w));
+        S.bounces = (int32_t)me.z;
+        S.rng.inc = ((uint64_t)me.x << 1u) | 1u;
         S.ray_o = mk(o.x, o.y, o.z);
         S.bsdf_pdf = o.w;
         S.next_d = mk(nd.x, nd.y, nd.z);
@@ -359,9 +361,13 @@ __device__ __forceinline__ void load_phase(const PathBuf& B, uint32_t j, PathSta
         S.radiance = mk(rd.x, rd.y, rd.z);
         S.direct = mk(di.x, di.y, di.z);
         S.factor = mk(fa.x, fa.y, fa.z);
-        S.ex = tg.x; S.ey = tg.y; S.ez = tg.z; S.light_is_voxel = tg.w;
+        S.ex = (int32_t)(exy & 0xffffu); S.ey = (int32_t)(exy >> 16);
+        S.ez = (int32_t)(ezb & 0x7fffu); S.light_is_voxel = (int32_t)((ezb >> 15) & 1u);
         return;
     }
+    me = B.meta[j];
+    S.bounces = (int32_t)me.z;
+    S.rng.inc = ((uint64_t)me.x << 1u) | 1u;
     if (PHASE == PH_PRIMARY) {
         S.basis_o = mk(o.x, o.y, o.z);
         S.ray_d = mk(nd.x, nd.y, nd.z);
@@ -379,28 +385,24 @@ __device__ __forceinline__ void load_phase(const PathBuf& B, uint32_t j, PathSta
     }
 }
 
-/* Store after a shade step that continues the path (the next phase is
- * SHADOW after PRIMARY/CONT, CONT after SHADOW) at position k of the next
- * queue's buffer. */
-template <int PHASE>
-__device__ __forceinline__ void store_phase(const PathBuf& N, uint32_t k, const PathState& S, uint4 me)
+/* Store after a shade step that continues the path (next: the result of
+ * its shadow ray, then its continuation) at position k of the next queue's
+ * buffer.  The path's metadata rides in the w components: nd.w pixel,
+ * thr.w sample slot, rad.w scene or training tag, dir.w ex | ey << 16,
+ * fac.w ez | light_is_voxel << 15 | bounces << 16 (voxel coordinates below
+ * 2^15). */
+__device__ __forceinline__ void store_cont(const PathBuf& N, uint32_t k, const PathState& S, uint4 me)
 {
-    N.rad[k] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, 0.0f);
-    N.thr[k] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
-    N.rng[k] = S.rng.state;
-    me.z = (uint32_t)S.bounces;
-    N.meta[k] = me;
-    if (PHASE == PH_SHADOW) {
-        /* next: CONT from basis_o (= ray_o) along ray_d (= next_d) */
-        N.o[k] = make_float4(S.basis_o.x, S.basis_o.y, S.basis_o.z, S.bsdf_pdf);
-        N.nd[k] = make_float4(S.ray_d.x, S.ray_d.y, S.ray_d.z, 0.0f);
-        return;
-    }
+    const uint32_t exy = ((uint32_t)S.ex & 0xffffu) | ((uint32_t)S.ey << 16);
+    const uint32_t ezb = ((uint32_t)S.ez & 0x7fffu) | ((uint32_t)(S.light_is_voxel != 0) << 15) |
+                         ((uint32_t)S.bounces << 16);
     N.o[k] = make_float4(S.ray_o.x, S.ray_o.y, S.ray_o.z, S.bsdf_pdf);
-    N.nd[k] = make_float4(S.next_d.x, S.next_d.y, S.next_d.z, 0.0f);
-    N.dir[k] = make_float4(S.direct.x, S.direct.y, S.direct.z, 0.0f);
-    N.fac[k] = make_float4(S.factor.x, S.factor.y, S.factor.z, 0.0f);
-    N.tgt[k] = make_int4(S.ex, S.ey, S.ez, S.light_is_voxel);
+    N.nd[k] = make_float4(S.next_d.x, S.next_d.y, S.next_d.z, __uint_as_float(me.x));
+    N.thr[k] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, __uint_as_float(me.y));
+    N.rad[k] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, __uint_as_float(me.w));
+    N.dir[k] = make_float4(S.direct.x, S.direct.y, S.direct.z, __uint_as_float(exy));
+    N.fac[k] = make_float4(S.factor.x, S.factor.y, S.factor.z, __uint_as_float(ezb));
+    N.rng[k] = S.rng.state;
 }
 
 template <bool STATS>
@@ -818,7 +820,7 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
             }
         }
         const uint32_t k = block_append2<WF_SHADE_THREADS>(keep, me.y, sid, ro, rd, rn, qao, qbo, &P.cnt[sel ^ 1]);
-        if (keep) store_phase<PH_CONT>(Bn, k, S, me);
+        if (keep) store_cont(Bn, k, S, me);
     }
     flush_counters<STATS>(BATCH ? B.stats : a0.stats, cnt);
 }
