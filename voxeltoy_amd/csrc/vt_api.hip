@@ -384,7 +384,7 @@ int alloc_paths(vt::WfPaths& P, uint64_t& alloc, uint64_t slots)
         VT_HIP(hipMalloc(&B.meta, n * 16)); VT_HIP(hipMalloc(&B.rng, n * 8));
     }
     /* queues and hits: two entries (shadow and continuation ray) per path */
-    VT_HIP(hipMalloc(&P.hit, 2 * n * 16));
+    VT_HIP(hipMalloc(&P.hit, 2 * n * 8));
     VT_HIP(hipMalloc(&P.smp, n * 16));
     VT_HIP(hipMalloc(&P.qa0, 2 * n * 16)); VT_HIP(hipMalloc(&P.qb0, 2 * n * 16));
     VT_HIP(hipMalloc(&P.qa1, 2 * n * 16)); VT_HIP(hipMalloc(&P.qb1, 2 * n * 16));

@@ -68,9 +68,23 @@ struct PathBuf {
     uint64_t* rng;    /* PCG32 state (increment = 2*pixel+1) */
 };
 
+/* Traversal result in 8 bytes: voxel coordinates + 1 (they lie in [-1, n]
+ * for n <= 4096) in 16-bit fields, kind in the top half of the second word. */
+__device__ __forceinline__ uint2 pack_hit(int32_t x, int32_t y, int32_t z, int32_t kind)
+{
+    return make_uint2((uint32_t)(x + 1) | ((uint32_t)(y + 1) << 16), (uint32_t)(z + 1) | ((uint32_t)kind << 16));
+}
+__device__ __forceinline__ int32_t unpack_hit(uint2 r, Hit& h)
+{
+    h.x = (int32_t)(r.x & 0xffffu) - 1;
+    h.y = (int32_t)(r.x >> 16) - 1;
+    h.z = (int32_t)(r.y & 0xffffu) - 1;
+    return (int32_t)(r.y >> 16);
+}
+
 struct WfPaths {
     PathBuf s[2];     /* state of the paths in queue 0 / queue 1 */
-    int4* hit;        /* traversal result per queue entry: x, y, z, kind << 8 */
+    uint2* hit;       /* traversal result per queue entry (pack_hit): x, y, z (each + 1), kind */
     float4* smp;      /* finished tonemapped sample of each path of the chunk */
     float4* qa0;      /* ray queue 0 (2 x cap entries: a path's shadow and continuation rays at 2k, 2k + 1):
                          origin.xyz moved by sign(d)*0.001, sample slot (as bits) */

@@ -655,7 +655,7 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
                     if (D.init(G, unpack_ray(o, d))) {
                         has = true;
                     } else {
-                        P.hit[j] = make_int4(0, 0, 0, KIND_MISS << 8);
+                        P.hit[j] = pack_hit(0, 0, 0, KIND_MISS);
                     }
                 }
             }
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
             if (status) {
                 /* kind and voxel only: the shade kernel reads the material slot */
                 const int kind = status == 1 ? KIND_VOXEL : (D.y < 0 ? KIND_GROUND : KIND_MISS);
-                P.hit[jq] = make_int4(D.x, D.y, D.z, kind << 8);
+                P.hit[jq] = pack_hit(D.x, D.y, D.z, kind);
                 has = false;
             }
         }
@@ -771,18 +771,22 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
             const uint32_t slot1 = CACHE ? me.w : 0u;   /* training slot + 1 */
             Hit h;
             h.steps = 0;
+            /* the shadow and continuation results of a path are adjacent: one 16-byte load */
+            uint2 hv;
             if (PHASE != PH_PRIMARY) {
+                const uint4 hp = ((const uint4*)P.hit)[j];
+                hv = make_uint2(hp.z, hp.w);
                 /* the shadow ray's result (never ends the path) */
-                const int4 hs = P.hit[2 * j];
-                h.x = hs.x; h.y = hs.y; h.z = hs.z; h.slot = 0u;   /* visibility needs no material */
-                shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, hs.w >> 8, cnt, -1, vraw, nullptr);
+                const int ks = unpack_hit(make_uint2(hp.x, hp.y), h);
+                h.slot = 0u;   /* visibility needs no material */
+                shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, ks, cnt, -1, vraw, nullptr);
+            } else {
+                hv = P.hit[j];
             }
-            const int4 hv = P.hit[PHASE == PH_PRIMARY ? j : 2 * j + 1];
-            const int kind = hv.w >> 8;
-            h.x = hv.x; h.y = hv.y; h.z = hv.z;
+            const int kind = unpack_hit(hv, h);
             /* the hit voxel's material slot is read here, where occupancy
              * hides the load, not by the traversal (ground: slot 0) */
-            h.slot = kind == KIND_VOXEL ? fetch_slot(a.g, bits, hv.x, hv.y, hv.z) : 0u;
+            h.slot = kind == KIND_VOXEL ? fetch_slot(a.g, bits, h.x, h.y, h.z) : 0u;
             bool vrec = false;
             const int st = shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, kind, cnt,
                                                     slot1 ? C.train_bounce : C.cache_bounce, vraw, &vrec);
