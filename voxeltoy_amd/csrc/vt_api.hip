@@ -365,7 +365,7 @@ void free_paths(vt::WfPaths& P, uint64_t& alloc)
         dfree(B.meta); dfree(B.rng);
     }
     dfree(P.hit); dfree(P.smp);
-    dfree(P.qa0); dfree(P.qb0); dfree(P.qa1); dfree(P.qb1); dfree(P.cnt);
+    dfree(P.qa0); dfree(P.qb0); dfree(P.qt0); dfree(P.qa1); dfree(P.qb1); dfree(P.qt1); dfree(P.cnt);
     P.cap = 0;
     alloc = 0;
 }
@@ -386,8 +386,8 @@ int alloc_paths(vt::WfPaths& P, uint64_t& alloc, uint64_t slots)
     /* queues and hits: two entries (shadow and continuation ray) per path */
     VT_HIP(hipMalloc(&P.hit, 2 * n * 8));
     VT_HIP(hipMalloc(&P.smp, n * 16));
-    VT_HIP(hipMalloc(&P.qa0, 2 * n * 16)); VT_HIP(hipMalloc(&P.qb0, 2 * n * 16));
-    VT_HIP(hipMalloc(&P.qa1, 2 * n * 16)); VT_HIP(hipMalloc(&P.qb1, 2 * n * 16));
+    VT_HIP(hipMalloc(&P.qa0, 2 * n * 16)); VT_HIP(hipMalloc(&P.qb0, 2 * n * 8)); VT_HIP(hipMalloc(&P.qt0, 2 * n * 4));
+    VT_HIP(hipMalloc(&P.qa1, 2 * n * 16)); VT_HIP(hipMalloc(&P.qb1, 2 * n * 8)); VT_HIP(hipMalloc(&P.qt1, 2 * n * 4));
     VT_HIP(hipMalloc(&P.cnt, 16));
     P.cap = slots;
     alloc = slots;

@@ -222,24 +222,27 @@ __device__ __forceinline__ PreRay prepare_ray(f3 o, f3 d)
     return r;
 }
 
-/* Queue encoding of a prepared ray: (origin, path id) and (1/d, tag with
- * the three signs + 1 in bits 26..31; tags stay below 2^26). */
-__device__ __forceinline__ uint32_t pack_signs(const PreRay& r, uint32_t tag)
+/* Queue encoding of a prepared ray in 24 bytes: (origin, 1/d.x) and
+ * (1/d.y, 1/d.z).  The step signs are not stored: the clamped direction
+ * component is never zero, so its sign is the sign bit of its reciprocal
+ * (also for a reciprocal of +-0 from an infinite component), and a NaN
+ * component (sign 0 in prepare_ray) has a NaN reciprocal. */
+__device__ __forceinline__ int32_t step_sign(float inv)
 {
-    return tag | ((uint32_t)(r.sx + 1) << 26) | ((uint32_t)(r.sy + 1) << 28) | ((uint32_t)(r.sz + 1) << 30);
+    return inv != inv ? 0 : ((__float_as_uint(inv) >> 31) ? -1 : 1);
 }
-__device__ __forceinline__ PreRay unpack_ray(float4 qa, float4 qb)
+__device__ __forceinline__ float4 queue_a(const PreRay& r) { return make_float4(r.o.x, r.o.y, r.o.z, r.inv.x); }
+__device__ __forceinline__ float2 queue_b(const PreRay& r) { return make_float2(r.inv.y, r.inv.z); }
+__device__ __forceinline__ PreRay unpack_ray(float4 qa, float2 qb)
 {
     PreRay r;
     r.o = mk(qa.x, qa.y, qa.z);
-    r.inv = mk(qb.x, qb.y, qb.z);
-    const uint32_t w = __float_as_uint(qb.w);
-    r.sx = (int32_t)((w >> 26) & 3u) - 1;
-    r.sy = (int32_t)((w >> 28) & 3u) - 1;
-    r.sz = (int32_t)(w >> 30) - 1;
+    r.inv = mk(qa.w, qb.x, qb.y);
+    r.sx = step_sign(r.inv.x);
+    r.sy = step_sign(r.inv.y);
+    r.sz = step_sign(r.inv.z);
     return r;
 }
-__device__ __forceinline__ uint32_t queue_tag(float4 qb) { return __float_as_uint(qb.w) & 0x03ffffffu; }
 
 /* The march as resumable state, for traversal kernels whose lanes take a
  * new ray as soon as theirs finishes.  init() returns false when the start

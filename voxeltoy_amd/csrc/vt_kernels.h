@@ -87,10 +87,12 @@ struct WfPaths {
     uint2* hit;       /* traversal result per queue entry (pack_hit): x, y, z (each + 1), kind */
     float4* smp;      /* finished tonemapped sample of each path of the chunk */
     float4* qa0;      /* ray queue 0 (2 x cap entries: a path's shadow and continuation rays at 2k, 2k + 1):
-                         origin.xyz moved by sign(d)*0.001, sample slot (as bits) */
-    float4* qb0;      /*              1/clamped direction, tag | direction signs << 26 (bits) */
+                         origin.xyz moved by sign(d)*0.001, 1/clamped d.x */
+    float2* qb0;      /*              1/clamped d.y, 1/clamped d.z */
+    uint32_t* qt0;    /*              scene of the ray (batch launches only) */
     float4* qa1;      /* ray queue 1 */
-    float4* qb1;
+    float2* qb1;
+    uint32_t* qt1;
     uint32_t* cnt;    /* queue lengths: cnt[0] for q0, cnt[1] for q1 */
     uint64_t cap;     /* slots */
 };

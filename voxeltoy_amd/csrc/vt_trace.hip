@@ -424,14 +424,14 @@ __device__ __forceinline__ void flush_counters(unsigned long long* stats, const 
 
 /* Append the ray (o, d) for the threads with `keep` to a ray queue (length
  * counter *n) and return each such thread's position; the queue stores the
- * rays themselves, already prepared for the march (prepare_ray), the path's
- * sample slot `id` and its scene index `tag` in a batch, so the traversal
- * reads them in queue order.  One atomic per workgroup of NT
+ * rays themselves, already prepared for the march (prepare_ray, 24 bytes in
+ * qa and qb) and, when qt is given (batch launches), the scene index `tag`,
+ * so the traversal reads them in queue order.  One atomic per workgroup of NT
  * threads: every queue counter is a single device address that all XCDs
  * contend for.  Every thread of the workgroup must call it. */
 template <int NT>
-__device__ __forceinline__ uint32_t block_append(bool keep, uint32_t id, uint32_t tag, f3 o, f3 d, float4* qa,
-                                                 float4* qb, uint32_t* n)
+__device__ __forceinline__ uint32_t block_append(bool keep, uint32_t tag, f3 o, f3 d, float4* qa, float2* qb,
+                                                 uint32_t* qt, uint32_t* n)
 {
     __shared__ uint32_t wsum[NT / 64];
     __shared__ uint32_t bbase;
@@ -448,8 +448,9 @@ __device__ __forceinline__ uint32_t block_append(bool keep, uint32_t id, uint32_
     const uint32_t j = bbase + wsum[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
     if (keep) {
         const PreRay r = prepare_ray(o, d);
-        qa[j] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(id));
-        qb[j] = make_float4(r.inv.x, r.inv.y, r.inv.z, __uint_as_float(pack_signs(r, tag)));
+        qa[j] = queue_a(r);
+        qb[j] = queue_b(r);
+        if (qt) qt[j] = tag;
     }
     __syncthreads();
     return j;
@@ -459,8 +460,8 @@ __device__ __forceinline__ uint32_t block_append(bool keep, uint32_t id, uint32_
  * shadow ray along d0 and a continuation ray along d1): they take queue
  * entries 2k and 2k + 1, k being the returned position. */
 template <int NT>
-__device__ __forceinline__ uint32_t block_append2(bool keep, uint32_t id, uint32_t tag, f3 o, f3 d0, f3 d1,
-                                                  float4* qa, float4* qb, uint32_t* n)
+__device__ __forceinline__ uint32_t block_append2(bool keep, uint32_t tag, f3 o, f3 d0, f3 d1, float4* qa,
+                                                  float2* qb, uint32_t* qt, uint32_t* n)
 {
     __shared__ uint32_t wsum2[NT / 64];
     __shared__ uint32_t bbase2;
@@ -477,10 +478,11 @@ __device__ __forceinline__ uint32_t block_append2(bool keep, uint32_t id, uint32
     const uint32_t k = bbase2 + wsum2[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
     if (keep) {
         const PreRay r0 = prepare_ray(o, d0), r1 = prepare_ray(o, d1);
-        qa[2 * k] = make_float4(r0.o.x, r0.o.y, r0.o.z, __uint_as_float(id));
-        qb[2 * k] = make_float4(r0.inv.x, r0.inv.y, r0.inv.z, __uint_as_float(pack_signs(r0, tag)));
-        qa[2 * k + 1] = make_float4(r1.o.x, r1.o.y, r1.o.z, __uint_as_float(id));
-        qb[2 * k + 1] = make_float4(r1.inv.x, r1.inv.y, r1.inv.z, __uint_as_float(pack_signs(r1, tag)));
+        qa[2 * k] = queue_a(r0);
+        qa[2 * k + 1] = queue_a(r1);
+        const float2 b0 = queue_b(r0), b1 = queue_b(r1);
+        ((float4*)qb)[k] = make_float4(b0.x, b0.y, b1.x, b1.y);
+        if (qt) qt[2 * k] = qt[2 * k + 1] = tag;
     }
     __syncthreads();
     return k;
@@ -569,7 +571,8 @@ __global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceA
             gpix = pix;
             gtag = tag;
         }
-        const uint32_t q = block_append<WF_GENERATE_THREADS>(keep, i, sid, ro, rd, P.qa0, P.qb0, &P.cnt[0]);
+        const uint32_t q =
+            block_append<WF_GENERATE_THREADS>(keep, sid, ro, rd, P.qa0, P.qb0, BATCH ? P.qt0 : nullptr, &P.cnt[0]);
         if (keep) store_generated(P.s[0], q, G, gpix, i, gtag);
     }
     flush_counters<STATS>(BATCH ? B.stats : a0.stats, cnt);
@@ -599,7 +602,8 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
         __syncthreads();
     }
     const float4* qa = sel ? P.qa1 : P.qa0;
-    const float4* qb = sel ? P.qb1 : P.qb0;
+    const float2* qb = sel ? P.qb1 : P.qb0;
+    const uint32_t* qt = sel ? P.qt1 : P.qt0;
     const uint32_t n = P.cnt[sel];
     uint32_t* head = &P.cnt[2 + sel];
     if (blockIdx.x == 0 && threadIdx.x == 0) P.cnt[sel ^ 1] = 0u;
@@ -646,12 +650,13 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
                 if (p < avail) j = c_pos + p;
                 else if (grab && nb + (p - avail) < n) j = nb + (p - avail);
                 if (j != 0xffffffffu) {
-                    float4 o = qa[j], d = qb[j];
-                    /* both halves of the entry in flight together: one
+                    float4 o = qa[j];
+                    float2 d = qb[j];
+                    /* both parts of the entry in flight together: one
                      * memory latency per refill instead of two */
-                    asm volatile("" : "+v"(o.x), "+v"(o.y), "+v"(o.z), "+v"(d.x), "+v"(d.y), "+v"(d.z), "+v"(d.w));
+                    asm volatile("" : "+v"(o.x), "+v"(o.y), "+v"(o.z), "+v"(o.w), "+v"(d.x), "+v"(d.y));
                     jq = j;
-                    if (BATCH) gl = B.tab[queue_tag(d)].g;
+                    if (BATCH) gl = B.tab[qt[j]].g;
                     if (D.init(G, unpack_ray(o, d))) {
                         has = true;
                     } else {
@@ -747,7 +752,8 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
     const uint32_t* bits0 = nullptr;
     if (!BATCH) stage_lds<false>(a0, smem, recs0, cdfv0, bits0);
     float4* qao = sel ? P.qa0 : P.qa1;
-    float4* qbo = sel ? P.qb0 : P.qb1;
+    float2* qbo = sel ? P.qb0 : P.qb1;
+    uint32_t* qto = BATCH ? (sel ? P.qt0 : P.qt1) : nullptr;
     const PathBuf& Bc = P.s[sel];
     const PathBuf& Bn = P.s[sel ^ 1];
     const uint32_t n = PHASE == PH_PRIMARY ? P.cnt[sel] : P.cnt[sel] / 2u;
@@ -823,7 +829,7 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
                 C.q_thr[qi] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, 0.0f);
             }
         }
-        const uint32_t k = block_append2<WF_SHADE_THREADS>(keep, me.y, sid, ro, rd, rn, qao, qbo, &P.cnt[sel ^ 1]);
+        const uint32_t k = block_append2<WF_SHADE_THREADS>(keep, sid, ro, rd, rn, qao, qbo, qto, &P.cnt[sel ^ 1]);
         if (keep) store_cont(Bn, k, S, me);
     }
     flush_counters<STATS>(BATCH ? B.stats : a0.stats, cnt);
