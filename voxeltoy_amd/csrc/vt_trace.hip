@@ -326,16 +326,20 @@ __global__ __launch_bounds__(256) void render_kernel(TraceArgs a)
  *   CONT     <- o, nd, thr, rad, meta, rng -> o, nd, thr, rad, dir, fac, tgt, meta, rng
  * After SHADOW, basis_o = ray_o and ray_d = next_d, so CONT reads them from
  * o and nd. */
+/* A camera path's state: origin (w: pixel) and direction (w: sample
+ * slot); with TAG (batch and cache launches) also the metadata record, for
+ * the scene or training tag.  Bounces are 0. */
+template <bool TAG>
 __device__ __forceinline__ void store_generated(const PathBuf& N, uint32_t k, const PathState& S, uint32_t pix,
                                                 uint32_t slot, uint32_t tag)
 {
-    N.o[k] = make_float4(S.basis_o.x, S.basis_o.y, S.basis_o.z, 0.0f);
-    N.nd[k] = make_float4(S.ray_d.x, S.ray_d.y, S.ray_d.z, 0.0f);
-    N.meta[k] = make_uint4(pix, slot, (uint32_t)S.bounces, tag);
+    N.o[k] = make_float4(S.basis_o.x, S.basis_o.y, S.basis_o.z, __uint_as_float(pix));
+    N.nd[k] = make_float4(S.ray_d.x, S.ray_d.y, S.ray_d.z, __uint_as_float(slot));
+    if (TAG) N.meta[k] = make_uint4(pix, slot, (uint32_t)S.bounces, tag);
     N.rng[k] = S.rng.state;
 }
 
-template <int PHASE>
+template <int PHASE, bool TAG = false>
 __device__ __forceinline__ void load_phase(const PathBuf& B, uint32_t j, PathState& S, uint4& me)
 {
     S.phase = PHASE;
@@ -365,7 +369,8 @@ __device__ __forceinline__ void load_phase(const PathBuf& B, uint32_t j, PathSta
         S.ez = (int32_t)(ezb & 0x7fffu); S.light_is_voxel = (int32_t)((ezb >> 15) & 1u);
         return;
     }
-    me = B.meta[j];
+    if (TAG) me = B.meta[j];
+    else me = make_uint4(__float_as_uint(o.w), __float_as_uint(nd.w), 0u, 0u);
     S.bounces = (int32_t)me.z;
     S.rng.inc = ((uint64_t)me.x << 1u) | 1u;
     if (PHASE == PH_PRIMARY) {
@@ -573,7 +578,7 @@ __global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceA
         }
         const uint32_t q =
             block_append<WF_GENERATE_THREADS>(keep, sid, ro, rd, P.qa0, P.qb0, BATCH ? P.qt0 : nullptr, &P.cnt[0]);
-        if (keep) store_generated(P.s[0], q, G, gpix, i, gtag);
+        if (keep) store_generated<BATCH || CACHE>(P.s[0], q, G, gpix, i, gtag);
     }
     flush_counters<STATS>(BATCH ? B.stats : a0.stats, cnt);
 }
@@ -768,7 +773,7 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
         PathState S;
         uint4 me = make_uint4(0u, 0u, 0u, 0u);
         if (j < n) {
-            load_phase<PHASE == PH_PRIMARY ? PH_PRIMARY : PH_SHADOW>(Bc, j, S, me);
+            load_phase<PHASE == PH_PRIMARY ? PH_PRIMARY : PH_SHADOW, BATCH || CACHE>(Bc, j, S, me);
             if (BATCH) sid = me.w;
             const TraceArgs& a = BATCH ? B.tab[sid] : a0;
             const MatRec* recs = BATCH ? a.recs : recs0;
