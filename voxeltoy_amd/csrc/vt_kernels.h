@@ -60,10 +60,10 @@ struct PathState {
 struct PathBuf {
     float4* o;        /* ray_o (or basis origin after generate).xyz, bsdf_pdf */
     float4* nd;       /* continuation direction (camera direction after generate), pixel (bits) */
-    float4* thr;      /* throughput, sample slot of the chunk (bits) */
-    float4* rad;      /* radiance, scene or training slot + 1 (bits) */
-    float4* dir;      /* pending light-sample contribution, emissive target x | y << 16 (bits) */
-    float4* fac;      /* continuation factor, target z | light_is_voxel << 15 | bounces << 16 (bits) */
+    float4* thr;      /* throughput times the continuation factor, sample slot of the chunk (bits) */
+    float4* rad;      /* radiance if the light sample is occluded, emissive target x | y << 16 (bits) */
+    float4* dir;      /* radiance if it is visible, target z | light_is_voxel << 15 | bounces << 16 (bits) */
+    float4* fac;      /* batch and cache launches only: w = scene or training slot + 1 (bits) */
     uint4* meta;      /* after generate only: pixel, sample slot, bounces (0), scene or training slot + 1 */
     uint64_t* rng;    /* PCG32 state (increment = 2*pixel+1) */
 };

@@ -349,11 +349,14 @@ __device__ __forceinline__ void load_phase(const PathBuf& B, uint32_t j, PathSta
     S.rng.state = B.rng[j];
     const float4 o = B.o[j], nd = B.nd[j];
     if (PHASE == PH_SHADOW) {
-        /* the state store_phase packs: pixel, slot and tag in the w of nd,
-         * thr, rad; light target and bounce count in the w of dir, fac */
-        const float4 th = B.thr[j], rd = B.rad[j], di = B.dir[j], fa = B.fac[j];
-        const uint32_t exy = __float_as_uint(di.w), ezb = __float_as_uint(fa.w);
-        me = make_uint4(__float_as_uint(nd.w), __float_as_uint(th.w), ezb >> 16, __float_as_uint(rd.w));
+        /* the state store_cont writes: the throughput already times the
+         * continuation factor, the radiance without and with the light
+         * sample (S.radiance, S.direct); pixel, slot, light target and
+         * bounce count in the w words, the tag in fac.w (TAG only) */
+        const float4 th = B.thr[j], rd = B.rad[j], di = B.dir[j];
+        const uint32_t exy = __float_as_uint(rd.w), ezb = __float_as_uint(di.w);
+        me = make_uint4(__float_as_uint(nd.w), __float_as_uint(th.w), ezb >> 16,
+                        TAG ? __float_as_uint(B.fac[j].w) : 0u);
         S.bounces = (int32_t)me.z;
         S.rng.inc = ((uint64_t)me.x << 1u) | 1u;
         S.ray_o = mk(o.x, o.y, o.z);
@@ -364,7 +367,6 @@ __device__ __forceinline__ void load_phase(const PathBuf& B, uint32_t j, PathSta
         S.throughput = mk(th.x, th.y, th.z);
         S.radiance = mk(rd.x, rd.y, rd.z);
         S.direct = mk(di.x, di.y, di.z);
-        S.factor = mk(fa.x, fa.y, fa.z);
         S.ex = (int32_t)(exy & 0xffffu); S.ey = (int32_t)(exy >> 16);
         S.ez = (int32_t)(ezb & 0x7fffu); S.light_is_voxel = (int32_t)((ezb >> 15) & 1u);
         return;
@@ -392,21 +394,29 @@ __device__ __forceinline__ void load_phase(const PathBuf& B, uint32_t j, PathSta
 
 /* Store after a shade step that continues the path (next: the result of
  * its shadow ray, then its continuation) at position k of the next queue's
- * buffer.  The path's metadata rides in the w components: nd.w pixel,
- * thr.w sample slot, rad.w scene or training tag, dir.w ex | ey << 16,
- * fac.w ez | light_is_voxel << 15 | bounces << 16 (voxel coordinates below
- * 2^15). */
+ * buffer.  What the shadow result does to the path (pathTracer.fs:133-164
+ * as shade_step applies it) is formed here with the same operations:
+ * radiance + throughput * 0 and radiance + throughput * direct (the
+ * radiance for an occluded and a visible light sample) and throughput *
+ * factor, so the round after the shadow ray only picks one.  Metadata in
+ * the w components: nd.w pixel, thr.w sample slot, rad.w ex | ey << 16,
+ * dir.w ez | light_is_voxel << 15 | bounces << 16 (voxel coordinates below
+ * 2^15), fac.w the scene or training tag (TAG only). */
+template <bool TAG>
 __device__ __forceinline__ void store_cont(const PathBuf& N, uint32_t k, const PathState& S, uint4 me)
 {
     const uint32_t exy = ((uint32_t)S.ex & 0xffffu) | ((uint32_t)S.ey << 16);
     const uint32_t ezb = ((uint32_t)S.ez & 0x7fffu) | ((uint32_t)(S.light_is_voxel != 0) << 15) |
                          ((uint32_t)S.bounces << 16);
+    const f3 r_occluded = S.radiance + S.throughput * mk(0.0f, 0.0f, 0.0f);
+    const f3 r_visible = S.radiance + S.throughput * S.direct;
+    const f3 t_next = S.throughput * S.factor;
     N.o[k] = make_float4(S.ray_o.x, S.ray_o.y, S.ray_o.z, S.bsdf_pdf);
     N.nd[k] = make_float4(S.next_d.x, S.next_d.y, S.next_d.z, __uint_as_float(me.x));
-    N.thr[k] = make_float4(S.throughput.x, S.throughput.y, S.throughput.z, __uint_as_float(me.y));
-    N.rad[k] = make_float4(S.radiance.x, S.radiance.y, S.radiance.z, __uint_as_float(me.w));
-    N.dir[k] = make_float4(S.direct.x, S.direct.y, S.direct.z, __uint_as_float(exy));
-    N.fac[k] = make_float4(S.factor.x, S.factor.y, S.factor.z, __uint_as_float(ezb));
+    N.thr[k] = make_float4(t_next.x, t_next.y, t_next.z, __uint_as_float(me.y));
+    N.rad[k] = make_float4(r_occluded.x, r_occluded.y, r_occluded.z, __uint_as_float(exy));
+    N.dir[k] = make_float4(r_visible.x, r_visible.y, r_visible.z, __uint_as_float(ezb));
+    if (TAG) N.fac[k] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(me.w));
     N.rng[k] = S.rng.state;
 }
 
@@ -787,10 +797,14 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
             if (PHASE != PH_PRIMARY) {
                 const uint4 hp = ((const uint4*)P.hit)[j];
                 hv = make_uint2(hp.z, hp.w);
-                /* the shadow ray's result (never ends the path) */
+                /* the shadow ray's result (never ends the path): it picks the
+                 * radiance with or without the light sample (store_cont) */
                 const int ks = unpack_hit(make_uint2(hp.x, hp.y), h);
-                h.slot = 0u;   /* visibility needs no material */
-                shade_step<STATS, CACHE>(a, recs, bits, cdfv, S, h, ks, cnt, -1, vraw, nullptr);
+                const bool visible = S.light_is_voxel ? (ks != KIND_MISS && h.x == S.ex && h.y == S.ey && h.z == S.ez)
+                                                      : (ks == KIND_MISS);
+                if (visible) S.radiance = S.direct;
+                S.phase = PH_CONT;
+                if (STATS) cnt.rays++;
             } else {
                 hv = P.hit[j];
             }
@@ -835,7 +849,7 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
             }
         }
         const uint32_t k = block_append2<WF_SHADE_THREADS>(keep, sid, ro, rd, rn, qao, qbo, qto, &P.cnt[sel ^ 1]);
-        if (keep) store_cont(Bn, k, S, me);
+        if (keep) store_cont<BATCH || CACHE>(Bn, k, S, me);
     }
     flush_counters<STATS>(BATCH ? B.stats : a0.stats, cnt);
 }
