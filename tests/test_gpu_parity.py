@@ -278,6 +278,48 @@ def test_render_large_grid_bit_exact(ctx, orc):
         assert stats[k] == cnt[k], k
 
 
+def test_render_grid_of_maximum_extent_bit_exact(ctx, orc):
+    """A 4096-voxel-long grid (the largest extent the volume accepts): hit
+    records, light targets and queue entries carry coordinates up to 4096
+    (the first voxel past the grid); emissive voxels sit near the far end."""
+    from voxeltoy_amd import api, scenes, workloads
+    nz, ny, nx = 6, 5, 4096
+    rng = np.random.default_rng(33)
+    occ = rng.random((nz, ny, nx)) < 0.0005
+    occ[:, 0, :] = True                                      # a floor the whole length
+    recs = [scenes.lambert((0, 0, 0), (0.6, 0.6, 0.6)), scenes.lambert((6.0, 5.0, 4.0), (0.5, 0.5, 0.5)),
+            scenes.metal((0, 0, 0), (0.9, 0.9, 0.9), 0.2)]
+    mat, offs = scenes.build_materials(recs)
+    id_offset = np.full(256, -1, np.int32)
+    id_offset[1:4] = offs
+    ids = np.where(occ, np.uint8(1), np.uint8(0))
+    ids[rng.random((nz, ny, nx)) < 0.0003] = 3
+    ids[:, 1:3, nx - 8:] = 2                                  # emitters at the far end
+    sc = scenes.Scene(ids, id_offset, mat)
+    sc.emissive = api.prune_emissive(sc.offsets(), scenes.emissive_list(ids, id_offset, mat))
+    assert len(sc.emissive) > 0 and (sc.emissive % nx).max() >= nx - 8
+    bmin, bmax = workloads.reference_bounds(sc.shape)
+    # looking down the length from just before the near end: rays march up
+    # to the full 4096 voxels and leave through the far end
+    eye = np.array([bmin[0] - 20.0, 0.8 * bmax[1], 0.1 * bmax[2]], np.float32)
+    cam = scenes.CameraSetup(eye, np.array([bmax[0], 0.3 * bmax[1], 0.0], np.float32), 0.06)
+    wl = workloads.Workload("long", sc, cam, 96, 24, spp=2, bounces=3, bmin=bmin, bmax=bmax,
+                            env_top=(1.2, 1.2, 1.3), env_bottom=(0.2, 0.2, 0.2))
+    workloads.load(ctx, wl)
+    ctx.set_render_mode(1, 0)
+    ctx.set_stats(True)
+    ctx.render(wl.seed, 0, wl.spp)
+    got = ctx.read_accum()
+    stats = ctx.stats()
+    ctx.set_stats(False)
+    ref, cnt = oracle_for(orc, wl).render(wl.width, wl.height, wl.seed, 0, wl.spp)
+    same = (got.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), f"{(~same).sum()} values differ"
+    for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
+        assert stats[k] == cnt[k], k
+    assert cnt["interactions"] > 0
+
+
 def test_render_settings_variants_bit_exact(ctx, orc):
     """Wireframe overlay, selected voxel, exposure/gamma and rng frame clamp."""
     from voxeltoy_amd import workloads
