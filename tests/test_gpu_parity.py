@@ -320,6 +320,39 @@ def test_render_grid_of_maximum_extent_bit_exact(ctx, orc):
     assert cnt["interactions"] > 0
 
 
+def test_benchmark_workload_at_full_size_bit_exact(ctx, orc):
+    """The benchmarked configuration at its full size (256^3 terrain with
+    metal and 23k emissive voxels, 1024x1024 thin lens, 1024x512
+    importance-sampled sky, 8 bounces): two passes of the whole frame equal
+    the oracle bit for bit, counters included; and the benchmark's 16-pass
+    step rendered in one call equals the same passes rendered 8 + 8."""
+    from voxeltoy_amd import workloads
+    wl = workloads.full()
+    assert (wl.width, wl.height, wl.bounces, wl.scene.shape) == (1024, 1024, 8, (256, 256, 256))
+    workloads.load(ctx, wl)
+    ctx.set_render_mode(1, 0)
+    ctx.set_stats(True)
+    ctx.render(wl.seed, 0, 2)
+    got = ctx.read_accum()
+    stats = ctx.stats()
+    ctx.set_stats(False)
+    ref, cnt = oracle_for(orc, wl).render(wl.width, wl.height, wl.seed, 0, 2)
+    same = (got.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), f"{(~same).sum()} values differ"
+    for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
+        assert stats[k] == cnt[k], k
+    assert cnt["env_samples"] > 0 and cnt["samples"] == 2 * wl.width * wl.height
+    ctx.reset_accum()
+    ctx.render(wl.seed, 0, wl.spp)
+    one = ctx.read_accum()
+    ctx.reset_accum()
+    ctx.render(wl.seed, 0, wl.spp // 2)
+    ctx.render(wl.seed, wl.spp // 2, wl.spp - wl.spp // 2)
+    two = ctx.read_accum()
+    assert np.array_equal(one.view(np.uint32), two.view(np.uint32))
+    assert (one[..., 3] == 1.0).all()           # the running mean of the samples' alpha
+
+
 def test_render_settings_variants_bit_exact(ctx, orc):
     """Wireframe overlay, selected voxel, exposure/gamma and rng frame clamp."""
     from voxeltoy_amd import workloads
