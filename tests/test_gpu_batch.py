@@ -191,3 +191,30 @@ def test_context_destroyed_before_its_batch():
     ctxs[0].render(wls[0].seed, 0, 1)
     assert ctxs[0].read_accum()[..., 3].min() == 1.0
     ctxs[0].close()
+
+
+def test_batch_workload_at_full_size_equals_single_renders():
+    """The benchmarked batch at its full size (64 scenes, grids 32^3-512^3,
+    images 128^2-1024^2, 1-64 spp, 4 bounces; 257 M paths): the HIP-graph
+    batch gives every scene exactly its own single-scene render."""
+    from voxeltoy_amd import api, workloads
+    wls = workloads.batch(64)
+    assert len(wls) == 64
+    ctxs = []
+    for wl in wls:
+        c = api.Context(0)
+        workloads.load(c, wl)
+        ctxs.append(c)
+    seeds = [wl.seed for wl in wls]
+    counts = [wl.spp for wl in wls]
+    b = api.Batch(ctxs)
+    b.render(seeds, counts, graph=True, stats=True)
+    assert b.stats()["samples"] == sum(wl.width * wl.height * wl.spp for wl in wls)
+    batched = [c.read_accum() for c in ctxs]
+    for i, (c, wl) in enumerate(zip(ctxs, wls)):
+        c.reset_accum()
+        c.render(wl.seed, 0, wl.spp)
+        assert np.array_equal(c.read_accum().view(np.uint32), batched[i].view(np.uint32)), f"scene {i} differs"
+    b.close()
+    for c in ctxs:
+        c.close()
