@@ -299,3 +299,38 @@ def test_context_destroyed_before_its_cache(vtapi):
     c2.render(wl.seed, 0, 1)
     assert c2.read_accum()[..., 3].min() == 1.0
     c2.close()
+
+
+def test_cache_workload_at_full_size_structure(vtapi):
+    """The benchmarked cache workload at its full size (the 256^3 scene at
+    1920x1080, 8 bounces), two frames: with the cache off (training paths
+    recording) the image is the plain render, and an all-zero cache at
+    bounce 2 without training equals the plain integrator cut at 2 bounces,
+    both bit for bit."""
+    from voxeltoy_amd import workloads
+    wl = workloads.full(width=1920, height=1080, spp=2, bounces=8)
+    plain = vtapi.Context(0)
+    workloads.load(plain, wl)
+    plain.render(wl.seed, 0, wl.spp)
+    a = vtapi.Context(0)
+    workloads.load(a, wl)
+    k = vtapi.RadianceCache(a, seed=4)
+    st = k.render(wl.seed, 0, wl.spp, cache_bounce=-1, train_paths=65536, lr=1e-3)
+    assert st["queries"] == 0 and st["samples"] > 0
+    assert np.array_equal(a.read_accum().view(np.uint32), plain.read_accum().view(np.uint32))
+    k.close()
+    a.close()
+    z = vtapi.Context(0)
+    workloads.load(z, wl)
+    k = vtapi.RadianceCache(z, seed=5)
+    k.set_params(np.zeros(k.n_params, np.float32))
+    st = k.render(wl.seed, 0, wl.spp, cache_bounce=2, train_paths=0)
+    assert st["queries"] > 0 and st["steps"] == 0
+    wl.bounces = 2
+    cut = vtapi.Context(0)
+    workloads.load(cut, wl)
+    cut.render(wl.seed, 0, wl.spp)
+    assert np.array_equal(z.read_accum().view(np.uint32), cut.read_accum().view(np.uint32))
+    k.close()
+    for c in (z, cut, plain):
+        c.close()
