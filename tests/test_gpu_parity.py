@@ -443,3 +443,43 @@ def test_voxelizer_full_size_bit_exact(ctx, orc):
     occ = ctx.voxelize(verts, tris, M, mode=0, material_offset=0)
     ref = orc.voxelize(verts, tris, M, [res] * 3, mode=0)
     assert np.array_equal(occ, ref), f"{np.sum(occ != ref)} voxels differ"
+
+
+def test_voxelize_workload_render_at_full_size_bit_exact(ctx, orc):
+    """The voxelize-then-render workload at its full size: 200k triangles
+    voxelized THIN into an empty 512^3 grid on the device, then two passes
+    at 1920x1080 with 6 bounces under a gradient sky; image and counters
+    equal the oracle's render of the oracle's voxelization."""
+    from voxeltoy_amd import scenes, workloads
+    verts, tris = scenes.displaced_icosphere(seed=4, freq=100)
+    res, W, H, passes, bounces = 512, 1920, 1080, 2, 6
+    M = orc.mesh_transform(verts.min(0), verts.max(0), [res] * 3)
+    mat = np.asarray(scenes.lambert((0.0, 0.0, 0.0), (0.7, 0.65, 0.6)), np.float32)
+    ctx.set_volume_u8(np.zeros((res, res, res), np.uint8), np.full(256, -1, np.int32), mat)
+    bmin, bmax = workloads.reference_bounds((res, res, res))
+    cam = scenes.orbit_camera(bmin, bmax, math.radians(70), math.radians(35), W, H)
+    wl = workloads.Workload("c3", None, cam, W, H, spp=passes, bounces=bounces, seed=77, bmin=bmin, bmax=bmax,
+                            env_top=(1.6, 1.7, 1.9), env_bottom=(0.35, 0.3, 0.25))
+    ctx.set_camera(**wl.camera_arrays())
+    ctx.set_environment(wl.env_top, wl.env_bottom)
+    ctx.set_settings(max_bounces=bounces)
+    ctx.set_frame(W, H)
+    ctx.set_render_mode(1, 0)
+    occ = ctx.voxelize(verts, tris, M, mode=0, material_offset=0)
+    ctx.set_stats(True)
+    ctx.render(wl.seed, 0, passes)
+    got = ctx.read_accum()
+    stats = ctx.stats()
+    ctx.set_stats(False)
+    ref_occ = orc.voxelize(verts, tris, M, [res] * 3, mode=0)
+    assert np.array_equal(occ, ref_occ)
+    o = orc.OracleScene(np.where(ref_occ > 0, 0, -1).astype(np.int32), mat, np.zeros(0, np.int32))
+    o.set_camera(**wl.camera_arrays())
+    o.set_environment(wl.env_top, wl.env_bottom)
+    o.set_settings(max_bounces=bounces)
+    ref, cnt = o.render(W, H, wl.seed, 0, passes)
+    same = (got.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), f"{(~same).sum()} values differ"
+    for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
+        assert stats[k] == cnt[k], k
+    assert cnt["interactions"] > 0 and cnt["samples"] == passes * W * H
