@@ -107,8 +107,11 @@ def test_trace_rays_injected_bit_exact(ctx, orc):
 
 
 @pytest.mark.parametrize("angle,int32_grid,mode,max_paths", [(0, False, 1, 0), (3, True, 0, 0), (6, False, 1, 70001),
-                                                              (5, False, 0, 0)])
+                                                              (5, False, 0, 0), (1, False, 1, 0), (2, False, 1, 0),
+                                                              (4, True, 1, 0), (7, False, 1, 0)])
 def test_render_small_scene_bit_exact(ctx, orc, angle, int32_grid, mode, max_paths):
+    """The small configuration at its own size (64^3, 256x256, 4 spp, 4
+    bounces, white sky) from all eight orbit angles."""
     from voxeltoy_amd import workloads
     wl = workloads.small(angle)
     workloads.load(ctx, wl, fp64=True, int32_grid=int32_grid)
