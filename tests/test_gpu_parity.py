@@ -391,6 +391,52 @@ def test_volume_roundtrip_and_nonuniform_grid(ctx, orc, vtapi):
     assert np.array_equal(ctx.trace_rays(org, d), o.trace_rays(org, d))
 
 
+@pytest.mark.parametrize("shape,fill", [((1, 1, 1), "single"), ((1, 17, 23), "random"), ((19, 1, 21), "random"),
+                                        ((18, 22, 1), "random"), ((12, 12, 12), "empty"), ((12, 12, 12), "full")])
+def test_render_degenerate_grids_bit_exact(ctx, orc, shape, fill):
+    """Whole renders of degenerate grids: a single (emissive) voxel, slabs one
+    voxel thick along each axis, an empty grid (every ray goes to the sky or
+    the ground) and a completely filled one (emitters survive pruning only on
+    its faces): image and counters equal the oracle."""
+    from voxeltoy_amd import api, scenes, workloads
+    rng = np.random.default_rng(int(np.prod(shape)) + len(fill))
+    recs = [scenes.lambert((0, 0, 0), (0.6, 0.55, 0.5)), scenes.lambert((5.0, 4.0, 3.0), (0.5, 0.5, 0.5)),
+            scenes.metal((0, 0, 0), (0.9, 0.85, 0.8), 0.3)]
+    mat, offs = scenes.build_materials(recs)
+    id_offset = np.full(256, -1, np.int32)
+    id_offset[1:4] = offs
+    if fill == "single":
+        ids = np.full(shape, 2, np.uint8)
+    elif fill == "empty":
+        ids = np.zeros(shape, np.uint8)
+    elif fill == "full":
+        ids = rng.choice(np.array([1, 2, 3], np.uint8), size=shape, p=[0.7, 0.1, 0.2])
+    else:
+        ids = np.where(rng.random(shape) < 0.4, rng.choice(np.array([1, 2, 3], np.uint8), size=shape, p=[0.6, 0.15, 0.25]),
+                       np.uint8(0)).astype(np.uint8)
+    sc = scenes.Scene(ids, id_offset, mat)
+    em = scenes.emissive_list(ids, id_offset, mat)
+    sc.emissive = api.prune_emissive(sc.offsets(), em) if len(em) else em
+    if fill != "empty":
+        assert len(sc.emissive) > 0
+    bmin, bmax = workloads.reference_bounds(sc.shape)
+    cam = scenes.orbit_camera(bmin, bmax, math.radians(55), math.radians(40), 64, 48)
+    wl = workloads.Workload("degenerate", sc, cam, 64, 48, spp=2, bounces=3, bmin=bmin, bmax=bmax,
+                            env_top=(1.5, 1.4, 1.2), env_bottom=(0.3, 0.3, 0.35))
+    workloads.load(ctx, wl)
+    ctx.set_render_mode(1, 0)
+    ctx.set_stats(True)
+    ctx.render(wl.seed, 0, wl.spp)
+    got = ctx.read_accum()
+    stats = ctx.stats()
+    ctx.set_stats(False)
+    ref, cnt = oracle_for(orc, wl).render(wl.width, wl.height, wl.seed, 0, wl.spp)
+    same = (got.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), f"{(~same).sum()} values differ"
+    for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
+        assert stats[k] == cnt[k], k
+
+
 def test_empty_and_full_grids(ctx, orc):
     mat = np.array([0, 0, 0, 0, .5, .5, .5], np.float32)
     for fill in (-1, 0):
