@@ -437,6 +437,50 @@ def test_render_degenerate_grids_bit_exact(ctx, orc, shape, fill):
         assert stats[k] == cnt[k], k
 
 
+@pytest.mark.parametrize("lens_model", [0, 1, 2])
+def test_render_camera_inside_grid_bit_exact(ctx, orc, lens_model):
+    """The eye inside the grid (slab test clamped to t = 0, camera rays start
+    mid-grid) in a sparse scene, for the pinhole, thin-lens and orthographic
+    cameras: image and counters equal the oracle."""
+    from voxeltoy_amd import api, scenes, workloads
+    n = 40
+    rng = np.random.default_rng(70 + lens_model)
+    recs = [scenes.lambert((0, 0, 0), (0.6, 0.55, 0.5)), scenes.lambert((5.0, 4.0, 3.0), (0.5, 0.5, 0.5)),
+            scenes.metal((0, 0, 0), (0.9, 0.85, 0.8), 0.3)]
+    mat, offs = scenes.build_materials(recs)
+    id_offset = np.full(256, -1, np.int32)
+    id_offset[1:4] = offs
+    ids = np.where(rng.random((n, n, n)) < 0.03, rng.choice(np.array([1, 2, 3], np.uint8), size=(n, n, n),
+                                                              p=[0.6, 0.15, 0.25]), np.uint8(0)).astype(np.uint8)
+    ids[17:24, 17:24, 17:24] = 0                     # room around the eye
+    sc = scenes.Scene(ids, id_offset, mat)
+    sc.emissive = api.prune_emissive(sc.offsets(), scenes.emissive_list(ids, id_offset, mat))
+    bmin, bmax = workloads.reference_bounds(sc.shape)
+    bmin, bmax = np.asarray(bmin, np.float32), np.asarray(bmax, np.float32)
+    eye = (bmin + (bmax - bmin) * np.array([0.51, 0.49, 0.505], np.float32)).astype(np.float32)
+    target = (bmin + (bmax - bmin) * np.array([0.9, 0.3, 0.7], np.float32)).astype(np.float32)
+    W, H = 64, 48
+    cam = scenes.CameraSetup(eye, target, scenes.default_fov(W, H), lens_model=lens_model,
+                             lens_radius=15.0 if lens_model == 1 else 0.0, focal_distance=300.0)
+    wl = workloads.Workload("inside", sc, cam, W, H, spp=2, bounces=3, bmin=bmin, bmax=bmax,
+                            env_top=(1.5, 1.4, 1.2), env_bottom=(0.3, 0.3, 0.35))
+    workloads.load(ctx, wl)
+    lo, hi, _ = ctx.bounds()
+    assert np.all(eye > lo) and np.all(eye < hi)
+    ctx.set_render_mode(1, 0)
+    ctx.set_stats(True)
+    ctx.render(wl.seed, 0, wl.spp)
+    got = ctx.read_accum()
+    stats = ctx.stats()
+    ctx.set_stats(False)
+    ref, cnt = oracle_for(orc, wl).render(wl.width, wl.height, wl.seed, 0, wl.spp)
+    same = (got.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), f"{(~same).sum()} values differ"
+    for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
+        assert stats[k] == cnt[k], k
+    assert cnt["interactions"] > 0
+
+
 def test_empty_and_full_grids(ctx, orc):
     mat = np.array([0, 0, 0, 0, .5, .5, .5], np.float32)
     for fill in (-1, 0):
