@@ -481,6 +481,47 @@ def test_render_camera_inside_grid_bit_exact(ctx, orc, lens_model):
     assert cnt["interactions"] > 0
 
 
+@pytest.mark.parametrize("rough", [2.0, 60.0])
+def test_render_plastic_materials_bit_exact(ctx, orc, rough):
+    """Plastic (plastic.h: microfacet with reflectance 1, the stored diffuse
+    albedo unused) as the dominant material beside matte, metal and emitters,
+    at a near-diffuse and a sharp Blinn exponent: image and counters equal
+    the oracle."""
+    from voxeltoy_amd import api, scenes, workloads
+    sc = scenes.small_scene(21, n=40)
+    recs = [scenes.plastic((0, 0, 0), (0.8, 0.2, 0.1), rough), scenes.lambert((0, 0, 0), (0.5, 0.6, 0.4)),
+            scenes.metal((0, 0, 0), (0.9, 0.8, 0.7), 0.4), scenes.lambert((4.0, 3.5, 3.0), (0.5, 0.5, 0.5)),
+            scenes.plastic((0.5, 0.5, 0.8), (0.1, 0.1, 0.1), 10.0)]
+    mat, offs = scenes.build_materials(recs)
+    rng = np.random.default_rng(int(rough))
+    occupied = sc.ids > 0
+    ids = np.zeros(sc.shape, np.uint8)
+    ids[occupied] = rng.choice(np.arange(1, 6, dtype=np.uint8), size=int(occupied.sum()),
+                               p=[0.55, 0.2, 0.1, 0.1, 0.05])
+    id_offset = np.full(256, -1, np.int32)
+    id_offset[1:6] = offs
+    sc = scenes.Scene(ids, id_offset, mat)
+    sc.emissive = api.prune_emissive(sc.offsets(), scenes.emissive_list(ids, id_offset, mat))
+    assert len(sc.emissive) > 0
+    bmin, bmax = workloads.reference_bounds(sc.shape)
+    cam = scenes.orbit_camera(bmin, bmax, math.radians(60), math.radians(30), 80, 60)
+    wl = workloads.Workload("plastic", sc, cam, 80, 60, spp=2, bounces=4, bmin=bmin, bmax=bmax,
+                            env_top=(1.5, 1.4, 1.2), env_bottom=(0.3, 0.3, 0.35))
+    workloads.load(ctx, wl)
+    ctx.set_render_mode(1, 0)
+    ctx.set_stats(True)
+    ctx.render(wl.seed, 0, wl.spp)
+    got = ctx.read_accum()
+    stats = ctx.stats()
+    ctx.set_stats(False)
+    ref, cnt = oracle_for(orc, wl).render(wl.width, wl.height, wl.seed, 0, wl.spp)
+    same = (got.view(np.uint32) == ref.view(np.uint32)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), f"{(~same).sum()} values differ"
+    for k in ("rays", "visits", "interactions", "samples", "env_samples", "env_evals"):
+        assert stats[k] == cnt[k], k
+    assert cnt["interactions"] > 0
+
+
 def test_empty_and_full_grids(ctx, orc):
     mat = np.array([0, 0, 0, 0, .5, .5, .5], np.float32)
     for fill in (-1, 0):

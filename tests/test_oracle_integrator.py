@@ -171,3 +171,24 @@ def test_metal_sampling_matches_its_pdf(orc, vtapi):
     mean = got.mean(0)
     sd = got.std(0) / math.sqrt(n)
     assert np.all(np.abs(mean - expect) < 6 * sd + 1e-3), (mean, expect, sd)
+
+
+def test_plastic_is_metal_of_unit_reflectance(orc, vtapi):
+    """plastic.h: the microfacet BSDF with reflectance 1, whatever diffuse
+    albedo the record stores (material.h:28-33). A plastic face renders
+    bit-identically to a metal face of reflectance 1 and the same exponent,
+    and differently from a metal face of another reflectance or exponent,
+    so neither the stored albedo nor a misplaced exponent goes unnoticed."""
+    e = 6.0
+    plastic = np.array([2, 0, 0, 0, 0.8, 0.2, 0.1, e], np.float32)
+    metal1 = np.array([1, 0, 0, 0, 1.0, 1.0, 1.0, e], np.float32)
+    renders = []
+    for mat in (plastic, metal1, np.array([1, 0, 0, 0, 0.8, 0.2, 0.1, e], np.float32),
+                np.array([1, 0, 0, 0, 1.0, 1.0, 1.0, 2 * e], np.float32)):
+        o = top_face_setup(orc, vtapi, mat, W=24, H=24)
+        acc, _ = o.render(24, 24, 5, 0, 8)
+        renders.append(acc)
+    assert np.isfinite(renders[0]).any()
+    assert np.array_equal(renders[0].view(np.uint32), renders[1].view(np.uint32))
+    assert not np.array_equal(renders[0].view(np.uint32), renders[2].view(np.uint32))
+    assert not np.array_equal(renders[0].view(np.uint32), renders[3].view(np.uint32))

@@ -42,6 +42,12 @@ def metal(emission, reflectance, roughness):
     return [float(MT_METAL), *map(float, emission), *map(float, reflectance), float(roughness)]
 
 
+def plastic(emission, diffuse_albedo, roughness):
+    """PlasticMaterialData (src/renderer/material/material.h:28-33); the
+    shader ignores the diffuse albedo (plastic.h: microfacet, reflectance 1)."""
+    return [float(MT_PLASTIC), *map(float, emission), *map(float, diffuse_albedo), float(roughness)]
+
+
 def build_materials(records):
     """Concatenate records; returns (mat float32, offsets list)."""
     mat, offs = [], []
