@@ -595,7 +595,7 @@ __global__ __launch_bounds__(WF_GENERATE_THREADS) void wf_generate_kernel(TraceA
 
 /* Persistent traversal: each lane marches its ray a.dda_budget steps at a
  * time; whenever at least a.refill_min lanes of the wave are idle they take
- * the next entries of the wave's private chunk of the queue, in queue order.
+ * the next entries of the wave's private chunk of the queue, from the queue's end.
  * A wave reserves a.refill_chunk entries per atomic on the queue head
  * (0 = exactly the idle lanes), so few waves contend for the counter, and
  * lanes stay busy until the queue drains instead of idling behind the
@@ -665,6 +665,11 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
                 if (p < avail) j = c_pos + p;
                 else if (grab && nb + (p - avail) < n) j = nb + (p - avail);
                 if (j != 0xffffffffu) {
+                    /* the queue is taken from its end: paths are queued in
+                     * pixel order, and the last rows (towards the horizon)
+                     * hold the longest rays, which then start first instead
+                     * of prolonging the end of the launch */
+                    j = n - 1u - j;
                     float4 o = qa[j];
                     float2 d = qb[j];
                     /* both parts of the entry in flight together: one
