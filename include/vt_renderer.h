@@ -86,6 +86,10 @@ int vtr_camera_set_film_size(vtr_renderer* r, float w, float h);
 int vtr_camera_set_lens_radius(vtr_renderer* r, float radius);
 int vtr_camera_set_fstop(vtr_renderer* r, float fstop);
 int vtr_camera_set_fov_y(vtr_renderer* r, float fov);
+/* Camera::setCameraController (src/camera/camera.h:22-28, camera.cpp:25-35):
+ * 0 = CCM_ORBIT, 1 = CCM_FLY; selects how vtr_on_mouse_move and
+ * vtr_on_key_press move the camera. */
+int vtr_camera_set_controller(vtr_renderer* r, int32_t mode);
 /* The vt_camera block the renderer uploads for the current state. */
 int vtr_camera_block(vtr_renderer* r, vt_camera* out);
 

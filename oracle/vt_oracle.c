@@ -1064,32 +1064,45 @@ static void transpose_img(const float* src, int w, int h, float* dst)
         for (int x = 0; x < w; ++x) dst[(size_t)x * h + y] = src[(size_t)y * w + x];
 }
 
-/* generateImageFunction  (src/renderer/image.cpp:321-382).  The luminance
- * is taken first (it commutes with the linear resize), then the image is
- * resized when larger than max_size, then blurred by the normalised 3x3
- * Gaussian exp(-2 (2x/3)^2) with black outside the image. */
+/* generateImageFunction  (src/renderer/image.cpp:285-346).  An image larger
+ * than max_size on its longer side is resized to
+ * (unsigned)((float)w / max(w, h) * max_size) x (same for h) (:309-321), each
+ * RGB channel by the separable Lanczos-3 passes above; then the Rec.709
+ * luminance ((r*0.2126 + g*0.7152) + b*0.0722, :328-335), then the 3x3
+ * Gaussian of make_kernel("gaussian", 3, 3): weights g(i)g(j)/sum with
+ * g(0) = 1, g(+-1) = exp(-2 (2/3)^2), and black outside the image
+ * (:338-342). */
 void orc_image_function(const float* rgb, int32_t w, int32_t h, int32_t max_size, float* out,
                         int32_t* ow, int32_t* oh)
 {
-    float* lum = (float*)malloc(sizeof(float) * (size_t)w * h);
-    for (size_t i = 0; i < (size_t)w * h; ++i)
-        lum[i] = (rgb[3 * i] * 0.2126f + rgb[3 * i + 1] * 0.7152f) + rgb[3 * i + 2] * 0.0722f;
     int nw = w, nh = h;
     if ((w > h ? w : h) > max_size) {
         const int mxd = w > h ? w : h;
         nw = (int)((float)w / (float)mxd * (float)max_size);
         nh = (int)((float)h / (float)mxd * (float)max_size);
+    }
+    float* chan[3];
+    for (int c = 0; c < 3; ++c) {
+        float* src = (float*)malloc(sizeof(float) * (size_t)w * h);
+        for (size_t i = 0; i < (size_t)w * h; ++i) src[i] = rgb[3 * i + c];
+        if (nw == w && nh == h) {
+            chan[c] = src;
+            continue;
+        }
         float* t1 = (float*)malloc(sizeof(float) * (size_t)nw * h);
-        resample_rows(lum, w, h, t1, nw);
+        resample_rows(src, w, h, t1, nw);
         float* t2 = (float*)malloc(sizeof(float) * (size_t)nw * h);
         transpose_img(t1, nw, h, t2);
         float* t3 = (float*)malloc(sizeof(float) * (size_t)nw * nh);
         resample_rows(t2, h, nw, t3, nh);
-        free(lum);
-        lum = (float*)malloc(sizeof(float) * (size_t)nw * nh);
-        transpose_img(t3, nh, nw, lum);
-        free(t1); free(t2); free(t3);
+        chan[c] = (float*)malloc(sizeof(float) * (size_t)nw * nh);
+        transpose_img(t3, nh, nw, chan[c]);
+        free(src); free(t1); free(t2); free(t3);
     }
+    float* lum = (float*)malloc(sizeof(float) * (size_t)nw * nh);
+    for (size_t i = 0; i < (size_t)nw * nh; ++i)
+        lum[i] = (chan[0][i] * 0.2126f + chan[1][i] * 0.7152f) + chan[2][i] * 0.0722f;
+    for (int c = 0; c < 3; ++c) free(chan[c]);
     const float g1 = expf(-2.0f * ((2.0f / 3.0f) * (2.0f / 3.0f)));
     const float k1[3] = { g1, 1.0f, g1 };
     float ksum = 0.0f;

@@ -17,7 +17,7 @@ def declared_functions():
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(vt_\w+)\s*\(", text, flags=re.M):
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(vtr?_\w+)\s*\(", text, flags=re.M):
             names.add(m.group(1))
     return names
 
@@ -25,6 +25,8 @@ def declared_functions():
 def test_library_exports_every_declared_symbol(vtapi):
     names = declared_functions()
     assert len(names) >= 25
+    # both headers: the kernel-level ABI and the renderer object's ABI
+    assert sum(n.startswith("vtr_") for n in names) >= 30 and "vt_render" in names
     out = subprocess.run(["nm", "-D", "--defined-only", vtapi.library_path()], capture_output=True, text=True,
                          check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
@@ -87,3 +89,17 @@ def test_prune_emissive_matches_oracle_and_brute_force(vtapi, orc):
         if not all(nb):
             keep.add(int(v))
     assert set(a.tolist()) == keep and len(a) == len(keep)
+
+
+def test_set_environment_rejects_bad_shapes(vtapi):
+    """The binding checks array ranks before the library copies h*w*3,
+    rows*cols and n floats from them (no device is touched)."""
+    ctx = object.__new__(vtapi.Context)
+    cu = np.zeros((4, 9), np.float32)
+    cv = np.linspace(0, 1, 5, dtype=np.float32)
+    for img, u, v in [(np.zeros((4, 8), np.float32), cu, cv),          # no channel axis
+                      (np.zeros((4, 8, 1), np.float32), cu, cv),       # one channel
+                      (np.zeros((4, 8, 3), np.float32), cu.ravel(), cv),
+                      (np.zeros((4, 8, 3), np.float32), cu, cv[:, None])]:
+        with pytest.raises(ValueError):
+            ctx.set_environment(image=img, cdf_u=u, cdf_v=v, integral=1.0)

@@ -151,6 +151,11 @@ def test_batch_rejects_bad_input():
     b.close()
     with pytest.raises(api.VTError):
         api.Batch([])
+    d = api.Context(0)
+    with pytest.raises(api.VTError, match="more than once"):
+        api.Batch([c, d, c])        # one frame would take two scenes' samples
+    api.Batch([c, d]).close()
+    d.close()
     c.close()
 
 

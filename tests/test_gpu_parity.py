@@ -547,6 +547,25 @@ def test_invalid_inputs_raise(ctx, vtapi):
         ctx.set_volume(off, mat, emissive=np.array([64], np.int32)) # out of range
 
 
+def test_rejected_environment_keeps_previous_one(ctx, vtapi):
+    """A set_environment call that the library rejects leaves the previous
+    image environment in place, whole: the next render equals the render
+    made before the rejected call."""
+    from voxeltoy_amd import workloads
+    wl = small_full_workload(1)
+    workloads.load(ctx, wl)
+    ctx.render(wl.seed, 0, 2)
+    before = ctx.read_accum()
+    cu, cv, integ = wl.cdf
+    with pytest.raises(vtapi.VTError):
+        # CDF U rows must equal CDF V entries - 1
+        ctx.set_environment(wl.env_top, wl.env_bottom, wl.env_image, cu[:-1], cv, integ, wl.env_rotation)
+    ctx.reset_accum()
+    ctx.render(wl.seed, 0, 2)
+    after = ctx.read_accum()
+    assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_voxelizer_bit_exact(ctx, orc, mode):
     from voxeltoy_amd import scenes

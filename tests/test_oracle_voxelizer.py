@@ -106,3 +106,39 @@ def test_thin_sphere_is_six_separating(orc):
                 q.append((a, b, cc))
     centre = res // 2
     assert not occ[centre, centre, centre] and not seen[centre, centre, centre]
+
+
+def test_mesh_transform_closed_form(orc, vtapi):
+    """computeMeshTransform (import.cpp:46-64): scale s = (1 - 2/res_major) /
+    size_major on the diagonal, translation (margin/s - bmin)*s in the last
+    column (uploaded with transpose, so M acts on column vectors): bmin maps
+    to one voxel of margin on every axis, bmax onto 1 - margin on the major
+    axis and inside the unit cube on the others.  The product's matrix is
+    the oracle's, bit for bit."""
+    rng = np.random.default_rng(11)
+    cases = [((-1, 2, 0), (3, 4, 1), (64, 64, 64)),
+             ((0, -5, 2), (1, 5, 3), (32, 64, 128)),        # major axis y, per-axis margins
+             ((10, 10, -7), (11, 12, 13), (512, 256, 16))]  # major axis z
+    for _ in range(20):
+        lo = rng.uniform(-100, 100, 3)
+        cases.append((tuple(lo), tuple(lo + rng.uniform(0.1, 50, 3)), tuple(int(x) for x in rng.integers(4, 600, 3))))
+    for bmin, bmax, res in cases:
+        bmin = np.asarray(bmin, np.float32)
+        bmax = np.asarray(bmax, np.float32)
+        M = orc.mesh_transform(bmin, bmax, res)
+        assert np.array_equal(M.view(np.uint32), vtapi.mesh_transform(bmin, bmax, res).view(np.uint32))
+        size = (bmax - bmin).astype(np.float64)
+        major = int(np.argmax(size))            # first of equal maxima, as Box3f::majorAxis
+        margin = 1.0 / np.asarray(res, np.float64)
+        s = (1.0 - 2.0 * margin[major]) / size[major]
+        assert np.allclose(np.diag(M)[:3], s, rtol=1e-6) and M[3, 3] == 1.0
+        assert np.count_nonzero(M[:3, :3] - np.diag(np.diag(M)[:3])) == 0 and not M[3, :3].any()
+        lo = M.astype(np.float64) @ np.r_[bmin, 1.0]
+        hi = M.astype(np.float64) @ np.r_[bmax, 1.0]
+        scale = max(1.0, float(np.abs(bmin).max()) * s)
+        assert np.allclose(lo[:3], margin, atol=2e-6 * scale)
+        assert abs(hi[major] - (1.0 - margin[major])) <= 2e-6 * scale
+        # other axes: margin + s * size (inside the cube when the grid is cubic)
+        assert np.allclose(hi[:3], margin + s * size, atol=2e-6 * scale) and lo[3] == 1.0
+        if len(set(res)) == 1:
+            assert np.all(hi[:3] <= 1.0 - margin + 2e-6 * scale)

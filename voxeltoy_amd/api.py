@@ -315,6 +315,13 @@ class Context:
             img = np.ascontiguousarray(image, np.float32)
             cu = np.ascontiguousarray(cdf_u, np.float32)
             cv = np.ascontiguousarray(cdf_v, np.float32)
+            # the library copies h*w*3, rows*cols and n floats from these
+            if img.ndim != 3 or img.shape[2] != 3:
+                raise ValueError(f"environment image must be (h, w, 3), got {img.shape}")
+            if cu.ndim != 2:
+                raise ValueError(f"cdf_u must be 2-D (h', w'+1), got {cu.shape}")
+            if cv.ndim != 1:
+                raise ValueError(f"cdf_v must be 1-D (h'+1,), got {cv.shape}")
             keep = [img, cu, cv]
             env.use_image = 1
             env.image_h, env.image_w = img.shape[:2]

@@ -620,26 +620,48 @@ int vt_set_camera(vt_context* c, const vt_camera* cam)
 int vt_set_environment(vt_context* c, const vt_environment* env)
 {
     if (!c || !env) return fail(VT_ERR_ARG, "null argument");
-    VT_HIP(hipSetDevice(c->device));
     vt_environment e = *env;
-    dfree(c->d_img); dfree(c->d_quad); dfree(c->d_cdfu); dfree(c->d_cdfv);
     if (e.use_image) {
         if (e.image_w <= 0 || e.image_h <= 0 || !e.image_rgb || e.cdf_u_w < 2 || e.cdf_u_h < 1 || !e.cdf_u ||
             e.cdf_v_n < 2 || !e.cdf_v)
             return fail(VT_ERR_ARG, "image environment needs an image and its CDF tables");
         if (e.cdf_u_h != e.cdf_v_n - 1) return fail(VT_ERR_ARG, "CDF U height must equal CDF V size - 1");
         if (e.cdf_v_n > 8193) return fail(VT_ERR_LIMIT, "CDF V larger than 8193 entries");
-        const size_t ni = (size_t)e.image_w * e.image_h * 3, nu = (size_t)e.cdf_u_w * e.cdf_u_h;
-        VT_HIP(hipMalloc(&c->d_img, ni * 4));
-        VT_HIP(hipMalloc(&c->d_cdfu, nu * 4));
-        VT_HIP(hipMalloc(&c->d_cdfv, (size_t)e.cdf_v_n * 4));
-        VT_HIP(hipMemcpyAsync(c->d_img, e.image_rgb, ni * 4, hipMemcpyHostToDevice, c->stream));
-        VT_HIP(hipMemcpyAsync(c->d_cdfu, e.cdf_u, nu * 4, hipMemcpyHostToDevice, c->stream));
-        VT_HIP(hipMemcpyAsync(c->d_cdfv, e.cdf_v, (size_t)e.cdf_v_n * 4, hipMemcpyHostToDevice, c->stream));
-        VT_HIP(hipMalloc(&c->d_quad, (size_t)(e.image_w + 1) * (size_t)(e.image_h + 1) * 64));
-        VT_HIP(vt::launch_env_quads(c->d_img, e.image_w, e.image_h, c->d_quad, c->stream));
-        VT_HIP(hipStreamSynchronize(c->stream));
+        if ((int64_t)e.image_w * e.image_h > (1ll << 28) || (int64_t)e.cdf_u_w * e.cdf_u_h > (1ll << 28))
+            return fail(VT_ERR_LIMIT, "environment image or CDF U larger than 2^28 entries");
     }
+    VT_HIP(hipSetDevice(c->device));
+    /* The new tables are built in buffers of their own and replace the
+     * context's only once every step has succeeded: a rejected or failed
+     * call leaves the previous environment in use, whole. */
+    float* img = nullptr;
+    float4* quad = nullptr;
+    float* cdfu = nullptr;
+    float* cdfv = nullptr;
+    if (e.use_image) {
+        const size_t ni = (size_t)e.image_w * e.image_h * 3, nu = (size_t)e.cdf_u_w * e.cdf_u_h;
+        hipError_t he = hipMalloc(&img, ni * 4);
+        if (he == hipSuccess) he = hipMalloc(&cdfu, nu * 4);
+        if (he == hipSuccess) he = hipMalloc(&cdfv, (size_t)e.cdf_v_n * 4);
+        if (he == hipSuccess) he = hipMalloc(&quad, (size_t)(e.image_w + 1) * (size_t)(e.image_h + 1) * 64);
+        if (he == hipSuccess) he = hipMemcpyAsync(img, e.image_rgb, ni * 4, hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess) he = hipMemcpyAsync(cdfu, e.cdf_u, nu * 4, hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(cdfv, e.cdf_v, (size_t)e.cdf_v_n * 4, hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess) he = vt::launch_env_quads(img, e.image_w, e.image_h, quad, c->stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+        if (he != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(c->stream);
+            dfree(img); dfree(cdfu); dfree(cdfv); dfree(quad);
+            return fail(VT_ERR_HIP, std::string("environment upload failed: ") + hipGetErrorString(he));
+        }
+    }
+    /* earlier renders may still read the old tables */
+    VT_HIP(hipStreamSynchronize(c->stream));
+    if (c->stream2) VT_HIP(hipStreamSynchronize(c->stream2));
+    dfree(c->d_img); dfree(c->d_quad); dfree(c->d_cdfu); dfree(c->d_cdfv);
+    c->d_img = img; c->d_quad = quad; c->d_cdfu = cdfu; c->d_cdfv = cdfv;
     e.image_rgb = nullptr; e.cdf_u = nullptr; e.cdf_v = nullptr;
     c->env = e;
     return VT_OK;
@@ -857,6 +879,14 @@ int vt_batch_create(vt_context* const* ctxs, int32_t n, vt_batch** out)
     for (int32_t i = 0; i < n; ++i) {
         if (!ctxs[i]) return fail(VT_ERR_ARG, "null context in batch");
         if (ctxs[i]->device != ctxs[0]->device) return fail(VT_ERR_ARG, "batch contexts must share one device");
+    }
+    {
+        /* one scene per context: a context listed twice would accumulate
+         * two scenes' samples into one frame in the same launch */
+        std::vector<const vt_context*> sorted(ctxs, ctxs + n);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(VT_ERR_ARG, "a context appears more than once in the batch");
     }
     VT_HIP(hipSetDevice(ctxs[0]->device));
     vt_batch* b = new vt_batch();
@@ -1479,10 +1509,21 @@ int vt_autotune(vt_context* c, uint32_t passes, vt_tuning* out)
     const size_t sum_bytes = c->d_sum64 ? (size_t)c->W * c->H * 3 * sizeof(double) : 0;
     void* save_acc = nullptr;
     void* save_sum = nullptr;
-    VT_HIP(hipMalloc(&save_acc, acc_bytes));
-    if (sum_bytes) VT_HIP(hipMalloc(&save_sum, sum_bytes));
-    VT_HIP(hipMemcpyAsync(save_acc, c->d_accum, acc_bytes, hipMemcpyDeviceToDevice, c->stream));
-    if (sum_bytes) VT_HIP(hipMemcpyAsync(save_sum, c->d_sum64, sum_bytes, hipMemcpyDeviceToDevice, c->stream));
+    {
+        hipError_t he = hipMalloc(&save_acc, acc_bytes);
+        if (he == hipSuccess && sum_bytes) he = hipMalloc(&save_sum, sum_bytes);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(save_acc, c->d_accum, acc_bytes, hipMemcpyDeviceToDevice, c->stream);
+        if (he == hipSuccess && sum_bytes)
+            he = hipMemcpyAsync(save_sum, c->d_sum64, sum_bytes, hipMemcpyDeviceToDevice, c->stream);
+        if (he != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(c->stream);
+            dfree(save_acc);
+            dfree(save_sum);
+            return fail(VT_ERR_HIP, std::string("autotune: saving the accumulation failed: ") + hipGetErrorString(he));
+        }
+    }
     const bool was_stats = c->stats_enabled;
     c->stats_enabled = false;
     const int mode0 = c->render_mode, b0 = c->dda_budget, r0 = c->refill_min, k0 = c->refill_chunk;

@@ -145,4 +145,50 @@ void CameraController::setEyeTarget(const V3f& eye, const V3f& target)
     if (m_parameters) m_parameters->setEyeTarget(eye, target);
 }
 
+/* The angle arithmetic is the controllers' own: the float offsets are
+ * promoted to double by M_PI and the sum rounded to float once
+ * (orbitCameraController.cpp:20-24, flyCameraController.cpp:20-24). */
+bool CameraController::onMouseMove(float dx, float dy, int buttons)
+{
+    if (!m_parameters) return false;
+    const float speed = 1.0f;
+    if (buttons & 2) {
+        const float theta = (float)((double)dy * M_PI * speed + m_parameters->rotationTheta());
+        const float phi = (float)(-(double)dx * 2.0 * M_PI * speed + m_parameters->rotationPhi());
+        if (m_mode == FLY)
+            m_parameters->orbitAroundEye(theta, phi);
+        else
+            m_parameters->orbitAroundTarget(theta, phi);
+        return true;
+    }
+    if (buttons & 4) {
+        if (m_mode == FLY) {
+            const V3f up = dy * m_parameters->upUnitVector() * 100.0f;
+            m_parameters->setEyeTarget(m_parameters->eye() + up, m_parameters->target() + up);
+        } else {
+            const float k = 1.05f;
+            const float d = m_parameters->distanceToTarget();
+            m_parameters->setDistanceFromTarget(dy > 0 ? d * k : d / k);
+        }
+        return true;
+    }
+    return false;
+}
+
+bool CameraController::onKeyPress(int key)
+{
+    if (!m_parameters || m_mode != FLY) return false;
+    const float step = 100.0f;
+    V3f move;
+    switch (key) {
+    case 'W': move = m_parameters->forwardUnitVector() * step; break;
+    case 'S': move = -(m_parameters->forwardUnitVector() * step); break;
+    case 'D': move = m_parameters->rightUnitVector() * step; break;
+    case 'A': move = -(m_parameters->rightUnitVector() * step); break;
+    default: return false;
+    }
+    m_parameters->setEyeTarget(m_parameters->eye() + move, m_parameters->target() + move);
+    return true;
+}
+
 }  // namespace voxeltoy

@@ -78,10 +78,26 @@ private:
 };
 
 /* The controller's programmatic operations (cameraController.cpp:4-24),
- * plus the orbit and fly motions the UI controllers drive. */
+ * plus the orbit and fly motions the UI controllers drive, and the input
+ * handling of the two controllers the reference switches between
+ * (orbitCameraController.cpp:15-46, flyCameraController.cpp:15-78).
+ * Mouse motion is in image-normalised units; buttons and keys use Qt's
+ * codes (right button 0x2, middle button 0x4, letter keys = upper-case
+ * ASCII). */
 class CameraController {
 public:
+    enum Mode { ORBIT, FLY };
     explicit CameraController(CameraParameters* p) : m_parameters(p) {}
+    /* ORBIT: right button orbits the eye around the target, middle button
+     * dollies by 1.05x.  FLY: right button turns the view around the eye,
+     * middle button moves eye and target along the up vector (100 units
+     * per unit of dy).  Returns whether the camera changed. */
+    bool onMouseMove(float dx, float dy, int buttons);
+    /* FLY: W/S move forward/back, D/A right/left by 100 units.  ORBIT
+     * takes no keys. */
+    bool onKeyPress(int key);
+    Mode mode() const { return m_mode; }
+    void setMode(Mode m) { m_mode = m; }
     void lookAt(const V3f& target);
     void setDistanceFromTarget(float distance);
     /* centre the bounds and back off so their diagonal fills the vertical fov */
@@ -95,15 +111,20 @@ public:
 
 private:
     CameraParameters* m_parameters;
+    Mode m_mode = ORBIT;
 };
 
 class Camera {
 public:
     Camera() : m_controller(&m_parameters) {}
-    Camera(const Camera& o) : m_parameters(o.m_parameters), m_controller(&m_parameters) {}
+    Camera(const Camera& o) : m_parameters(o.m_parameters), m_controller(&m_parameters)
+    {
+        m_controller.setMode(o.m_controller.mode());
+    }
     Camera& operator=(const Camera& o)
     {
         m_parameters = o.m_parameters;
+        m_controller.setMode(o.m_controller.mode());
         return *this;
     }
 
@@ -119,6 +140,18 @@ public:
     void setFovY(float fov) { m_parameters.setFovY(fov); }
     void setNearDistance(float d) { m_parameters.setNearDistance(d); }
     void setFarDistance(float d) { m_parameters.setFarDistance(d); }
+
+    /* Camera::CameraControllerMode (src/camera/camera.h:22-27): which
+     * controller interprets mouse and key input; a new camera orbits. */
+    enum CameraControllerMode { CCM_ORBIT, CCM_FLY };
+    void setCameraController(CameraControllerMode mode)
+    {
+        m_controller.setMode(mode == CCM_FLY ? CameraController::FLY : CameraController::ORBIT);
+    }
+    CameraControllerMode cameraControllerMode() const
+    {
+        return m_controller.mode() == CameraController::FLY ? CCM_FLY : CCM_ORBIT;
+    }
 
 private:
     CameraParameters m_parameters;

@@ -366,34 +366,31 @@ void Renderer::saveImage(const std::string& file)
     if (!writeImage(file, px.data(), W, H, &err)) fail("saveImage: " + err);
 }
 
-/* Orbit controller (orbitCameraController.cpp:15-40) and the renderer's
- * key handling (renderer.cpp:658-692). */
+/* Renderer::onMouseMove (renderer.cpp:658-669): pixel motion normalised by
+ * the image resolution, handed to the active camera controller. */
 bool Renderer::onMouseMove(int dx, int dy, int buttons)
 {
     const float ndx = (float)dx / m_renderSettings.m_imageResolution.x;
     const float ndy = (float)dy / m_renderSettings.m_imageResolution.y;
-    const CameraParameters& p = m_camera.parameters();
-    bool change = false;
-    if (buttons & 2) {
-        const float theta = (float)(ndy * M_PI * 1.0f + p.rotationTheta());
-        const float phi = (float)(-ndx * 2.0f * M_PI * 1.0f + p.rotationPhi());
-        m_camera.controller().orbit(theta, phi);
-        change = true;
-    } else if (buttons & 4) {
-        const float speed = 1.05f;
-        m_camera.controller().setDistanceFromTarget(ndy > 0 ? p.distanceToTarget() * speed
-                                                            : p.distanceToTarget() / speed);
-        change = true;
-    }
-    if (change) {
+    if (m_camera.controller().onMouseMove(ndx, ndy, buttons)) {
         updateCamera();
         m_numberSamples = 0;
+        return true;
     }
-    return change;
+    return false;
 }
 
+/* Renderer::onKeyPress (renderer.cpp:670-692): the active controller first
+ * (the fly controller's W/A/S/D), then 'F' focuses on the volume bounds.
+ * The space key's switch to the edit-mode preview integrator has no
+ * counterpart here. */
 bool Renderer::onKeyPress(int key)
 {
+    if (m_camera.controller().onKeyPress(key)) {
+        updateCamera();
+        m_numberSamples = 0;
+        return true;
+    }
     if (key == 'F' || key == 'f') {
         m_camera.controller().focusOnBounds(m_volumeBounds);
         m_numberSamples = 0;
@@ -823,6 +820,13 @@ int vtr_camera_set_film_size(vtr_renderer* r, float w, float h) { VTR_CAM(r->r.c
 int vtr_camera_set_lens_radius(vtr_renderer* r, float rad) { VTR_CAM(r->r.camera().setLensRadius(rad)) }
 int vtr_camera_set_fstop(vtr_renderer* r, float f) { VTR_CAM(r->r.camera().setFStop(f)) }
 int vtr_camera_set_fov_y(vtr_renderer* r, float f) { VTR_CAM(r->r.camera().setFovY(f)) }
+
+int vtr_camera_set_controller(vtr_renderer* r, int32_t mode)
+{
+    if (!r || (mode != 0 && mode != 1)) return VT_ERR_ARG;
+    r->r.camera().setCameraController(mode == 1 ? voxeltoy::Camera::CCM_FLY : voxeltoy::Camera::CCM_ORBIT);
+    return VT_OK;
+}
 
 int vtr_camera_block(vtr_renderer* r, vt_camera* out)
 {

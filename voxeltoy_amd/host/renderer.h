@@ -111,9 +111,9 @@ public:
     void saveImage(const std::string& file);
     void resetRender();
 
-    /* Orbit-controller mouse handling (orbitCameraController.cpp:15-40):
-     * buttons bit 1 (right) orbits, bit 2 (middle) dollies.  Keys: 'F'
-     * focuses on the volume bounds. */
+    /* Mouse and key input through the camera's active controller
+     * (Camera::setCameraController: orbit or fly), pixel motion normalised
+     * by the image resolution; 'F' focuses on the volume bounds. */
     bool onMouseMove(int dx, int dy, int buttons);
     bool onKeyPress(int key);
 

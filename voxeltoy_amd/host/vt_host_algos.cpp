@@ -180,20 +180,29 @@ void calculate_cdf(const float* fimg, int w, int h, float* cdfu, float* cdfv, fl
 
 void image_function(const float* rgb, int w, int h, int max_size, std::vector<float>& out, int& ow, int& oh)
 {
-    std::vector<float> lum((size_t)w * h);
-    for (size_t i = 0; i < lum.size(); ++i)
-        lum[i] = (rgb[3 * i] * 0.2126f + rgb[3 * i + 1] * 0.7152f) + rgb[3 * i + 2] * 0.0722f;
+    /* the RGB image is resized first, channel by channel, and the luminance
+     * taken from the resized channels (image.cpp:309-335 orders the steps
+     * so) */
     int nw = w, nh = h;
+    std::vector<float> chan[3];
+    for (int c = 0; c < 3; ++c) {
+        chan[c].resize((size_t)w * h);
+        for (size_t i = 0; i < chan[c].size(); ++i) chan[c][i] = rgb[3 * i + c];
+    }
     if (std::max(w, h) > max_size) {
         const int mxd = std::max(w, h);
         nw = (int)((float)w / (float)mxd * (float)max_size);
         nh = (int)((float)h / (float)mxd * (float)max_size);
-        std::vector<float> t1, t2, t3;
-        resample_rows(lum, w, h, t1, nw);
-        transpose(t1, nw, h, t2);
-        resample_rows(t2, h, nw, t3, nh);
-        transpose(t3, nh, nw, lum);
+        for (int c = 0; c < 3; ++c) {
+            std::vector<float> t1, t2, t3;
+            resample_rows(chan[c], w, h, t1, nw);
+            transpose(t1, nw, h, t2);
+            resample_rows(t2, h, nw, t3, nh);
+            transpose(t3, nh, nw, chan[c]);
+        }
     }
+    std::vector<float> lum((size_t)nw * nh);
+    for (size_t i = 0; i < lum.size(); ++i) lum[i] = (chan[0][i] * 0.2126f + chan[1][i] * 0.7152f) + chan[2][i] * 0.0722f;
     const float g1 = std::exp(-2.0f * ((2.0f / 3.0f) * (2.0f / 3.0f)));
     const float k1[3] = { g1, 1.0f, g1 };
     float ksum = 0.0f;

@@ -8,20 +8,46 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
+#include <utility>
 
 namespace voxeltoy {
+
+/* Vector types of other libraries (Imath::V2i, Imath::V3f, ...) convert to
+ * and from these by their x, y (and z) members, so host code written
+ * against Imath-typed fields can assign its own vectors to them:
+ * has_xy<T> / has_xyz<T> select the two- and three-component types. */
+namespace detail {
+template <typename T, typename = void>
+struct has_z : std::false_type {};
+template <typename T>
+struct has_z<T, std::void_t<decltype(std::declval<const T&>().z)>> : std::true_type {};
+template <typename T, typename = void>
+struct has_xy : std::false_type {};
+template <typename T>
+struct has_xy<T, std::void_t<decltype(std::declval<const T&>().x), decltype(std::declval<const T&>().y)>>
+    : std::true_type {};
+}  // namespace detail
+template <typename T>
+using enable_if_vec2 = std::enable_if_t<detail::has_xy<T>::value && !detail::has_z<T>::value, int>;
+template <typename T>
+using enable_if_vec3 = std::enable_if_t<detail::has_xy<T>::value && detail::has_z<T>::value, int>;
 
 struct V2f {
     float x = 0.0f, y = 0.0f;
     V2f() = default;
     V2f(float a, float b) : x(a), y(b) {}
     explicit V2f(float a) : x(a), y(a) {}
+    template <typename T, enable_if_vec2<T> = 0>
+    V2f(const T& v) : x((float)v.x), y((float)v.y) {}
 };
 
 struct V2i {
     int x = 0, y = 0;
     V2i() = default;
     V2i(int a, int b) : x(a), y(b) {}
+    template <typename T, enable_if_vec2<T> = 0>
+    V2i(const T& v) : x((int)v.x), y((int)v.y) {}
 };
 
 struct V3f {
@@ -29,6 +55,8 @@ struct V3f {
     V3f() = default;
     V3f(float a, float b, float c) : x(a), y(b), z(c) {}
     explicit V3f(float a) : x(a), y(a), z(a) {}
+    template <typename T, enable_if_vec3<T> = 0>
+    V3f(const T& v) : x((float)v.x), y((float)v.y), z((float)v.z) {}
     float& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
     float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
     V3f operator+(const V3f& o) const { return V3f(x + o.x, y + o.y, z + o.z); }
@@ -52,6 +80,8 @@ struct V3i {
     V3i() = default;
     V3i(int a, int b, int c) : x(a), y(b), z(c) {}
     explicit V3i(int a) : x(a), y(a), z(a) {}
+    template <typename T, enable_if_vec3<T> = 0>
+    V3i(const T& v) : x((int)v.x), y((int)v.y), z((int)v.z) {}
     bool operator==(const V3i& o) const { return x == o.x && y == o.y && z == o.z; }
 };
 

@@ -72,6 +72,7 @@ SIGNATURES = {
     "vtr_camera_set_lens_radius": (C.c_int, [_R, C.c_float]),
     "vtr_camera_set_fstop": (C.c_int, [_R, C.c_float]),
     "vtr_camera_set_fov_y": (C.c_int, [_R, C.c_float]),
+    "vtr_camera_set_controller": (C.c_int, [_R, C.c_int32]),
     "vtr_camera_block": (C.c_int, [_R, C.POINTER(api.Camera)]),
     "vtr_on_mouse_move": (C.c_int, [_R, C.c_int32, C.c_int32, C.c_int32]),
     "vtr_on_key_press": (C.c_int, [_R, C.c_int32]),
@@ -261,6 +262,10 @@ class Renderer:
 
     def set_fov_y(self, f: float):
         self._check(lib().vtr_camera_set_fov_y(self._r, f))
+
+    def set_camera_controller(self, mode: int):
+        """0 = orbit (CCM_ORBIT), 1 = fly (CCM_FLY)."""
+        self._check(lib().vtr_camera_set_controller(self._r, mode))
 
     def on_mouse_move(self, dx: int, dy: int, buttons: int) -> bool:
         return bool(self._check(lib().vtr_on_mouse_move(self._r, dx, dy, buttons)))
