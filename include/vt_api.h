@@ -266,6 +266,18 @@ int vt_trace_first_hit(vt_context* ctx, uint64_t seed, uint32_t frame, int32_t* 
 /* Injected rays (origin[3], dir[3] each), same record. */
 int vt_trace_rays(vt_context* ctx, int32_t n, const float* origins, const float* dirs, int32_t* rec);
 
+/* Allocation-free variant of vt_trace_rays.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (picking on every
+ * mouse move in an interactive loop, serving loops and harnesses that forbid
+ * hipMalloc, code around stream capture).  Same arguments, records and errors
+ * as vt_trace_rays for n <= VT_NOALLOC_MAX_RAYS (VT_ERR_LIMIT beyond, nothing
+ * launched).  Rays and records are staged in a fixed device array of
+ * VT_NOALLOC_MAX_RAYS x 56 bytes (896 KiB), one per device, that the library
+ * holds until the process ends; calls that would overlap on one device take
+ * turns on it (each returns after its records are on the host). */
+#define VT_NOALLOC_MAX_RAYS 16384
+int vt_trace_rays_noalloc(vt_context* ctx, int32_t n, const float* origins, const float* dirs, int32_t* rec);
+
 /* ---- voxelizer ----------------------------------------------------------
  * GPUVoxelizer::voxelizeMesh (src/voxelize/gpuVoxelizer.cpp:46-72 with
  * voxelize.vs/gs).  Vertices are float xyz, triangles uint32 index

@@ -116,6 +116,7 @@ SIGNATURES = {
                                        C.POINTER(C.c_double)]),
     "vt_trace_first_hit": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, i32p]),
     "vt_trace_rays": (C.c_int, [C.c_void_p, C.c_int32, f32p, f32p, i32p]),
+    "vt_trace_rays_noalloc": (C.c_int, [C.c_void_p, C.c_int32, f32p, f32p, i32p]),
     "vt_voxelize": (C.c_int, [C.c_void_p, f32p, C.c_int64, u32p, C.c_int64, f32p, C.c_int32, C.c_int32,
                               C.c_void_p]),
     "vt_camera_matrices": (C.c_int, [f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_int32, C.c_int32,
@@ -417,6 +418,16 @@ class Context:
         n = len(o) // 3
         out = np.zeros((n, 8), np.int32)
         _check(lib().vt_trace_rays(self._c, n, o, d, out.reshape(-1)))
+        return out
+
+    def trace_rays_noalloc(self, origins: np.ndarray, dirs: np.ndarray) -> np.ndarray:
+        """trace_rays through the library's fixed per-device staging array
+        (no device allocation; at most VT_NOALLOC_MAX_RAYS rays)."""
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1)
+        n = len(o) // 3
+        out = np.zeros((n, 8), np.int32)
+        _check(lib().vt_trace_rays_noalloc(self._c, n, o, d, out.reshape(-1)))
         return out
 
     def voxelize(self, verts: np.ndarray, tris: np.ndarray, M: np.ndarray, mode: int = 0, material_offset: int = 0,

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -60,6 +61,47 @@ void dfree(T*& p)
     p = nullptr;
 }
 
+}  // namespace
+
+/* ---- staging memory of vt_trace_rays_noalloc -------------------------------------
+ * A fixed device array, one copy per device, held until the process ends:
+ * origins (VT_NOALLOC_MAX_RAYS x 12 B), directions (x 12 B) and hit records
+ * (x 32 B).  Its host-side name is a stand-in, not a device address, so host
+ * code uses only the address that hipGetSymbolAddress returns for each
+ * device (looked up once per device by vt_create, with that device current). */
+#define VT_STR_(x) #x
+#define VT_STR(x) VT_STR_(x)
+namespace {
+constexpr size_t VT_NOALLOC_O = 0;
+constexpr size_t VT_NOALLOC_D = (size_t)VT_NOALLOC_MAX_RAYS * 12u;
+constexpr size_t VT_NOALLOC_R = (size_t)VT_NOALLOC_MAX_RAYS * 24u;
+constexpr size_t VT_NOALLOC_BYTES = (size_t)VT_NOALLOC_MAX_RAYS * 56u;
+constexpr int VT_MAX_DEVICES = 64;
+static_assert(VT_NOALLOC_R % 256 == 0, "record staging must stay aligned");
+
+std::mutex g_noalloc_lock;                       /* guards the table and serializes calls */
+void* g_noalloc_ptr[VT_MAX_DEVICES] = {};
+
+/* Device address of the current device's copy of `scratch`. */
+int noalloc_lookup(int device);
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[VT_NOALLOC_BYTES];
+
+namespace {
+int noalloc_lookup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_lock);
+    if (g_noalloc_ptr[device]) return VT_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return fail(VT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_noalloc_ptr[device] = ptr;
+    return VT_OK;
+}
 }  // namespace
 
 struct vt_context {
@@ -408,7 +450,9 @@ int vt_create(int device, vt_context** out)
     int n = 0;
     VT_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(VT_ERR_ARG, "no such device");
+    if (device >= VT_MAX_DEVICES) return fail(VT_ERR_LIMIT, "device index beyond the library's table");
     VT_HIP(hipSetDevice(device));
+    if (int r = noalloc_lookup(device)) return r;
     vt_context* c = new vt_context();
     c->device = device;
     hipDeviceProp_t prop;
@@ -1655,6 +1699,32 @@ int vt_trace_rays(vt_context* c, int32_t n, const float* o, const float* dir, in
     VT_HIP(hipMemcpyAsync(rec, d_r, (size_t)n * 32, hipMemcpyDeviceToHost, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
     dfree(d_o); dfree(d_d); dfree(d_r);
+    return VT_OK;
+}
+
+int vt_trace_rays_noalloc(vt_context* c, int32_t n, const float* o, const float* dir, int32_t* rec)
+{
+    if (!c || n < 0 || (n > 0 && (!o || !dir || !rec))) return fail(VT_ERR_ARG, "bad argument");
+    if (n > VT_NOALLOC_MAX_RAYS)
+        return fail(VT_ERR_LIMIT, "vt_trace_rays_noalloc: more rays than its staging limit of "
+                                  VT_STR(VT_NOALLOC_MAX_RAYS) " (use vt_trace_rays)");
+    if (!c->has_volume) return fail(VT_ERR_STATE, "no volume");
+    if (n == 0) return VT_OK;
+    VT_HIP(hipSetDevice(c->device));
+    /* one call at a time uses the device's staging array; a call returns only
+     * after its records are back on the host */
+    std::lock_guard<std::mutex> lk(g_noalloc_lock);
+    unsigned char* base = (unsigned char*)g_noalloc_ptr[c->device];
+    if (!base) return fail(VT_ERR_STATE, "staging array of the device was not looked up");
+    float* d_o = (float*)(base + VT_NOALLOC_O);
+    float* d_d = (float*)(base + VT_NOALLOC_D);
+    int32_t* d_r = (int32_t*)(base + VT_NOALLOC_R);
+    vt::TraceArgs a = trace_args(c);
+    VT_HIP(hipMemcpyAsync(d_o, o, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipMemcpyAsync(d_d, dir, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_trace_rays(a, n, d_o, d_d, d_r, c->stream));
+    VT_HIP(hipMemcpyAsync(rec, d_r, (size_t)n * 32, hipMemcpyDeviceToHost, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
     return VT_OK;
 }
 
