@@ -15,7 +15,14 @@ environment lookup) over the kernel's HIP-event time.  The CPU baseline is
 the C oracle (OpenMP over the host cores) on the same frame for as many of
 the step's passes as fit a ~12 s budget.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+The same line carries the metric's other half (leave out with --no-extras):
+"batch", 64 mixed scenes rendered as one batch against the same renders one
+after another, and "time_to_rmse", the 1920x1080 scene at one sample per
+pixel per frame, 256 frames of the plain tracer against the radiance cache
+and plain path truncation given the same time, scored against a
+16384-sample FP64 reference.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras]
 """
 from __future__ import annotations
 
@@ -91,13 +98,21 @@ def main():
     ap.add_argument("--mode", type=int, default=1, help="0 megakernel, 1 wavefront (default)")
     ap.add_argument("--streams", type=int, default=1,
                     help="wavefront: 1 (default) runs the step on one stream, 2 runs its pass halves on two streams")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run only the streaming copy kernel (--copy-bytes read and written, --copy-iters times), "
+                         "the known-size reference for the FETCH_SIZE / WRITE_SIZE counter passes")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--copy-iters", type=int, default=3)
     ap.add_argument("--workload", default="full", choices=["full", "batch", "cache", "voxelize"],
                     help="full: the 256^3 full-shading scene (default); batch: 64 mixed scenes, one batched "
                          "graph launch against 64 sequential renders; cache: time to RMSE 1e-2 against an FP64 "
                          "reference, plain tracer against the radiance cache; voxelize: 200k triangles into 512^3, "
                          "then 1920x1080, 8 spp, 6 bounces")
-    ap.add_argument("--ref-spp", type=int, default=4096, help="cache workload: passes of the FP64 reference")
-    ap.add_argument("--budget-s", type=float, default=2.0, help="cache workload: render time of each contender")
+    ap.add_argument("--ref-spp", type=int, default=16384, help="time-to-RMSE race: passes of the FP64 reference")
+    ap.add_argument("--frames", type=int, default=256,
+                    help="time-to-RMSE race: frames of the plain tracer, whose time every contender gets")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="full workload only: leave out the batch and time-to-RMSE objects")
     ap.add_argument("--cache-bounce", type=int, default=2, help="cache workload: bounce at which paths end")
     ap.add_argument("--train-paths", type=int, default=65536,
                     help="cache workload: training paths per frame (65536 give about 58k samples per frame)")
@@ -111,6 +126,12 @@ def main():
 
     from voxeltoy_amd import api, workloads
 
+    if args.calibrate:
+        with api.Context(0) as ctx:
+            ms = ctx.copy_probe(args.copy_bytes, args.copy_iters)
+        print(json.dumps({"calibrate": "copy", "bytes": args.copy_bytes, "iters": args.copy_iters,
+                          "kernel_ms": ms, "gbps_read_plus_write": 2 * args.copy_bytes / (ms / 1e3) / 1e9}))
+        return None
     if args.workload == "batch":
         return bench_batch(args, api, workloads)
     if args.workload == "cache":
@@ -205,8 +226,15 @@ def main():
                      "render_achieved_gbs": round(render_gbs, 1)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
     ctx.close()
+    if not args.no_extras:
+        # the metric's other half: batched against sequential scenes, and time
+        # to RMSE of the plain tracer against the radiance cache
+        out["batch"] = batch_race(api, workloads, steps=3, warmup=1)
+        out["time_to_rmse"] = rmse_race(api, workloads, frames=args.frames, ref_spp=args.ref_spp,
+                                        cache_bounce=args.cache_bounce, train_paths=args.train_paths,
+                                        train_extra=args.train_extra)
+    print(json.dumps(out))
 
 
 def bench_voxelize(args, api, workloads):
@@ -296,10 +324,11 @@ def bench_voxelize(args, api, workloads):
     ctx.close()
 
 
-def bench_batch(args, api, workloads):
+def batch_race(api, workloads, steps: int = 5, warmup: int = 1) -> dict:
     """64 scenes (grid 32^3-512^3, image 128^2-1024^2, 1-64 spp, 5-60%
-    occupancy, 4 bounces): one batched wavefront launch replayed from a HIP
-    graph, against the same 64 renders launched one scene after another."""
+    occupancy, 4 bounces): one batched render (HIP graph replay) against the
+    same 64 renders launched one scene after another, each on its own
+    context stream; wall times with the device synchronised."""
     wls = workloads.batch(64)
     ctxs = []
     for wl in wls:
@@ -325,150 +354,195 @@ def bench_batch(args, api, workloads):
         for c in ctxs:
             c.synchronize()
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         batched()
         sequential()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         batched()
-    t_b = (time.perf_counter() - t0) / args.steps
+    t_b = (time.perf_counter() - t0) / steps
     k_ms = b.stats()["kernel_ms"]
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         sequential()
-    t_s = (time.perf_counter() - t0) / args.steps
+    t_s = (time.perf_counter() - t0) / steps
+    b.close()
+    for c in ctxs:
+        c.close()
+    return {"workload": "batch of 64 scenes: grid 32^3-512^3, image 128^2-1024^2, 1-64 spp, 5-60% occupancy, "
+                        "70/20/10 lambert/metal/emissive, gradient env, 4 bounces",
+            "scenes": len(wls), "steps": steps, "rays_per_batch": int(rays), "samples_per_batch": int(st["samples"]),
+            "batched_ms": round(t_b * 1000.0, 3), "batched_kernel_ms": round(k_ms, 3),
+            "batched_mrays_per_s": round(rays / t_b / 1e6, 2),
+            "sequential_ms": round(t_s * 1000.0, 3), "sequential_mrays_per_s": round(rays / t_s / 1e6, 2),
+            "speedup_vs_sequential": round(t_s / t_b, 3)}
+
+
+def bench_batch(args, api, workloads):
+    res = batch_race(api, workloads, args.steps, args.warmup)
     out = {
         "metric": METRIC,
-        "value": round(rays / t_b / 1e6, 2),
+        "value": res["batched_mrays_per_s"],
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_b * 1000.0, 3),
+        "ms_per_step": res["batched_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "batch of 64 scenes: grid 32^3-512^3, image 128^2-1024^2, 1-64 spp, "
-                               "5-60% occupancy, 70/20/10 lambert/metal/emissive, gradient env, 4 bounces",
-                   "scenes": len(wls), "rays_per_step": int(rays), "samples_per_step": int(st["samples"]),
-                   "batched_graph_ms": round(t_b * 1000.0, 3), "batched_kernel_ms": round(k_ms, 3),
-                   "sequential_ms": round(t_s * 1000.0, 3),
-                   "sequential_mrays_per_s": round(rays / t_s / 1e6, 2),
-                   "speedup_vs_sequential": round(t_s / t_b, 3)},
+        "config": {"workload": res["workload"]},
+        "batch": res,
         "roofline": None,
         "cpu_baseline": None,
     }
     print(json.dumps(out))
-    b.close()
-    for c in ctxs:
-        c.close()
 
 
-def bench_cache(args, api, workloads, target=1e-2):
-    """Time to RMSE `target` (over pixels x RGB of the tonemapped mean)
-    against an FP64-accumulated reference of the same scene: the plain
-    tracer (8 bounces) against radiance-cache rendering (paths end into the
-    cache at bounce --cache-bounce; --train-paths training paths per frame
-    trace --train-extra more bounces and train the cache), one sample per
-    pixel per frame, each given the same render-time budget; a third run
-    ends the paths at the same bounce into an untrained zero cache."""
+def rmse_race(api, workloads, frames: int = 256, ref_spp: int = 16384, cache_bounce: int = 2,
+              train_paths: int = 65536, train_extra: int = 1, levels=(1e-2, 3e-3)) -> dict:
+    """Time to RMSE against an FP64 reference on the 1920x1080 full-shading
+    scene, one sample per pixel per frame: the plain tracer (8 bounces) for
+    `frames` frames, whose wall time is the budget the other contenders get:
+    radiance-cache rendering (paths end into the cache at bounce
+    `cache_bounce`; `train_paths` training paths per frame trace
+    `train_extra` more bounces and train it) and, as the control, the same
+    paths ending into an untrained cache that predicts 0 (plain truncation).
+    RMSE is over pixels x RGB of the tonemapped running mean (the reference
+    output convention), on one pixel mask for all contenders: the pixels
+    finite in the reference and in every contender's last image (a non-finite
+    sample stays in a running mean, so such a pixel is non-finite from then on
+    in every later image).  Frame times are wall times with the device
+    synchronised; reading the images for the curve is not timed."""
     import numpy as np
     wl = workloads.full(width=1920, height=1080, spp=1, bounces=8)
+    t_ref = time.perf_counter()
     ref_ctx = api.Context(0)
     workloads.load(ref_ctx, wl, fp64=True)
     done = 0
-    while done < args.ref_spp:
-        k = min(256, args.ref_spp - done)
+    while done < ref_spp:
+        k = min(512, ref_spp - done)
         ref_ctx.render(wl.seed + 1, done, k)
         done += k
-    ref = (ref_ctx.read_sum64() / args.ref_spp).astype(np.float64)
+    ref = ref_ctx.read_sum64() / ref_spp
     ref_ctx.close()
-    finite = np.isfinite(ref).all(axis=-1)
+    t_ref = time.perf_counter() - t_ref
+    ref_finite = np.isfinite(ref).all(axis=-1)
 
-    def rmse(img):
+    def sq_err(img):
+        """Per-pixel squared error summed over RGB (float64)."""
         d = img[..., :3].astype(np.float64) - ref
-        d = d[finite & np.isfinite(img[..., :3]).all(axis=-1)]
-        return float(np.sqrt(np.mean(d * d)))
+        return np.einsum("hwc,hwc->hw", d, d)
 
-    def crossing(curve, level):
-        """Time (s) at which the RMSE curve first reaches `level`,
-        interpolated between checkpoints."""
-        prev = None
-        for t, e in curve:
-            if e <= level:
-                if prev is not None and prev[1] > level:
-                    return round((prev[0] + (prev[1] - level) / (prev[1] - e) * (t - prev[0])) / 1000.0, 3)
-                return round(t / 1000.0, 3)
-            prev = (t, e)
-        return None
-
-    def run(step_fn, read_fn):
-        """Frames until the time budget is spent; the RMSE is read at frames
-        1, 2, 4, ... and every 64 frames (reading is not timed)."""
-        t, n, curve, first = 0.0, 0, [], None
-        prev = None
-        while t < args.budget_s * 1000.0:
-            t += step_fn(n)
+    def run(step_fn, read_fn, stop):
+        """Frames while `stop(frames, elapsed_ms, mean_frame_ms)` is false;
+        images are read at frames 1, 2, 4, ... and every 32 frames and at
+        the end."""
+        t, n, marks = 0.0, 0, []
+        while not stop(n, t, t / n if n else 0.0):
+            t0 = time.perf_counter()
+            step_fn(n)
+            t += (time.perf_counter() - t0) * 1000.0
             n += 1
-            if (n & (n - 1)) == 0 or n % 64 == 0:
-                e = rmse(read_fn())
-                curve.append((round(t, 2), round(e, 5)))
-                if first is None and e <= target:
-                    # interpolate between this and the previous checkpoint
-                    if prev is not None and prev[1] > target:
-                        f = (prev[1] - target) / (prev[1] - e)
-                        first = prev[0] + f * (t - prev[0])
-                    else:
-                        first = t
-                prev = (t, e)
-        e = rmse(read_fn())
-        curve.append((round(t, 2), round(e, 5)))
-        return {"frames": n, "ms": round(t, 1), "final_rmse": curve[-1][1],
-                "time_to_target_s": None if first is None else round(first / 1000.0, 3),
-                "time_to_3e-3_s": crossing(curve, 3e-3), "curve": curve}
+            if (n & (n - 1)) == 0 or n % 32 == 0:
+                marks.append((n, t, sq_err(read_fn())))
+        if not marks or marks[-1][0] != n:
+            marks.append((n, t, sq_err(read_fn())))
+        return {"frames": n, "ms": t, "marks": marks, "nonfinite": int((~np.isfinite(marks[-1][2])).sum())}
 
     plain_ctx = api.Context(0)
     workloads.load(plain_ctx, wl)
 
     def plain_step(n):
         plain_ctx.render(wl.seed, n, 1)
-        return plain_ctx.stats()["kernel_ms"]
+        plain_ctx.synchronize()
 
-    plain = run(plain_step, plain_ctx.read_accum)
+    plain = run(plain_step, plain_ctx.read_accum, lambda n, t, f: n >= frames)
+    plain_ctx.close()
+    budget_ms = plain["ms"]
+
+    def within_budget(n, t, f):
+        return n > 0 and t + f > budget_ms
+
     cache_ctx = api.Context(0)
     workloads.load(cache_ctx, wl)
     cache = api.RadianceCache(cache_ctx, seed=1)
+    cstats = []
 
     def cache_step(n):
-        st = cache.render(wl.seed, n, 1, cache_bounce=args.cache_bounce, train_paths=args.train_paths, lr=3e-3,
-                          train_extra=args.train_extra)
-        stats.append(st)
-        return st["ms"]
+        cstats.append(cache.render(wl.seed, n, 1, cache_bounce=cache_bounce, train_paths=train_paths, lr=3e-3,
+                                   train_extra=train_extra))
 
-    stats = []
+    cached = run(cache_step, cache_ctx.read_accum, within_budget)
+    cache.close()
+    cache_ctx.close()
 
-    cached = run(cache_step, cache_ctx.read_accum)
-    tp, tc = plain["time_to_target_s"], cached["time_to_target_s"]
-
-    # the same paths ending into a cache that predicts 0 and never trains:
-    # what the learned network adds over plain path truncation
     cut_ctx = api.Context(0)
     workloads.load(cut_ctx, wl)
     zero = api.RadianceCache(cut_ctx, seed=1)
     zero.set_params(np.zeros(zero.n_params, np.float32))
 
     def cut_step(n):
-        return zero.render(wl.seed, n, 1, cache_bounce=args.cache_bounce, train_paths=0)["ms"]
+        zero.render(wl.seed, n, 1, cache_bounce=cache_bounce, train_paths=0)
 
-    truncated = run(cut_step, cut_ctx.read_accum)
+    truncated = run(cut_step, cut_ctx.read_accum, within_budget)
+    zero.close()
+    cut_ctx.close()
+
+    runs = {"plain": plain, "cache": cached, "truncation": truncated}
+    mask = ref_finite.copy()
+    for r in runs.values():
+        mask &= np.isfinite(r["marks"][-1][2])
+    n_mask = int(mask.sum())
+
+    def summary(r):
+        curve = [(round(t / 1000.0, 4), float(np.sqrt(e[mask].sum() / (3 * n_mask)))) for _, t, e in r["marks"]]
+        out = {"frames": r["frames"], "seconds": round(r["ms"] / 1000.0, 4),
+               "ms_per_frame": round(r["ms"] / max(1, r["frames"]), 3),
+               "rmse_at_equal_time": round(curve[-1][1], 6), "nonfinite_pixels": r["nonfinite"]}
+        for level in levels:
+            hit, prev = None, None
+            for t, e in curve:
+                if e <= level:
+                    hit = t if prev is None or prev[1] <= level else prev[0] + (prev[1] - level) / (prev[1] - e) * (
+                        t - prev[0])
+                    break
+                prev = (t, e)
+            out[f"seconds_to_{level:g}"] = None if hit is None else round(hit, 4)
+        out["curve"] = [(t, round(e, 6)) for t, e in curve]
+        return out
+
+    res = {name: summary(r) for name, r in runs.items()}
+    res["plain"]["note"] = f"{frames} frames; its time is every contender's budget"
+    for lv in levels:
+        key = f"seconds_to_{lv:g}"
+        tp, tc, tt = res["plain"][key], res["cache"][key], res["truncation"][key]
+        res[f"cache_speedup_to_{lv:g}_vs_plain"] = None if tp is None or tc is None else round(tp / tc, 3)
+        res[f"cache_speedup_to_{lv:g}_vs_truncation"] = None if tt is None or tc is None else round(tt / tc, 3)
+    res.update({
+        "workload": "full-shading 256^3 scene at 1920x1080, 1 spp per frame, 8 bounces; FP64 reference of "
+                    f"{ref_spp} spp (own seed); RMSE over pixels x RGB of the tonemapped mean",
+        "reference_spp": ref_spp, "reference_seconds": round(t_ref, 2),
+        "reference_nonfinite_pixels": int((~ref_finite).sum()), "scored_pixels": n_mask,
+        "cache_bounce": cache_bounce, "train_paths_per_frame": train_paths, "train_extra_bounces": train_extra,
+        "cache_mean_samples_per_frame": round(sum(x["samples"] for x in cstats) / max(1, len(cstats)), 1),
+        "cache_mean_queries_per_frame": round(sum(x["queries"] for x in cstats) / max(1, len(cstats)), 1),
+    })
+    return res
+
+
+def bench_cache(args, api, workloads):
+    """The time-to-RMSE race on its own line (value: the cache's time to 1e-2)."""
+    res = rmse_race(api, workloads, frames=args.frames, ref_spp=args.ref_spp, cache_bounce=args.cache_bounce,
+                    train_paths=args.train_paths, train_extra=args.train_extra)
     out = {
         "metric": METRIC,
-        "value": tc,
+        "value": res["cache"]["seconds_to_0.01"],
         "unit": "s",
         "n_gpus": 1,
-        "steps": plain["frames"] + cached["frames"],
+        "steps": res["plain"]["frames"] + res["cache"]["frames"] + res["truncation"]["frames"],
         "warmup": 0,
         "ms_per_step": None,
         "higher_is_better": False,
@@ -476,24 +550,12 @@ def bench_cache(args, api, workloads, target=1e-2):
         "vs_baseline": None,
         "dtype": "f32 tracer, fp16 mlp",
         "data": "synthetic",
-        "config": {"workload": "full-shading 256^3 scene at 1920x1080, 1 spp per frame; FP64 reference of "
-                               f"{args.ref_spp} spp; target RMSE {target} (tonemapped mean, pixels x RGB)",
-                   "plain": plain, "cache": cached, "truncated_untrained": truncated,
-                   "cache_bounce": args.cache_bounce,
-                   "train_paths_per_frame": args.train_paths, "train_extra_bounces": args.train_extra,
-                   "cache_last_frame": stats[-1] if stats else None,
-                   "cache_mean_samples_per_frame": round(sum(x["samples"] for x in stats) / max(1, len(stats)), 1),
-                   "cache_mean_queries_per_frame": round(sum(x["queries"] for x in stats) / max(1, len(stats)), 1),
-                   "speedup_time_to_target": None if (tp is None or tc is None) else round(tp / tc, 3)},
+        "config": {"workload": res["workload"]},
+        "time_to_rmse": res,
         "roofline": None,
         "cpu_baseline": None,
     }
     print(json.dumps(out))
-    cache.close()
-    zero.close()
-    plain_ctx.close()
-    cache_ctx.close()
-    cut_ctx.close()
 
 
 if __name__ == "__main__":

@@ -310,14 +310,16 @@ int vt_mesh_transform(const float* bmin, const float* bmax, const int32_t* res, 
 int32_t vt_prune_emissive(const int32_t* material_offsets, int32_t nx, int32_t ny, int32_t nz,
                           int32_t* emissive, int32_t n_emissive);
 
-/* Device evaluation of the elementary functions used by the kernels, for
+/* ---- diagnostics (not part of the renderer's interface) -------------------
+ * Device evaluation of the elementary functions used by the kernels, for
  * checking that device and host arithmetic agree bit for bit.  fn: 0 sqrt,
  * 1 a/b, 2 sin, 3 cos, 4 acos, 5 atan2(a,b), 6 log2, 7 exp2, 8 pow(a,b). */
 int vt_device_math(vt_context* ctx, int32_t fn, int32_t n, const float* a, const float* b, float* out);
 
-/* Streaming copy of `bytes` bytes between two device buffers, `iters`
- * times, with a plain vectorized copy kernel; returns the mean kernel time.
- * A known-size reference for hardware byte counters and the HBM roofline. */
+/* Diagnostic: streaming copy of `bytes` bytes between two device buffers,
+ * `iters` times, with a plain vectorized copy kernel; returns the mean kernel
+ * time.  A known-size reference for hardware byte counters and the HBM
+ * roofline (bench.py --calibrate). */
 int vt_copy_probe(vt_context* ctx, uint64_t bytes, int32_t iters, double* kernel_ms);
 
 #ifdef __cplusplus
