@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=16384, help="time-to-RMSE race: passes of the FP64 reference")
     ap.add_argument("--frames", type=int, default=256,
                     help="time-to-RMSE race: frames of the plain tracer, whose time every contender gets")
+    ap.add_argument("--only", choices=["plain", "cache", "truncation"], default=None,
+                    help="cache workload: run --frames frames of this contender alone (for a kernel trace)")
     ap.add_argument("--no-extras", action="store_true",
                     help="full workload only: leave out the batch and time-to-RMSE objects")
     ap.add_argument("--cache-bounce", type=int, default=2, help="cache workload: bounce at which paths end")
@@ -295,6 +297,20 @@ def bench_voxelize(args, api, workloads):
     # voxelizer: 3 uint32 indices + 3 float3 vertices per triangle read, 1 B per covered voxel written
     vox_bytes = len(tris) * (12 + 36) + covered
     vox_gbs = vox_bytes / (v_ms / 1000.0) / 1e9
+    # the reference's mesh fixture (resources/bunny.obj, 4968 triangles) at
+    # 512^3: large triangles, a wave each
+    from voxeltoy_amd import renderer as vr
+    bv, bt, _, _ = vr.load_obj(os.path.join(ROOT, "tests", "golden", "bunny.obj"))
+    bM = api.mesh_transform(bv.min(0), bv.max(0), [res] * 3)
+    bctx = api.Context(0)
+    bctx.set_volume_u8(np.zeros((res, res, res), np.uint8), id_offset, mat)
+    b_cov = int(bctx.voxelize(bv, bt, bM, mode=0, material_offset=0, want_occupancy=True).sum())
+    b_ms = []
+    for _ in range(max(3, args.steps)):
+        bctx.voxelize(bv, bt, bM, mode=0, material_offset=0, want_occupancy=False)
+        b_ms.append(bctx.stats()["kernel_ms"])
+    bctx.close()
+    b_ms = sum(b_ms) / len(b_ms)
     out = {
         "metric": METRIC,
         "value": round(rays / (ms_per_step / 1000.0) / 1e6, 2),
@@ -314,7 +330,10 @@ def bench_voxelize(args, api, workloads):
                    "voxelize_kernel_ms": round(v_ms, 3),
                    "voxelize_mtri_per_s": round(len(tris) / (v_ms / 1000.0) / 1e6, 1),
                    "render_kernel_ms": round(r_ms, 3),
-                   "render_mrays_per_s": round(rays / (r_ms / 1000.0) / 1e6, 2), "streams": args.streams},
+                   "render_mrays_per_s": round(rays / (r_ms / 1000.0) / 1e6, 2), "streams": args.streams,
+                   "bunny_512": {"triangles": int(len(bt)), "covered_voxels": b_cov,
+                                 "voxelize_kernel_ms": round(b_ms, 4),
+                                 "voxelize_mtri_per_s": round(len(bt) / (b_ms / 1000.0) / 1e6, 2)}},
         "roofline": {"bound": "hbm", "kernel": "voxelize_kernel", "achieved": round(vox_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(vox_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": int(vox_bytes), "avg_launch_ms": round(v_ms, 4)},
@@ -326,9 +345,10 @@ def bench_voxelize(args, api, workloads):
 
 def batch_race(api, workloads, steps: int = 5, warmup: int = 1) -> dict:
     """64 scenes (grid 32^3-512^3, image 128^2-1024^2, 1-64 spp, 5-60%
-    occupancy, 4 bounces): one batched render (HIP graph replay) against the
-    same 64 renders launched one scene after another, each on its own
-    context stream; wall times with the device synchronised."""
+    occupancy, 4 bounces): one batched render (HIP graph replay, launch
+    shapes chosen by vt_autotune_batch on this batch) against the same 64
+    renders launched one scene after another, each on its own context
+    stream; wall times with the device synchronised."""
     wls = workloads.batch(64)
     ctxs = []
     for wl in wls:
@@ -341,6 +361,11 @@ def batch_race(api, workloads, steps: int = 5, warmup: int = 1) -> dict:
     b.render(seeds, counts, stats=True)
     st = b.stats()
     rays = st["rays"]
+    # the batch's launch shapes (solo threshold, path capacity, merged
+    # traversal tuning) chosen by timing graph replays of this very batch
+    t_tune = time.perf_counter()
+    tuning = b.autotune(seeds, counts)
+    t_tune = time.perf_counter() - t_tune
 
     def batched():
         for c in ctxs:
@@ -375,7 +400,11 @@ def batch_race(api, workloads, steps: int = 5, warmup: int = 1) -> dict:
             "batched_ms": round(t_b * 1000.0, 3), "batched_kernel_ms": round(k_ms, 3),
             "batched_mrays_per_s": round(rays / t_b / 1e6, 2),
             "sequential_ms": round(t_s * 1000.0, 3), "sequential_mrays_per_s": round(rays / t_s / 1e6, 2),
-            "speedup_vs_sequential": round(t_s / t_b, 3)}
+            "speedup_vs_sequential": round(t_s / t_b, 3),
+            "autotuned": {"solo_paths": tuning["solo_paths"], "max_paths": tuning["max_paths"],
+                          "dda_budget": tuning["dda_budget"], "refill_min": tuning["refill_min"],
+                          "refill_chunk": tuning["refill_chunk"], "candidates": tuning["candidates"],
+                          "best_replay_ms": round(tuning["ms"], 3), "seconds": round(t_tune, 2)}}
 
 
 def bench_batch(args, api, workloads):
@@ -401,6 +430,37 @@ def bench_batch(args, api, workloads):
     print(json.dumps(out))
 
 
+def profile_contender(api, workloads, which: str, frames: int, cache_bounce: int = 2, train_paths: int = 65536,
+                      train_extra: int = 1) -> dict:
+    """`frames` frames of one time-to-RMSE contender alone (no reference, no
+    scoring), for a kernel trace of its frames: wall ms per frame."""
+    import numpy as np
+    wl = workloads.full(width=1920, height=1080, spp=1, bounces=8)
+    ctx = api.Context(0)
+    workloads.load(ctx, wl)
+    cache = None
+    if which != "plain":
+        cache = api.RadianceCache(ctx, seed=1)
+        if which == "truncation":
+            cache.set_params(np.zeros(cache.n_params, np.float32))
+    ms = []
+    for n in range(frames):
+        t0 = time.perf_counter()
+        if which == "plain":
+            ctx.render(wl.seed, n, 1)
+        elif which == "cache":
+            cache.render(wl.seed, n, 1, cache_bounce=cache_bounce, train_paths=train_paths, lr=3e-3,
+                         train_extra=train_extra)
+        else:
+            cache.render(wl.seed, n, 1, cache_bounce=cache_bounce, train_paths=0)
+        ctx.synchronize()
+        ms.append((time.perf_counter() - t0) * 1000.0)
+    if cache:
+        cache.close()
+    ctx.close()
+    return {"contender": which, "frames": frames, "ms_per_frame": round(sum(ms[1:]) / max(1, frames - 1), 3)}
+
+
 def rmse_race(api, workloads, frames: int = 256, ref_spp: int = 16384, cache_bounce: int = 2,
               train_paths: int = 65536, train_extra: int = 1, levels=(1e-2, 3e-3)) -> dict:
     """Time to RMSE against an FP64 reference on the 1920x1080 full-shading
@@ -421,15 +481,33 @@ def rmse_race(api, workloads, frames: int = 256, ref_spp: int = 16384, cache_bou
     t_ref = time.perf_counter()
     ref_ctx = api.Context(0)
     workloads.load(ref_ctx, wl, fp64=True)
-    done = 0
+    done, bias_spp, ref_part = 0, min(4096, ref_spp), None
     while done < ref_spp:
-        k = min(512, ref_spp - done)
+        k = min(512, ref_spp - done, bias_spp - done if done < bias_spp else ref_spp)
         ref_ctx.render(wl.seed + 1, done, k)
         done += k
+        if done == bias_spp:
+            ref_part = ref_ctx.read_sum64() / bias_spp
     ref = ref_ctx.read_sum64() / ref_spp
     ref_ctx.close()
     t_ref = time.perf_counter() - t_ref
     ref_finite = np.isfinite(ref).all(axis=-1)
+
+    # what ending the paths at the cache bounce leaves out: the first
+    # `bias_spp` passes of the reference against the same passes (same
+    # random streams, so the first bounces are identical) with every path
+    # ended there into a cache that predicts 0
+    t_bias = time.perf_counter()
+    trunc_ctx = api.Context(0)
+    workloads.load(trunc_ctx, wl, fp64=True)
+    trunc0 = api.RadianceCache(trunc_ctx, seed=1)
+    trunc0.set_params(np.zeros(trunc0.n_params, np.float32))
+    for n0 in range(0, bias_spp, 512):
+        trunc0.render(wl.seed + 1, n0, min(512, bias_spp - n0), cache_bounce=cache_bounce, train_paths=0)
+    trunc_part = trunc_ctx.read_sum64() / bias_spp
+    trunc0.close()
+    trunc_ctx.close()
+    t_bias = time.perf_counter() - t_bias
 
     def sq_err(img):
         """Per-pixel squared error summed over RGB (float64)."""
@@ -515,6 +593,15 @@ def rmse_race(api, workloads, frames: int = 256, ref_spp: int = 16384, cache_bou
         return out
 
     res = {name: summary(r) for name, r in runs.items()}
+    bmask = mask & np.isfinite(ref_part).all(axis=-1) & np.isfinite(trunc_part).all(axis=-1)
+    dtail = (ref_part - trunc_part)[bmask]
+    res["truncation_bias"] = {
+        "passes": bias_spp, "seconds": round(t_bias, 2), "pixels": int(bmask.sum()),
+        "rmse": round(float(np.sqrt(np.mean(dtail * dtail))), 7),
+        "mean": round(float(dtail.mean()), 7),
+        "note": f"tonemapped mean of {bias_spp} plain passes minus the same passes ended at bounce {cache_bounce} "
+                "into a zero cache (identical random streams): what truncation leaves out; compare with "
+                "rmse_at_equal_time"}
     res["plain"]["note"] = f"{frames} frames; its time is every contender's budget"
     for lv in levels:
         key = f"seconds_to_{lv:g}"
@@ -534,7 +621,12 @@ def rmse_race(api, workloads, frames: int = 256, ref_spp: int = 16384, cache_bou
 
 
 def bench_cache(args, api, workloads):
-    """The time-to-RMSE race on its own line (value: the cache's time to 1e-2)."""
+    """The time-to-RMSE race on its own line (value: the cache's time to 1e-2);
+    with --only, frames of one contender alone, for a kernel trace."""
+    if args.only:
+        print(json.dumps(profile_contender(api, workloads, args.only, args.frames, args.cache_bounce,
+                                           args.train_paths, args.train_extra)))
+        return
     res = rmse_race(api, workloads, frames=args.frames, ref_spp=args.ref_spp, cache_bounce=args.cache_bounce,
                     train_paths=args.train_paths, train_extra=args.train_extra)
     out = {

@@ -180,14 +180,16 @@ int vt_get_launch_timing(vt_context* ctx, double* traverse_ms, uint32_t* launche
 /* ---- batches -------------------------------------------------------------
  * Many scenes (contexts on one device, each with its own volume, camera,
  * environment, settings and frame) rendered together: scenes of at least
- * 2^22 paths (width x height x count) get their own wavefront launches on
- * their own context streams, concurrently; the others share one wavefront
- * launch whose queues hold the paths of all of them, optionally recorded
- * once as a HIP graph and replayed.  Each context's accumulation afterwards equals what
+ * 2^22 paths (width x height x count; vt_batch_set_tuning) get their own
+ * wavefront launches on their own context streams, concurrently; the others
+ * share one wavefront launch whose queues hold the paths of all of them.
+ * With a graph, every launch of the batch (the solo scenes' branches and the
+ * merged launch) is recorded once as one HIP graph and replayed.  Each context's accumulation afterwards equals what
  * vt_render(ctx, seeds[i], n0[i], counts[i], rng_frame_max) gives.  The
  * traversal tuning of ctxs[0] applies to the shared launch, each scene's
- * own tuning to its own launches; the path capacity (vt_set_render_mode
- * max_paths) of ctxs[0] applies to both.  No counterpart in the reference
+ * own tuning to its own launches, and the path capacity (vt_set_render_mode
+ * max_paths) of each launch's context, unless vt_batch_set_tuning or
+ * vt_autotune_batch set them for the batch.  No counterpart in the reference
  * (one scene per GL context); the contexts must outlive the batch. */
 typedef struct vt_batch vt_batch;
 int vt_batch_create(vt_context* const* ctxs, int32_t n, vt_batch** out);
@@ -198,6 +200,30 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
 /* Counters of the last batch render (summed over scenes) and its time. */
 int vt_batch_get_stats(vt_batch* b, vt_stats* out);
 void vt_batch_destroy(vt_batch* b);
+/* Tuning of the batch's launches (results are identical for every value):
+ * solo_paths   paths from which a scene gets its own launches (0 = 2^22);
+ * max_paths    paths in flight per launch, merged and solo (0 = each
+ *              launch's context's vt_set_render_mode capacity);
+ * dda_budget, refill_min, refill_chunk  the merged launch's traversal
+ *              tuning as in vt_set_traversal_tuning (0 = the first
+ *              context's).  No counterpart in the reference. */
+typedef struct vt_batch_tuning {
+    uint64_t solo_paths, max_paths;
+    int32_t dda_budget, refill_min, refill_chunk;
+    double ms;              /* vt_autotune_batch: time of the chosen candidate */
+    int32_t candidates;     /* vt_autotune_batch: number timed */
+} vt_batch_tuning;
+int vt_batch_set_tuning(vt_batch* b, uint64_t solo_paths, uint64_t max_paths, int32_t dda_budget, int32_t refill_min,
+                        int32_t refill_chunk);
+/* The tuning in effect, defaults resolved. */
+int vt_batch_get_tuning(vt_batch* b, vt_batch_tuning* out);
+/* Times graph replays of vt_batch_render(b, seeds, n0, counts,
+ * rng_frame_max) over the solo threshold, the path capacity and the merged
+ * launch's traversal tuning (coordinate descent, best of two replays per
+ * candidate), keeps the fastest and reports it.  Every candidate renders
+ * identical images; every scene's accumulation is restored afterwards. */
+int vt_autotune_batch(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, const uint32_t* counts,
+                      uint32_t rng_frame_max, vt_batch_tuning* out);
 
 /* ---- autotuning ------------------------------------------------------------
  * Times `passes`-pass renders of the context's current scene under each

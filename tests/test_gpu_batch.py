@@ -180,6 +180,67 @@ def test_autotune_keeps_image_and_accumulation():
     d.close()
 
 
+def _big_and_small():
+    from voxeltoy_amd import scenes, workloads
+    wls = mixed_scenes()[:3]
+    big = []
+    for i, (n, W, H, spp) in enumerate([(32, 1024, 1024, 4), (40, 1024, 768, 6)]):
+        sc = scenes.small_scene(30 + i, n=n)
+        bmin, bmax = workloads.reference_bounds(sc.shape)
+        cam = scenes.orbit_camera(bmin, bmax, math.radians(120), math.radians(40 + 50 * i), W, H)
+        big.append(workloads.Workload(f"b{i}", sc, cam, W, H, spp=spp, bounces=2, seed=300 + i, bmin=bmin, bmax=bmax,
+                                      env_top=(1.5, 1.4, 1.2), env_bottom=(0.3, 0.3, 0.35)))
+    return [wls[0], big[0], wls[1], big[1], wls[2]]
+
+
+def test_batch_autotune_keeps_images_and_accumulations():
+    """vt_autotune_batch times graph replays over the solo threshold, the
+    path capacity and the merged launch's tuning, restores every scene's
+    accumulation, and the batch rendered with the chosen tuning (solo scenes
+    inside the graph) still equals each scene's single render."""
+    from voxeltoy_amd import api, workloads
+    wls = _big_and_small()
+    seeds = [wl.seed for wl in wls]
+    counts = [wl.spp for wl in wls]
+    ctxs = []
+    for wl in wls:
+        c = api.Context(0)
+        workloads.load(c, wl)
+        ctxs.append(c)
+    b = api.Batch(ctxs)
+    b.render(seeds, [1] * len(wls), graph=True)
+    before = [c.read_accum() for c in ctxs]
+    t = b.autotune(seeds, counts)
+    print("batch tuning", t, flush=True)
+    assert t["candidates"] == 17 and t["ms"] > 0
+    assert t["solo_paths"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24, (1 << 64) - 1)
+    assert t["max_paths"] in (4 << 20, 8 << 20, 16 << 20)
+    assert b.tuning()["solo_paths"] == t["solo_paths"]
+    for i, c in enumerate(ctxs):
+        assert np.array_equal(c.read_accum().view(np.uint32), before[i].view(np.uint32)), f"scene {i} not restored"
+    ref = singles(api, workloads, wls, seeds, [0] * len(wls))
+    for c in ctxs:
+        c.reset_accum()
+    b.render(seeds, counts, graph=True)
+    for i, c in enumerate(ctxs):
+        assert np.array_equal(c.read_accum().view(np.uint32), ref[i].view(np.uint32)), f"scene {i} differs"
+    # every scene solo, then none, through the graph
+    for solo in (1, (1 << 64) - 1):
+        b.set_tuning(solo_paths=solo, max_paths=1 << 20, dda_budget=16, refill_min=16, refill_chunk=128)
+        for c in ctxs:
+            c.reset_accum()
+        b.render(seeds, counts, graph=True)
+        for i, c in enumerate(ctxs):
+            assert np.array_equal(c.read_accum().view(np.uint32), ref[i].view(np.uint32)), f"solo {solo} scene {i}"
+    with pytest.raises(api.VTError):
+        b.set_tuning(max_paths=100)
+    with pytest.raises(api.VTError):
+        b.set_tuning(refill_chunk=10)
+    b.close()
+    for c in ctxs:
+        c.close()
+
+
 def test_context_destroyed_before_its_batch():
     from voxeltoy_amd import api, workloads
     wls = mixed_scenes()[:2]

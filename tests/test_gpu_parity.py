@@ -598,6 +598,35 @@ def test_voxelizer_full_size_bit_exact(ctx, orc):
     assert np.array_equal(occ, ref), f"{np.sum(occ != ref)} voxels differ"
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_voxelizer_large_triangles_bit_exact(ctx, orc, mode):
+    """Triangles of more than 32 (x, y) columns take a wave each: the
+    reference fixture bunny.obj (4968 triangles) at 512^3, where every
+    triangle is large, and a mesh mixing large and small triangles at 128^3,
+    compared with the oracle voxel for voxel."""
+    import os
+    from voxeltoy_amd import renderer as vr, scenes
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bunny.obj")
+    v, t, bmin, bmax = vr.load_obj(path)
+    mat = np.array([0, 0, 0, 0, .5, .5, .5], np.float32)
+    ids = np.r_[-1, np.zeros(255, np.int32) - 1].astype(np.int32)
+    for res, verts, tris in ((512, v, t), (128, None, None)):
+        if tris is None:
+            sv, st = scenes.displaced_icosphere(seed=3, freq=8)           # small triangles
+            big_v = np.array([[0, 0, 0], [1, 0, 0.1], [0, 1, 0.3], [1, 1, 0.9], [0.5, 0.2, 1]], np.float32)
+            big_t = np.array([[0, 1, 2], [1, 3, 2], [0, 4, 3]], np.uint32)
+            sv = (sv - sv.min(0)) / (sv.max(0) - sv.min(0)).max()
+            verts = np.concatenate([sv, big_v]).astype(np.float32)
+            tris = np.concatenate([st, big_t + len(sv)]).astype(np.uint32)
+        M = orc.mesh_transform(verts.min(0), verts.max(0), [res] * 3)
+        print(f"voxelizer large triangles: res {res}, {len(tris)} triangles, mode {mode}", flush=True)
+        ctx.set_volume_u8(np.zeros((res, res, res), np.uint8), ids, mat)
+        occ = ctx.voxelize(verts, tris, M, mode=mode, material_offset=0)
+        ref = orc.voxelize(verts, tris, M, [res] * 3, mode=mode)
+        assert ref.sum() > 10000
+        assert np.array_equal(occ, ref), f"res {res}: {np.sum(occ != ref)} voxels differ"
+
+
 def test_voxelize_workload_render_at_full_size_bit_exact(ctx, orc):
     """The voxelize-then-render workload at its full size: 200k triangles
     voxelized THIN into an empty 512^3 grid on the device, then two passes

@@ -67,6 +67,14 @@ class CacheStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("solo_paths", C.c_uint64), ("max_paths", C.c_uint64), ("dda_budget", C.c_int32),
+                ("refill_min", C.c_int32), ("refill_chunk", C.c_int32), ("ms", C.c_double), ("candidates", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # symbol -> (restype, argtypes)
 SIGNATURES = {
     "vt_last_error": (C.c_char_p, []),
@@ -95,6 +103,10 @@ SIGNATURES = {
                                   C.c_uint32, C.c_int32]),
     "vt_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "vt_batch_destroy": (None, [C.c_void_p]),
+    "vt_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int32, C.c_int32, C.c_int32]),
+    "vt_batch_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(BatchTuning)]),
+    "vt_autotune_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                    C.c_uint32, C.POINTER(BatchTuning)]),
     "vt_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Tuning)]),
     "vt_cache_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "vt_cache_destroy": (None, [C.c_void_p]),
@@ -475,6 +487,26 @@ class Batch:
         st = Stats()
         _check(lib().vt_batch_get_stats(self._b, C.byref(st)))
         return st.as_dict()
+
+    def set_tuning(self, solo_paths: int = 0, max_paths: int = 0, dda_budget: int = 0, refill_min: int = 0,
+                   refill_chunk: int = 0):
+        _check(lib().vt_batch_set_tuning(self._b, solo_paths, max_paths, dda_budget, refill_min, refill_chunk))
+
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        _check(lib().vt_batch_get_tuning(self._b, C.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, seeds, counts, n0=None, rng_frame_max: int = 0xFFFFFFFF) -> dict:
+        """Time graph replays of this render over the batch's tunings and
+        keep the fastest (the scenes' accumulations are restored)."""
+        n = len(self.contexts)
+        s = (C.c_uint64 * n)(*[int(x) for x in seeds])
+        k = (C.c_uint32 * n)(*[int(x) for x in counts])
+        z = None if n0 is None else (C.c_uint32 * n)(*[int(x) for x in n0])
+        t = BatchTuning()
+        _check(lib().vt_autotune_batch(self._b, s, z, k, rng_frame_max, C.byref(t)))
+        return t.as_dict()
 
     def close(self):
         if self._b:

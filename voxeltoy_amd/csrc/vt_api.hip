@@ -38,6 +38,8 @@ int fail(int code, const std::string& msg)
 #define VT_DEFAULT_DDA_BUDGET 32
 #define VT_DEFAULT_REFILL_MIN 8
 #define VT_DEFAULT_REFILL_CHUNK 256
+/* (x, y) columns above which the voxelizer gives a triangle a whole wave */
+#define VT_VOX_BIG_AREA 32
 /* paths from which a scene of a batch gets its own launches */
 #ifndef VT_BATCH_SOLO_PATHS
 #define VT_BATCH_SOLO_PATHS (1ull << 22)
@@ -137,6 +139,7 @@ struct vt_context {
     uint32_t* d_bits = nullptr;
     uint32_t bit_words = 0;
     uint64_t* d_occp = nullptr;     /* padded levels of the wave march */
+    uint64_t* d_occr = nullptr;     /* padded masks with reversed bits, for the packed march */
     uint32_t* d_bitsp = nullptr;
     uint32_t bit_words_p = 0;
     int32_t* d_emissive = nullptr;
@@ -216,7 +219,8 @@ vt::TraceArgs trace_args(const vt_context* c);
  * the emissive light table. */
 int refresh_occupancy(vt_context* c)
 {
-    VT_HIP(vt::launch_block_masks(c->dims, c->d_bricks, c->d_occ64, c->d_bits, c->d_occp, c->d_bitsp, c->stream));
+    VT_HIP(vt::launch_block_masks(c->dims, c->d_bricks, c->d_occ64, c->d_bits, c->d_occp, c->d_bitsp, c->d_occr,
+                                  c->stream));
     dfree(c->d_lights);
     if (c->n_emissive > 0 && c->d_emissive) {
         VT_HIP(hipMalloc(&c->d_lights, (size_t)c->n_emissive * 16u));
@@ -266,6 +270,7 @@ int common_volume_setup(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const
     dfree(c->d_bits);
     dfree(c->d_occp);
     dfree(c->d_bitsp);
+    dfree(c->d_occr);
     dfree(c->d_emissive);
     c->dims = d;
     if (bmin && bmax) {
@@ -286,6 +291,8 @@ int common_volume_setup(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const
     VT_HIP(hipMalloc(&c->d_occp, np * 8u));
     c->bit_words_p = (uint32_t)((np + 31) / 32);
     VT_HIP(hipMalloc(&c->d_bitsp, (size_t)c->bit_words_p * 4u));
+    if (nx <= VT_PACKED_MAX_EXTENT && ny <= VT_PACKED_MAX_EXTENT && nz <= VT_PACKED_MAX_EXTENT)
+        VT_HIP(hipMalloc(&c->d_occr, np * 8u));
     VT_HIP(vt::launch_fill_u8(c->d_bricks, d.n_bricks * 512u, 0, c->stream));
     return VT_OK;
 }
@@ -349,6 +356,7 @@ vt::GridView grid_view(const vt_context* c)
     g.occp = c->d_occp;
     g.bitsp = c->d_bitsp;
     g.bit_words_p = c->bit_words_p;
+    g.occr = c->d_occr;
     g.wide = ((uint64_t)(c->dims.kx + 2) * (uint64_t)(c->dims.ky + 2) * (uint64_t)(c->dims.kz + 2)) > (1ull << 29);
     g.n_emissive = c->n_emissive;
     g.emissive = c->d_emissive;
@@ -490,7 +498,7 @@ void vt_destroy(vt_context* c)
     while (!c->batches.empty()) batch_release(c->batches.back());
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    dfree(c->d_bricks); dfree(c->d_occ64); dfree(c->d_bits); dfree(c->d_occp); dfree(c->d_bitsp); dfree(c->d_emissive); dfree(c->d_lights); dfree(c->d_recs);
+    dfree(c->d_bricks); dfree(c->d_occ64); dfree(c->d_bits); dfree(c->d_occp); dfree(c->d_bitsp); dfree(c->d_occr); dfree(c->d_emissive); dfree(c->d_lights); dfree(c->d_recs);
     dfree(c->d_img); dfree(c->d_quad); dfree(c->d_cdfu); dfree(c->d_cdfv);
     dfree(c->d_accum); dfree(c->d_sum64); dfree(c->d_tile); dfree(c->d_stats);
     free_wavefront(c);
@@ -913,8 +921,13 @@ struct vt_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<uint64_t> graph_key;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t fork = nullptr;            /* start of the solo scenes' branches inside the graph */
     std::vector<hipEvent_t> join;         /* per scene: end of its own launches on its context stream */
     bool timed = false, stats_valid = false;
+    /* tuning (vt_batch_set_tuning, vt_autotune_batch); 0 = the default */
+    uint64_t solo_paths = VT_BATCH_SOLO_PATHS;
+    uint64_t max_paths = 0;               /* 0: each launch's own context's capacity */
+    int32_t dda_budget = 0, refill_min = 0, refill_chunk = 0;   /* merged launch; 0: the lead context's */
 };
 
 int vt_batch_create(vt_context* const* ctxs, int32_t n, vt_batch** out)
@@ -943,7 +956,8 @@ int vt_batch_create(vt_context* const* ctxs, int32_t n, vt_batch** out)
     if (hipMalloc(&b->d_tab, sizeof(vt::TraceArgs) * (size_t)n) != hipSuccess ||
         hipMalloc(&b->d_base, sizeof(uint64_t) * b->h_base.size()) != hipSuccess ||
         hipMalloc(&b->d_stats, 6 * sizeof(unsigned long long)) != hipSuccess ||
-        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
         vt_batch_destroy(b);
         return fail(VT_ERR_HIP, "batch allocation failed");
@@ -964,7 +978,8 @@ static void batch_release(vt_batch* b)
     free_paths(b->wf, b->wf_alloc);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
-    b->ev0 = b->ev1 = nullptr;
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    b->ev0 = b->ev1 = b->fork = nullptr;
     for (hipEvent_t e : b->join)
         if (e) (void)hipEventDestroy(e);
     b->join.clear();
@@ -972,6 +987,141 @@ static void batch_release(vt_batch* b)
     for (vt_context* c : b->ctxs) c->batches.erase(std::remove(c->batches.begin(), c->batches.end(), b), c->batches.end());
     b->ctxs.clear();
     b->lead = nullptr;
+}
+
+int vt_batch_set_tuning(vt_batch* b, uint64_t solo_paths, uint64_t max_paths, int32_t dda_budget, int32_t refill_min,
+                        int32_t refill_chunk)
+{
+    if (!b) return fail(VT_ERR_ARG, "null batch");
+    if (!b->lead) return fail(VT_ERR_STATE, "a context of the batch was destroyed");
+    if (max_paths != 0 && (max_paths < 4096 || max_paths > (1ull << 28)))
+        return fail(VT_ERR_ARG, "max_paths must be 0 or in [4096, 2^28]");
+    if (dda_budget < 0 || dda_budget > 4096) return fail(VT_ERR_ARG, "dda_budget must be in [0, 4096]");
+    if (refill_min < 0 || refill_min > 64) return fail(VT_ERR_ARG, "refill_min must be in [0, 64]");
+    if (refill_chunk != -1 && refill_chunk != 0 && (refill_chunk < 64 || refill_chunk > 65536))
+        return fail(VT_ERR_ARG, "refill_chunk must be -1, 0 or in [64, 65536]");
+    b->solo_paths = solo_paths ? solo_paths : VT_BATCH_SOLO_PATHS;
+    b->max_paths = max_paths;
+    b->dda_budget = dda_budget;
+    b->refill_min = refill_min;
+    b->refill_chunk = refill_chunk;
+    return VT_OK;
+}
+
+int vt_batch_get_tuning(vt_batch* b, vt_batch_tuning* out)
+{
+    if (!b || !out) return fail(VT_ERR_ARG, "null argument");
+    if (!b->lead) return fail(VT_ERR_STATE, "a context of the batch was destroyed");
+    std::memset(out, 0, sizeof(*out));
+    out->solo_paths = b->solo_paths;
+    out->max_paths = b->max_paths ? b->max_paths : b->lead->max_paths;
+    out->dda_budget = b->dda_budget ? b->dda_budget : b->lead->dda_budget;
+    out->refill_min = b->refill_min ? b->refill_min : b->lead->refill_min;
+    out->refill_chunk = b->refill_chunk ? (b->refill_chunk < 0 ? -1 : b->refill_chunk) : b->lead->refill_chunk;
+    return VT_OK;
+}
+
+int vt_autotune_batch(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, const uint32_t* counts,
+                      uint32_t rng_frame_max, vt_batch_tuning* out)
+{
+    if (!b || !seeds || !counts || !out) return fail(VT_ERR_ARG, "null argument");
+    if (!b->lead) return fail(VT_ERR_STATE, "a context of the batch was destroyed");
+    VT_HIP(hipSetDevice(b->lead->device));
+    /* the scenes' accumulations, restored at the end */
+    struct Saved { vt_context* c; void* acc; void* sum; size_t acc_b, sum_b; };
+    std::vector<Saved> saved;
+    int rc = VT_OK;
+    auto release = [&]() {
+        for (Saved& v : saved) { dfree(v.acc); dfree(v.sum); }
+    };
+    for (vt_context* c : b->ctxs) {
+        if (!c->has_volume || !c->d_accum) { release(); return fail(VT_ERR_STATE, "batch scene without a volume or frame"); }
+        Saved v = { c, nullptr, nullptr, (size_t)c->W * c->H * sizeof(float4),
+                    c->d_sum64 ? (size_t)c->W * c->H * 3 * sizeof(double) : 0 };
+        hipError_t he = hipStreamSynchronize(c->stream);
+        if (he == hipSuccess) he = hipMalloc(&v.acc, v.acc_b);
+        if (he == hipSuccess && v.sum_b) he = hipMalloc(&v.sum, v.sum_b);
+        if (he == hipSuccess) he = hipMemcpyAsync(v.acc, c->d_accum, v.acc_b, hipMemcpyDeviceToDevice, c->stream);
+        if (he == hipSuccess && v.sum_b) he = hipMemcpyAsync(v.sum, c->d_sum64, v.sum_b, hipMemcpyDeviceToDevice, c->stream);
+        saved.push_back(v);
+        if (he != hipSuccess) {
+            (void)hipGetLastError();
+            for (vt_context* d : b->ctxs) (void)hipStreamSynchronize(d->stream);
+            release();
+            return fail(VT_ERR_HIP, std::string("batch autotune: saving the accumulations failed: ") + hipGetErrorString(he));
+        }
+    }
+    const uint64_t t_solo = b->solo_paths, t_max = b->max_paths;
+    const int32_t t_bud = b->dda_budget, t_ref = b->refill_min, t_chk = b->refill_chunk;
+    vt_batch_tuning best;
+    std::memset(&best, 0, sizeof(best));
+    best.solo_paths = t_solo; best.max_paths = t_max;
+    best.dda_budget = t_bud; best.refill_min = t_ref; best.refill_chunk = t_chk;
+    best.ms = 1e30;
+    int ncand = 0;
+    /* one graph recording (untimed), then the best of two replays */
+    auto time_it = [&](const vt_batch_tuning& t, double& ms) -> int {
+        b->solo_paths = t.solo_paths; b->max_paths = t.max_paths;
+        b->dda_budget = t.dda_budget; b->refill_min = t.refill_min; b->refill_chunk = t.refill_chunk;
+        int r = vt_batch_render(b, seeds, n0, counts, rng_frame_max, 1);
+        ms = 1e30;
+        for (int rep = 0; rep < 2 && r == VT_OK; ++rep) {
+            r = vt_batch_render(b, seeds, n0, counts, rng_frame_max, 1);
+            float e = 0.0f;
+            if (r == VT_OK && hipEventElapsedTime(&e, b->ev0, b->ev1) == hipSuccess) ms = std::min(ms, (double)e);
+        }
+        ++ncand;
+        return r;
+    };
+    /* coordinate descent: solo threshold, path capacity, then the merged
+     * launch's step budget, refill threshold and queue chunk */
+    const uint64_t solo_c[] = { 1ull << 20, 1ull << 21, 1ull << 22, 1ull << 23, 1ull << 24, ~0ull };
+    const uint64_t max_c[] = { 4ull << 20, 8ull << 20, 16ull << 20 };
+    const int32_t bud_c[] = { 16, 32, 64 }, ref_c[] = { 8, 16 }, chk_c[] = { 128, 256, 512 };
+    for (int stage = 0; stage < 5 && rc == VT_OK; ++stage) {
+        const int nc = stage == 0 ? 6 : stage == 1 ? 3 : stage == 2 ? 3 : stage == 3 ? 2 : 3;
+        vt_batch_tuning stage_best = best;
+        for (int k = 0; k < nc && rc == VT_OK; ++k) {
+            vt_batch_tuning t = best;
+            if (stage == 0) t.solo_paths = solo_c[k];
+            if (stage == 1) t.max_paths = max_c[k];
+            if (stage == 2) t.dda_budget = bud_c[k];
+            if (stage == 3) t.refill_min = ref_c[k];
+            if (stage == 4) t.refill_chunk = chk_c[k];
+            double ms = 0.0;
+            rc = time_it(t, ms);
+            if (rc == VT_OK && ms < stage_best.ms) { stage_best = t; stage_best.ms = ms; }
+        }
+        best = stage_best;
+    }
+    if (rc == VT_OK) {
+        b->solo_paths = best.solo_paths; b->max_paths = best.max_paths;
+        b->dda_budget = best.dda_budget; b->refill_min = best.refill_min; b->refill_chunk = best.refill_chunk;
+    } else {
+        b->solo_paths = t_solo; b->max_paths = t_max;
+        b->dda_budget = t_bud; b->refill_min = t_ref; b->refill_chunk = t_chk;
+    }
+    const std::string err = rc == VT_OK ? std::string() : g_err;
+    /* restore every scene's accumulation */
+    int rr = VT_OK;
+    for (Saved& v : saved) {
+        hipError_t he = hipStreamSynchronize(b->lead->stream);
+        if (he == hipSuccess) he = hipMemcpyAsync(v.c->d_accum, v.acc, v.acc_b, hipMemcpyDeviceToDevice, v.c->stream);
+        if (he == hipSuccess && v.sum_b)
+            he = hipMemcpyAsync(v.c->d_sum64, v.sum, v.sum_b, hipMemcpyDeviceToDevice, v.c->stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(v.c->stream);
+        if (he != hipSuccess && rr == VT_OK) {
+            (void)hipGetLastError();
+            rr = fail(VT_ERR_HIP, std::string("batch autotune: restoring an accumulation failed: ") + hipGetErrorString(he));
+        }
+    }
+    release();
+    if (rc != VT_OK) return fail(rc, err);
+    if (rr != VT_OK) return rr;
+    const int r = vt_batch_get_tuning(b, out);
+    out->ms = best.ms;
+    out->candidates = ncand;
+    return r;
 }
 
 void vt_batch_destroy(vt_batch* b)
@@ -990,11 +1140,12 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
     vt_context* L = b->lead;
     VT_HIP(hipSetDevice(L->device));
     const bool stats = (flags & 2) != 0, graph = (flags & 1) != 0;
-    /* Scenes with at least VT_BATCH_SOLO_PATHS paths fill the GPU by
-     * themselves: they get the single-scene launches (block bits in LDS,
-     * the grid as scalars) on their own context streams, concurrently with
-     * each other and with one batched launch of all other scenes on the
-     * lead stream, where merging saves their many small launches. */
+    /* Scenes with at least b->solo_paths paths fill the GPU by themselves:
+     * they get the single-scene launches (the grid as scalars) on their own
+     * context streams, concurrently with each other and with one batched
+     * launch of all other scenes on the lead stream, where merging saves
+     * their many small launches.  With a graph, all of it is one graph: the
+     * solo scenes' launches branch off the lead stream and join it again. */
     std::vector<std::pair<size_t, vt::TraceArgs>> solo;
     std::vector<size_t> merged;
     for (size_t i = 0; i < n; ++i) {
@@ -1009,7 +1160,7 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
         a.stats = stats ? b->d_stats : nullptr;
         const uint64_t paths = (uint64_t)c->W * (uint64_t)c->H * counts[i];
         if (paths == 0) continue;
-        if (paths >= VT_BATCH_SOLO_PATHS) {
+        if (paths >= b->solo_paths) {
             solo.emplace_back(i, a);
         } else {
             b->h_tab[merged.size()] = a;
@@ -1040,7 +1191,7 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
         VT_HIP(hipMemcpy(b->d_base, b->h_base.data(), sizeof(uint64_t) * 2 * (m + 1), hipMemcpyHostToDevice));
     }
     if (m) {
-        const int r = alloc_paths(b->wf, b->wf_alloc, std::min<uint64_t>(total, L->max_paths));
+        const int r = alloc_paths(b->wf, b->wf_alloc, std::min<uint64_t>(total, b->max_paths ? b->max_paths : L->max_paths));
         if (r != VT_OK) return r;
     }
     if (b->join.size() != n) {
@@ -1052,7 +1203,7 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
     for (const auto& sa : solo) {
         vt_context* c = b->ctxs[sa.first];
         const uint64_t paths = (uint64_t)sa.second.W * (uint64_t)sa.second.H * sa.second.count;
-        const int r = ensure_wavefront(c, std::min<uint64_t>(paths, c->max_paths));
+        const int r = ensure_wavefront(c, std::min<uint64_t>(paths, b->max_paths ? b->max_paths : c->max_paths));
         if (r != VT_OK) return r;
     }
     vt::BatchTable B;
@@ -1061,29 +1212,58 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
     B.pix_base = b->d_base + m + 1;
     B.n = (uint32_t)m;
     B.stats = stats ? b->d_stats : nullptr;
+    B.packed = 1;
+    for (size_t k = 0; k < m; ++k)
+        if (!b->h_tab[k].g.occr) B.packed = 0;
     vt::TraceArgs a0 = trace_args(L);
+    if (b->dda_budget) a0.dda_budget = b->dda_budget;
+    if (b->refill_min) a0.refill_min = b->refill_min;
+    if (b->refill_chunk) a0.refill_chunk = b->refill_chunk < 0 ? 0 : b->refill_chunk;
     if (stats) VT_HIP(vt::launch_reset_counter(L->d_tile, b->d_stats, L->stream));
-    VT_HIP(hipEventRecord(b->ev0, L->stream));
-    for (const auto& sa : solo) {
-        vt_context* c = b->ctxs[sa.first];
-        if (c->stream != L->stream) VT_HIP(hipStreamWaitEvent(c->stream, b->ev0, 0));
-        VT_HIP(vt::launch_render_wavefront(sa.second, c->wf, c->cus, c->stream, stats));
-        VT_HIP(hipEventRecord(b->join[sa.first], c->stream));
-    }
-    if (m && graph) {
+    /* every launch of the batch: the solo scenes on their own streams,
+     * forked from and joined back to the lead stream, and the merged launch */
+    auto issue = [&](hipEvent_t fork) -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (!solo.empty()) e = hipEventRecord(fork, L->stream);
+        for (const auto& sa : solo) {
+            vt_context* c = b->ctxs[sa.first];
+            if (e == hipSuccess && c->stream != L->stream) e = hipStreamWaitEvent(c->stream, fork, 0);
+            if (e == hipSuccess) e = vt::launch_render_wavefront(sa.second, c->wf, c->cus, c->stream, stats);
+            if (e == hipSuccess) e = hipEventRecord(b->join[sa.first], c->stream);
+        }
+        if (e == hipSuccess && m)
+            e = vt::launch_render_batch(a0, B, b->wf, total, pixels, max_bounces, L->cus, L->stream, stats);
+        for (const auto& sa : solo)
+            if (e == hipSuccess) e = hipStreamWaitEvent(L->stream, b->join[sa.first], 0);
+        return e;
+    };
+    if (graph) {
+        /* the graph holds every launch argument: re-recorded when any of
+         * them changes (shape, tuning, path buffers, solo scenes' state) */
         std::vector<uint64_t> key = { (uint64_t)m, total, pixels, b->wf.cap, (uint64_t)max_bounces,
                                       (uint64_t)stats, (uint64_t)a0.dda_budget, (uint64_t)a0.refill_min,
-                                      (uint64_t)a0.refill_chunk, (uint64_t)(uintptr_t)b->wf.s[0].o };
+                                      (uint64_t)a0.refill_chunk, (uint64_t)(uintptr_t)b->wf.s[0].o,
+                                      (uint64_t)B.packed };
         key.insert(key.end(), merged.begin(), merged.end());
+        for (const auto& sa : solo) {
+            const vt_context* c = b->ctxs[sa.first];
+            key.push_back(~(uint64_t)sa.first);
+            key.push_back((uint64_t)(uintptr_t)c->wf.s[0].o);
+            key.push_back(c->wf.cap);
+            uint64_t w[(sizeof(vt::TraceArgs) + 7) / 8] = {};
+            std::memcpy(w, &sa.second, sizeof(vt::TraceArgs));
+            key.insert(key.end(), w, w + (sizeof(vt::TraceArgs) + 7) / 8);
+        }
         if (!b->exec || key != b->graph_key) {
             if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+            b->graph_key.clear();
             hipGraph_t g = nullptr;
             VT_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
-            const hipError_t le = vt::launch_render_batch(a0, B, b->wf, total, pixels, max_bounces, L->cus, L->stream,
-                                                          stats);
+            const hipError_t le = issue(b->fork);
             const hipError_t ce = hipStreamEndCapture(L->stream, &g);
             if (le != hipSuccess || ce != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
                 return fail(VT_ERR_HIP, std::string("batch graph capture failed: ") +
                                             hipGetErrorString(le != hipSuccess ? le : ce));
             }
@@ -1095,11 +1275,12 @@ int vt_batch_render(vt_batch* b, const uint64_t* seeds, const uint32_t* n0, cons
             }
             b->graph_key = key;
         }
+        VT_HIP(hipEventRecord(b->ev0, L->stream));
         VT_HIP(hipGraphLaunch(b->exec, L->stream));
-    } else if (m) {
-        VT_HIP(vt::launch_render_batch(a0, B, b->wf, total, pixels, max_bounces, L->cus, L->stream, stats));
+    } else {
+        VT_HIP(hipEventRecord(b->ev0, L->stream));
+        VT_HIP(issue(b->fork));
     }
-    for (const auto& sa : solo) VT_HIP(hipStreamWaitEvent(L->stream, b->join[sa.first], 0));
     VT_HIP(hipEventRecord(b->ev1, L->stream));
     /* each context's later reads and renders run on its own stream */
     VT_HIP(hipStreamSynchronize(L->stream));
@@ -1508,10 +1689,19 @@ int vt_render_cached(vt_context* c, vt_cache* k, uint64_t seed, uint32_t n0, uin
         VT_HIP(vt::launch_fill_u32(k->d_counts, 4, 0u, s));
         VT_HIP(vt::launch_render_cached(a, c->wf, C, k->d_pack, k->d_bias, c->cus, s));
         VT_HIP(vt::launch_cache_tally(k->d_counts, (uint32_t)k->scap, k->d_tally, s));
-        /* training batches over the samples the pass stored; the count stays
-         * on the device (empty batches exit at once) */
-        if (nb) VT_HIP(vt::launch_fill_u32((uint32_t*)k->d_loss, nb, 0u, s));
-        for (uint32_t bi = 0; bi < nb; ++bi) {
+        /* training batches over the samples the pass stored: as many as hold
+         * samples (reading the count back costs one round trip, launching
+         * every possible batch a few microseconds per empty one) */
+        uint32_t nb_used = 0;
+        if (nb) {
+            VT_HIP(vt::launch_fill_u32((uint32_t*)k->d_loss, nb, 0u, s));
+            uint32_t stored = 0;
+            VT_HIP(hipMemcpyAsync(&stored, k->d_counts + 1, 4, hipMemcpyDeviceToHost, s));
+            VT_HIP(hipStreamSynchronize(s));
+            const uint64_t used = std::min<uint64_t>(stored, k->scap);
+            nb_used = (uint32_t)((used + batch - 1) / batch);
+        }
+        for (uint32_t bi = 0; bi < nb_used; ++bi) {
             const vt::TrainBatch tb = { k->d_sraw, k->d_stgt, k->d_counts + 1, (uint32_t)k->scap, bi * batch, batch };
             r = cache_train_batch(k, tb, nullptr, lr, (int)bi);
             if (r != VT_OK) return r;
@@ -1773,8 +1963,18 @@ int vt_voxelize(vt_context* c, const float* verts, int64_t nv, const uint32_t* i
         VT_HIP(vt::launch_fill_u8(d_occ, (uint64_t)nvox, 0, c->stream));
     }
     a.verts = d_v; a.idx = d_i; a.occ = d_occ;
+    /* the list of large triangles (a wave each) and its counter */
+    uint32_t* d_big = nullptr;
+    if (ntri > 0) {
+        VT_HIP(hipMalloc(&d_big, ((size_t)ntri + 1) * 4));
+        VT_HIP(vt::launch_fill_u32(d_big + ntri, 1, 0u, c->stream));
+        a.big = d_big;
+        a.big_count = d_big + ntri;
+        a.big_cap = (uint32_t)std::min<int64_t>(ntri, 0x7fffffff);   /* beyond it, triangles keep their thread */
+        a.big_area = VT_VOX_BIG_AREA;
+    }
     VT_HIP(hipEventRecord(c->ev0, c->stream));
-    VT_HIP(vt::launch_voxelize(a, c->stream));
+    VT_HIP(vt::launch_voxelize(a, c->cus, c->stream));
     VT_HIP(hipEventRecord(c->ev1, c->stream));
     c->timed = true;           /* vt_get_stats: kernel_ms of the voxelizer launch */
     c->stats_valid = false;
@@ -1784,7 +1984,7 @@ int vt_voxelize(vt_context* c, const float* verts, int64_t nv, const uint32_t* i
     }
     if (occupancy) VT_HIP(hipMemcpyAsync(occupancy, d_occ, (size_t)nvox, hipMemcpyDeviceToHost, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
-    dfree(d_v); dfree(d_i); dfree(d_occ);
+    dfree(d_v); dfree(d_i); dfree(d_occ); dfree(d_big);
     return VT_OK;
 }
 

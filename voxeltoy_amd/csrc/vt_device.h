@@ -83,6 +83,10 @@ struct GridView {
     const uint64_t* occp;
     const uint32_t* bitsp;
     uint32_t bit_words_p;
+    /* occp with each mask's bits reversed (voxel l of the block at bit
+     * 63 - l), for the packed-coordinate march; null when an extent of the
+     * grid exceeds VT_PACKED_MAX_EXTENT */
+    const uint64_t* occr;
     int32_t wide;               /* padded masks span 4 GiB or more (byte offsets need 64 bits) */
     int32_t n_emissive;
     const int32_t* emissive;    /* linear voxel indices */
@@ -354,6 +358,16 @@ struct DdaState {
     __device__ __forceinline__ int advance_wave(const GridView& g, const uint32_t* __restrict__ bits, int budget,
                                                 uint32_t& visits);
 
+    /* advance_wave (same steps, same order of the tests, same result) with
+     * the voxel coordinates packed into one register for grids whose
+     * extents are at most VT_PACKED_MAX_EXTENT (g.occr set).  UNI: every
+     * lane marches the same grid (else g is the lane's own, as in batch
+     * launches); LDSB: `bits` (the padded block bits) is in LDS and is
+     * tested before a block's mask is loaded. */
+    template <bool COUNT, bool UNI, bool LDSB>
+    __device__ __forceinline__ int advance_wave_packed(const GridView& g, const uint32_t* __restrict__ bits,
+                                                       int budget, uint32_t& visits);
+
     /* traverse() result of a finished march: kind and hit record. */
     __device__ __forceinline__ int finish(const GridView& g, int status, Hit& h) const
     {
@@ -527,6 +541,160 @@ __device__ __forceinline__ int DdaState::advance_wave(const GridView& g, const u
     const bool inside = (uint32_t)x < (uint32_t)g.nx && (uint32_t)y < (uint32_t)g.ny && (uint32_t)z < (uint32_t)g.nz;
     /* a visit per iteration spent inside the grid under the limit: every
      * step, plus the voxel a hit stopped in */
+    if (COUNT) visits += (uint32_t)(steps - steps0) + (!marching && inside ? 1u : 0u);
+    if (marching) return steps >= g.max_steps ? 2 : 0;
+    return inside ? 1 : 2;
+}
+
+/* ---- packed wave march ---------------------------------------------------------------
+ * The loop of advance_wave over P = (x+4) | (y+4) << 10 | (z+4) << 20, the
+ * padded coordinates of the voxel in 10-bit fields.  An axis step adds that
+ * axis's sign in its field (+-1, +-1 << 10, +-1 << 20); no field leaves
+ * [3, extent + 4] because the march stops at the first voxel outside the
+ * grid.  Per iteration, for the marching lanes:
+ *   key  = P & 0x3fcff3fc (each field without its two low bits): the
+ *          voxel's padded block (bx, by, bz) at bits 2, 12 and 22, so a
+ *          block change is one AND and one compare; lanes whose key changed
+ *          form the block's linear index bx + bxp (by + byp bz) from it and
+ *          reload: (LDS: the block's bit first, then) the block's mask from
+ *          g.occr; ne = lanes whose block is non-empty
+ *   hit  = bit (x&3) + 4 (y&3) + 16 (z&3) of the mask: the byte sum of
+ *          P & 0x00300c03 (x&3 in byte 0, (y&3) << 2 in byte 1, (z&3) << 4
+ *          in byte 2) is that index, and the masks are stored bit-reversed,
+ *          so shifting left by it puts the voxel's bit in the sign
+ *   step, step count and exit as in advance_wave. */
+#define VT_PACKED_MAX_EXTENT 1016
+#define VT_WAVE_DDA_PACKED_HEAD                                                           \
+    "s_mov_b64 %[live], exec\n\t"                                                         \
+    "v_cmp_gt_i32_e64 %[ne], 0, %[key]\n\t"                                               \
+    "v_bfe_u32 %[key], %[key], 0, 31\n\t"                                                 \
+    "s_mov_b32 %[it], 0\n"                                                                \
+    ".Lvp_top%=:\n\t"                                                                     \
+    "s_cmp_ge_u32 %[it], %[n]\n\t"                                                        \
+    "s_cbranch_scc1 .Lvp_end%=\n\t"                                                       \
+    "v_and_b32 %[t0], 0x3fcff3fc, %[p]\n\t"                                               \
+    "v_cmp_ne_u32_e64 %[s0], %[t0], %[key]\n\t"                                           \
+    "s_and_saveexec_b64 %[s1], %[s0]\n\t"                                                 \
+    "s_cbranch_scc0 .Lvp_norl%=\n\t"                                                      \
+    "v_mov_b32 %[key], %[t0]\n\t"                                                         \
+    "v_lshrrev_b32 %[t1], 22, %[t0]\n\t"                                                  \
+    "v_bfe_u32 %[t2], %[t0], 12, 8\n\t"                                                   \
+    "v_mad_u32_u24 %[t1], %[t1], %[byp], %[t2]\n\t"                                       \
+    "v_bfe_u32 %[t2], %[t0], 2, 8\n\t"                                                    \
+    "v_mad_u32_u24 %[t1], %[t1], %[bxp], %[t2]\n\t"
+/* between HEAD and TAIL: t1 = linear index of the block; mask = its mask
+ * after its block bit in LDS (LDS), or directly from the lane's own array
+ * (LANE, batch launches); ne updated for the lanes that reloaded */
+#define VT_WAVE_DDA_PACKED_LANE                                                           \
+    "v_mad_u64_u32 %[t64], vcc, %[t1], 8, %[occ]\n\t"                                     \
+    "global_load_dwordx2 %[mask], %[t64], off\n\t"                                        \
+    "s_waitcnt vmcnt(0)\n\t"                                                              \
+    "v_cmp_ne_u64_e64 %[s0], 0, %[mask]\n\t"                                              \
+    "s_andn2_b64 %[ne], %[ne], exec\n\t"                                                  \
+    "s_or_b64 %[ne], %[ne], %[s0]\n"
+#define VT_WAVE_DDA_PACKED_LDS                                                            \
+    "v_lshrrev_b32 %[t2], 5, %[t1]\n\t"                                                   \
+    "v_lshl_add_u32 %[t2], %[t2], 2, %[bits]\n\t"                                         \
+    "ds_read_b32 %[t2], %[t2]\n\t"                                                        \
+    "s_waitcnt lgkmcnt(0)\n\t"                                                            \
+    "v_bfe_u32 %[t2], %[t2], %[t1], 1\n\t"                                                \
+    "v_cmp_ne_u32_e64 %[s0], 0, %[t2]\n\t"                                                \
+    "s_andn2_b64 %[ne], %[ne], exec\n\t"                                                  \
+    "s_or_b64 %[ne], %[ne], %[s0]\n\t"                                                    \
+    "s_and_b64 exec, exec, %[s0]\n\t"                                                     \
+    "s_cbranch_execz .Lvp_norl%=\n\t"                                                     \
+    "v_lshlrev_b32 %[t1], 3, %[t1]\n\t"                                                   \
+    "global_load_dwordx2 %[mask], %[t1], %[occ]\n\t"                                      \
+    "s_waitcnt vmcnt(0)\n"
+#define VT_WAVE_DDA_PACKED_TAIL                                                           \
+    ".Lvp_norl%=:\n\t"                                                                    \
+    "s_mov_b64 exec, %[s1]\n\t"                                                           \
+    "s_and_b64 %[s0], %[ne], exec\n\t"                                                    \
+    "s_cbranch_scc0 .Lvp_go%=\n\t"                                                        \
+    "s_and_saveexec_b64 %[s1], %[s0]\n\t"                                                 \
+    "v_and_b32 %[t0], 0x300c03, %[p]\n\t"                                                 \
+    "v_sad_u8 %[t0], %[t0], 0, 0\n\t"                                                     \
+    "v_lshlrev_b64 %[t64], %[t0], %[mask]\n\t"                                            \
+    "v_cmp_gt_i64_e64 %[s0], 0, %[t64]\n\t"                                               \
+    "s_mov_b64 exec, %[s1]\n\t"                                                           \
+    "s_cmp_eq_u64 %[s0], 0\n\t"                                                           \
+    "s_cbranch_scc1 .Lvp_go%=\n\t"                                                        \
+    "s_andn2_b64 %[s1], exec, %[s0]\n\t"                                                  \
+    "s_mov_b64 exec, %[s0]\n\t"                                                           \
+    "v_add_u32 %[steps], %[it], %[steps]\n\t"                                             \
+    "s_mov_b64 exec, %[s1]\n\t"                                                           \
+    "s_cbranch_execz .Lvp_out%=\n"                                                        \
+    ".Lvp_go%=:\n\t"                                                                      \
+    "v_min3_f32 %[t0], %[dx], %[dy], %[dz]\n\t"                                           \
+    "s_mov_b64 %[s1], exec\n\t"                                                           \
+    "v_cmp_eq_f32_e64 %[s0], %[dx], %[t0]\n\t"                                            \
+    "v_cmp_eq_f32_e64 %[s2], %[dy], %[t0]\n\t"                                            \
+    "v_cmp_eq_f32_e64 vcc, %[dz], %[t0]\n\t"                                              \
+    "s_mov_b64 exec, %[s0]\n\t"                                                           \
+    "v_add_f32 %[dx], %[dx], %[ax]\n\t"                                                   \
+    "v_add_u32 %[p], %[p], %[ix]\n\t"                                                     \
+    "s_mov_b64 exec, %[s2]\n\t"                                                           \
+    "v_add_f32 %[dy], %[dy], %[ay]\n\t"                                                   \
+    "v_add_u32 %[p], %[p], %[iy]\n\t"                                                     \
+    "s_mov_b64 exec, vcc\n\t"                                                             \
+    "v_add_f32 %[dz], %[dz], %[az]\n\t"                                                   \
+    "v_add_u32 %[p], %[p], %[iz]\n\t"                                                     \
+    "s_mov_b64 exec, %[s1]\n\t"                                                           \
+    "s_add_u32 %[it], %[it], 1\n\t"                                                       \
+    "s_branch .Lvp_top%=\n"                                                               \
+    ".Lvp_end%=:\n\t"                                                                     \
+    "v_add_u32 %[steps], %[it], %[steps]\n"                                               \
+    ".Lvp_out%=:\n\t"                                                                     \
+    "s_mov_b64 %[s0], exec\n\t"                                                           \
+    "s_and_b64 exec, %[live], %[ne]\n\t"                                                  \
+    "v_or_b32 %[key], 0x80000000, %[key]\n\t"                                             \
+    "s_mov_b64 exec, %[live]\n\t"                                                         \
+    "v_cndmask_b32_e64 %[t1], 0, 1, %[s0]\n\t"
+
+template <bool COUNT, bool UNI, bool LDSB>
+__device__ __forceinline__ int DdaState::advance_wave_packed(const GridView& g, const uint32_t* __restrict__ bits,
+                                                             int budget, uint32_t& visits)
+{
+    static_assert(UNI == LDSB, "one grid: block bits in LDS; per-lane grids: masks read directly");
+    const int room = g.max_steps - steps;
+    int n = budget;
+    for (uint64_t near = __builtin_amdgcn_ballot_w64(room < budget); near; near &= near - 1ull) {
+        const int r = __builtin_amdgcn_readlane(room, (int)__builtin_ctzll(near));
+        n = r < n ? r : n;
+    }
+    n = n > 0 ? n : 0;
+    const int32_t steps0 = steps;
+    uint32_t p = (uint32_t)(x + 4) | ((uint32_t)(y + 4) << 10) | ((uint32_t)(z + 4) << 20);
+    const int32_t ix = stx, iy = (int32_t)((uint32_t)sty << 10), iz = (int32_t)((uint32_t)stz << 20);
+    uint32_t t0, t1, t2;
+    uint64_t t64, s_live, s_ne, s_0, s_1, s_2;
+    uint32_t it;
+#define VT_WAVE_DDA_PACKED_OUTS                                                           \
+    [p] "+v"(p), [dx] "+v"(disx), [dy] "+v"(disy), [dz] "+v"(disz), [steps] "+v"(steps), [key] "+v"(key), \
+        [mask] "+v"(mask), [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t64] "=&v"(t64),              \
+        [live] "=&s"(s_live), [ne] "=&s"(s_ne), [s0] "=&s"(s_0), [s1] "=&s"(s_1), [s2] "=&s"(s_2),          \
+        [it] "=&s"(it)
+#define VT_WAVE_DDA_PACKED_INS                                                            \
+    [ax] "v"(ax), [ay] "v"(ay), [az] "v"(az), [ix] "v"(ix), [iy] "v"(iy), [iz] "v"(iz), [n] "s"(n)
+    if constexpr (LDSB) {
+        const uint32_t lds = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const uint32_t*)bits;
+        asm volatile(VT_WAVE_DDA_PACKED_HEAD VT_WAVE_DDA_PACKED_LDS VT_WAVE_DDA_PACKED_TAIL
+                     : VT_WAVE_DDA_PACKED_OUTS
+                     : VT_WAVE_DDA_PACKED_INS, [occ] "s"(g.occr), [bxp] "s"(g.bxp), [byp] "s"(g.byp), [bits] "s"(lds)
+                     : "vcc", "scc", "memory");
+    } else {
+        asm volatile(VT_WAVE_DDA_PACKED_HEAD VT_WAVE_DDA_PACKED_LANE VT_WAVE_DDA_PACKED_TAIL
+                     : VT_WAVE_DDA_PACKED_OUTS
+                     : VT_WAVE_DDA_PACKED_INS, [occ] "v"(g.occr), [bxp] "v"(g.bxp), [byp] "v"(g.byp)
+                     : "vcc", "scc", "memory");
+    }
+#undef VT_WAVE_DDA_PACKED_OUTS
+#undef VT_WAVE_DDA_PACKED_INS
+    x = (int32_t)(p & 1023u) - 4;
+    y = (int32_t)((p >> 10) & 1023u) - 4;
+    z = (int32_t)(p >> 20) - 4;
+    const bool marching = t1 != 0u;
+    const bool inside = (uint32_t)x < (uint32_t)g.nx && (uint32_t)y < (uint32_t)g.ny && (uint32_t)z < (uint32_t)g.nz;
     if (COUNT) visits += (uint32_t)(steps - steps0) + (!marching && inside ? 1u : 0u);
     if (marching) return steps >= g.max_steps ? 2 : 0;
     return inside ? 1 : 2;

@@ -106,6 +106,7 @@ struct BatchTable {
     const uint64_t* pix_base;
     uint32_t n;
     unsigned long long* stats;   /* 6 counters or null */
+    int32_t packed;              /* every scene has packed-march masks (GridView::occr) */
 };
 
 /* Radiance-cache rendering of one pass: paths that are not training paths
@@ -189,7 +190,7 @@ hipError_t launch_brick_from_ids(const BrickDims& d, const uint8_t* dense, const
  * ((kx+2)(ky+2)(kz+2) words in occp), where voxels outside the grid are
  * set. */
 hipError_t launch_block_masks(const BrickDims& d, const uint8_t* bricks, uint64_t* occ64, uint32_t* bits,
-                              uint64_t* occp, uint32_t* bitsp, hipStream_t s);
+                              uint64_t* occp, uint32_t* bitsp, uint64_t* occr, hipStream_t s);
 /* Bricked slots -> dense int32 offsets via slot_off[256]. */
 hipError_t launch_dense_from_bricks(const BrickDims& d, const uint8_t* bricks, const int32_t* slot_off, int32_t* dense,
                                     hipStream_t s);
@@ -213,7 +214,14 @@ struct VoxArgs {
     uint8_t slot;
     uint8_t* bricks;             /* may be null */
     uint8_t* occ;                /* dense occupancy, may be null */
+    /* triangles with more than big_area (x, y) columns go to a list of
+     * big_cap entries (counter big_count, zero at launch) that one wave per
+     * triangle then voxelizes; big = null keeps every triangle on one thread */
+    uint32_t* big;
+    uint32_t* big_count;
+    uint32_t big_cap;
+    int32_t big_area;
 };
-hipError_t launch_voxelize(const VoxArgs& a, hipStream_t s);
+hipError_t launch_voxelize(const VoxArgs& a, int cus, hipStream_t s);
 
 }  // namespace vt

@@ -699,9 +699,15 @@ __global__ __launch_bounds__(WF_TRAVERSE_THREADS) void wf_traverse_kernel(TraceA
         if (has) {
             /* per-lane grids (batch) and grids whose padded masks span 4 GiB
              * address the masks with 64-bit per-lane arithmetic */
-            const int status = (BATCH || G.wide)
-                                   ? D.advance_wave<STATS, false, false>(G, G.bitsp, budget, cnt.visits)
-                                   : D.advance_wave<STATS, LDSB, true>(G, bits, budget, cnt.visits);
+            int status;
+            if (BATCH && B.packed)
+                status = D.advance_wave_packed<STATS, false, false>(G, nullptr, budget, cnt.visits);
+            else if (BATCH || G.wide)
+                status = D.advance_wave<STATS, false, false>(G, G.bitsp, budget, cnt.visits);
+            else if (LDSB && G.occr)
+                status = D.advance_wave_packed<STATS, true, true>(G, bits, budget, cnt.visits);
+            else
+                status = D.advance_wave<STATS, LDSB, true>(G, bits, budget, cnt.visits);
             if (status) {
                 /* kind and voxel only: the shade kernel reads the material slot */
                 const int kind = status == 1 ? KIND_VOXEL : (D.y < 0 ? KIND_GROUND : KIND_MISS);
@@ -972,15 +978,16 @@ static hipError_t wf_run(const TraceArgs& a, const BatchTable& B, const WfPaths&
 hipError_t launch_render_wavefront(const TraceArgs& a, const WfPaths& P, int cus, hipStream_t s, bool stats,
                                    const LaunchTimer* timer, hipEvent_t acc_wait)
 {
-    const BatchTable B = { nullptr, nullptr, nullptr, 0u, nullptr };
+    const BatchTable B = { nullptr, nullptr, nullptr, 0u, nullptr, 0 };
     const uint64_t WH = (uint64_t)a.W * (uint64_t)a.H;
     const uint64_t total = WH * a.count;
     const int rounds = 1 + (a.s.max_bounces > 0 ? a.s.max_bounces : 0);
+    const bool lds_bits = a.lds_bits_p;
     if (stats)
-        return a.lds_bits_p ? wf_run<true, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
-                            : wf_run<true, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
-    return a.lds_bits_p ? wf_run<false, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
-                        : wf_run<false, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
+        return lds_bits ? wf_run<true, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
+                        : wf_run<true, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
+    return lds_bits ? wf_run<false, true, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait)
+                    : wf_run<false, false, false>(a, B, P, total, WH, rounds, cus, s, timer, acc_wait);
 }
 
 hipError_t launch_render_batch(const TraceArgs& a0, const BatchTable& B, const WfPaths& P, uint64_t total_paths,
@@ -1019,7 +1026,7 @@ template <bool LDSB>
 static hipError_t wf_run_cached(const TraceArgs& a, const WfPaths& P, const CacheRender& C, const void* pack,
                                 const float* bias, int cus, hipStream_t s)
 {
-    const BatchTable B = { nullptr, nullptr, nullptr, 0u, nullptr };
+    const BatchTable B = { nullptr, nullptr, nullptr, 0u, nullptr, 0 };
     const size_t lds_t = LDSB ? (size_t)a.g.bit_words_p * 4u : 0u;
     const size_t lds_s = 256 * sizeof(MatRec) + (size_t)a.cdfv_lds_bytes;
     const int bt = blocks_per_cu(wf_traverse_kernel<false, LDSB, false>, WF_TRAVERSE_THREADS, lds_t);

@@ -125,6 +125,15 @@ __global__ __launch_bounds__(256) void block_masks_padded_kernel(BrickDims d, co
     }
 }
 
+/* occr: occp with each mask's bits reversed (voxel l of the block at bit
+ * 63 - l): the packed march tests a voxel by shifting its bit into the sign. */
+__global__ __launch_bounds__(256) void block_masks_reversed_kernel(const uint64_t* __restrict__ occp,
+                                                                   uint64_t* __restrict__ occr, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        occr[i] = __builtin_bitreverse64(occp[i]);
+}
+
 __global__ __launch_bounds__(256) void dense_from_bricks_kernel(BrickDims d, const uint8_t* __restrict__ bricks,
                                                                 const int32_t* __restrict__ slot_off,
                                                                 int32_t* __restrict__ dense)
@@ -187,13 +196,14 @@ hipError_t launch_brick_from_ids(const BrickDims& d, const uint8_t* dense, const
 }
 
 hipError_t launch_block_masks(const BrickDims& d, const uint8_t* bricks, uint64_t* occ64, uint32_t* bits,
-                              uint64_t* occp, uint32_t* bitsp, hipStream_t s)
+                              uint64_t* occp, uint32_t* bitsp, uint64_t* occr, hipStream_t s)
 {
     block_masks_kernel<<<grid_for(d.n_blocks), 256, 0, s>>>(d, bricks, occ64);
     block_bits_kernel<<<grid_for((d.n_blocks + 31) / 32), 256, 0, s>>>(d.n_blocks, occ64, bits);
     const uint64_t np = (uint64_t)(d.kx + 2) * (uint64_t)(d.ky + 2) * (uint64_t)(d.kz + 2);
     block_masks_padded_kernel<<<grid_for(np), 256, 0, s>>>(d, occ64, occp);
     block_bits_kernel<<<grid_for((np + 31) / 32), 256, 0, s>>>(np, occp, bitsp);
+    if (occr) block_masks_reversed_kernel<<<grid_for(np), 256, 0, s>>>(occp, occr, np);
     return hipGetLastError();
 }
 
@@ -242,12 +252,23 @@ __device__ __forceinline__ bool finite3(f3 v)
     return vt_absf(v.x) < vt_inff() && vt_absf(v.y) < vt_inff() && vt_absf(v.z) < vt_inff();
 }
 
-__global__ __launch_bounds__(256) void voxelize_kernel(VoxArgs a)
+/* Per-triangle state of voxelize.gs after the set-up (:55-208): the
+ * swizzled vertices' edge functions in the three planes, the z-range plane
+ * and the (x, y) bounds of the column loop. */
+struct VoxTri {
+    int32_t lut;                 /* swizzle: 0 (dominant x), 1 (dominant y), 2 (dominant z) */
+    int32_t mn[3], mx[3];
+    V2 nxy[3], nyz[3], nzx[3];
+    float dxy[3], dyz[3], dzx[3];
+    f3 np;
+    float dMin, dMax, nzInv;
+};
+
+/* voxelize.vs:20-28 and the set-up of voxelize.gs; false for a triangle
+ * with a non-finite vertex (skipped). */
+__device__ __forceinline__ bool vox_setup(const VoxArgs& a, int64_t tri, VoxTri& T)
 {
-    const int64_t tri = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tri >= a.ntri) return;
     const float rf[3] = { (float)a.d.nx, (float)a.d.ny, (float)a.d.nz };
-    const int32_t res[3] = { a.d.nx, a.d.ny, a.d.nz };
     f3 v[3];
     bool finite = true;
 #pragma unroll
@@ -258,95 +279,140 @@ __global__ __launch_bounds__(256) void voxelize_kernel(VoxArgs a)
         v[k] = mk(ls[0] * rf[0], ls[1] * rf[1], ls[2] * rf[2]);
         finite = finite && finite3(v[k]);
     }
-    if (!finite) return;
+    if (!finite) return false;
     f3 n = cross(v[1] - v[0], v[2] - v[1]);
     const float anx = vt_absf(n.x), any = vt_absf(n.y), anz = vt_absf(n.z);
-    int lut;
     if (anx >= any && anx >= anz) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) v[k] = mk(v[k].y, v[k].z, v[k].x);
         n = mk(n.y, n.z, n.x);
-        lut = 0;
+        T.lut = 0;
     } else if (any >= anx && any >= anz) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) v[k] = mk(v[k].z, v[k].x, v[k].y);
         n = mk(n.z, n.x, n.y);
-        lut = 1;
+        T.lut = 1;
     } else {
-        lut = 2;
+        T.lut = 2;
     }
     const float amn[3] = { vt_minf(vt_minf(v[0].x, v[1].x), v[2].x), vt_minf(vt_minf(v[0].y, v[1].y), v[2].y),
                            vt_minf(vt_minf(v[0].z, v[1].z), v[2].z) };
     const float amx[3] = { vt_maxf(vt_maxf(v[0].x, v[1].x), v[2].x), vt_maxf(vt_maxf(v[0].y, v[1].y), v[2].y),
                            vt_maxf(vt_maxf(v[0].z, v[1].z), v[2].z) };
-    int32_t mn[3], mx[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        mn[k] = (int32_t)vt_minf(vt_maxf(floorf(amn[k]), 0.0f), rf[k]);
-        mx[k] = (int32_t)vt_minf(vt_maxf(ceilf(amx[k]), 0.0f), rf[k]);
+        T.mn[k] = (int32_t)vt_minf(vt_maxf(floorf(amn[k]), 0.0f), rf[k]);
+        T.mx[k] = (int32_t)vt_minf(vt_maxf(ceilf(amx[k]), 0.0f), rf[k]);
     }
     const f3 ee[3] = { v[1] - v[0], v[2] - v[1], v[0] - v[2] };
-    V2 nxy[3], nyz[3], nzx[3];
-    float dxy[3], dyz[3], dzx[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const f3 ek = ee[k];
-        nxy[k] = (n.z >= 0.0f) ? V2{ -ek.y, ek.x } : V2{ ek.y, -ek.x };
-        nyz[k] = (n.x >= 0.0f) ? V2{ -ek.z, ek.y } : V2{ ek.z, -ek.y };
-        nzx[k] = (n.y >= 0.0f) ? V2{ -ek.x, ek.z } : V2{ ek.x, -ek.z };
+        T.nxy[k] = (n.z >= 0.0f) ? V2{ -ek.y, ek.x } : V2{ ek.y, -ek.x };
+        T.nyz[k] = (n.x >= 0.0f) ? V2{ -ek.z, ek.y } : V2{ ek.z, -ek.y };
+        T.nzx[k] = (n.y >= 0.0f) ? V2{ -ek.x, ek.z } : V2{ ek.x, -ek.z };
         const f3 vk = v[k];
         if (a.mode == 0) {
-            dxy[k] = dot2(nxy[k], 0.5f - vk.x, 0.5f - vk.y) + 0.5f * vt_maxf(vt_absf(nxy[k].x), vt_absf(nxy[k].y));
-            dyz[k] = dot2(nyz[k], 0.5f - vk.y, 0.5f - vk.z) + 0.5f * vt_maxf(vt_absf(nyz[k].x), vt_absf(nyz[k].y));
-            dzx[k] = dot2(nzx[k], 0.5f - vk.z, 0.5f - vk.x) + 0.5f * vt_maxf(vt_absf(nzx[k].x), vt_absf(nzx[k].y));
+            T.dxy[k] = dot2(T.nxy[k], 0.5f - vk.x, 0.5f - vk.y) +
+                       0.5f * vt_maxf(vt_absf(T.nxy[k].x), vt_absf(T.nxy[k].y));
+            T.dyz[k] = dot2(T.nyz[k], 0.5f - vk.y, 0.5f - vk.z) +
+                       0.5f * vt_maxf(vt_absf(T.nyz[k].x), vt_absf(T.nyz[k].y));
+            T.dzx[k] = dot2(T.nzx[k], 0.5f - vk.z, 0.5f - vk.x) +
+                       0.5f * vt_maxf(vt_absf(T.nzx[k].x), vt_absf(T.nzx[k].y));
         } else {
-            dxy[k] = (-dot2(nxy[k], vk.x, vk.y) + vt_maxf(0.0f, nxy[k].x)) + vt_maxf(0.0f, nxy[k].y);
-            dyz[k] = (-dot2(nyz[k], vk.y, vk.z) + vt_maxf(0.0f, nyz[k].x)) + vt_maxf(0.0f, nyz[k].y);
-            dzx[k] = (-dot2(nzx[k], vk.z, vk.x) + vt_maxf(0.0f, nzx[k].x)) + vt_maxf(0.0f, nzx[k].y);
+            T.dxy[k] = (-dot2(T.nxy[k], vk.x, vk.y) + vt_maxf(0.0f, T.nxy[k].x)) + vt_maxf(0.0f, T.nxy[k].y);
+            T.dyz[k] = (-dot2(T.nyz[k], vk.y, vk.z) + vt_maxf(0.0f, T.nyz[k].x)) + vt_maxf(0.0f, T.nyz[k].y);
+            T.dzx[k] = (-dot2(T.nzx[k], vk.z, vk.x) + vt_maxf(0.0f, T.nzx[k].x)) + vt_maxf(0.0f, T.nzx[k].y);
         }
     }
-    const f3 np = (n.z < 0.0f) ? -n : n;
-    const float dTri = dot(np, v[0]);
-    const float dMin = (a.mode == 0) ? dTri - (np.x * 0.5f + np.y * 0.5f)
-                                     : (dTri - vt_maxf(np.x, 0.0f)) - vt_maxf(np.y, 0.0f);
-    const float dMax = (a.mode == 0) ? dMin : (dTri - vt_minf(np.x, 0.0f)) - vt_minf(np.y, 0.0f);
-    const float nzInv = 1.0f / np.z;
-    for (int32_t x = mn[0]; x < mx[0]; ++x) {
-        const float fx = (float)x;
-        for (int32_t y = mn[1]; y < mx[1]; ++y) {
-            const float fy = (float)y;
-            const float d0 = dxy[0] + dot2(nxy[0], fx, fy);
-            const float d1 = dxy[1] + dot2(nxy[1], fx, fy);
-            const float d2 = dxy[2] + dot2(nxy[2], fx, fy);
-            if (!(d0 >= 0.0f && d1 >= 0.0f && d2 >= 0.0f)) continue;
-            const float dnp = np.x * fx + np.y * fy;
-            const float zMinInt = (-dnp + dMin) * nzInv;
-            const float zMaxInt = (a.mode == 0) ? zMinInt : (-dnp + dMax) * nzInv;
-            const float zMinFloor = floorf(zMinInt), zMaxCeil = ceilf(zMaxInt);
-            int32_t zMin = wrap_add(cvt_i32(zMinFloor), -(int32_t)(zMinFloor == zMinInt));
-            int32_t zMax = wrap_add(cvt_i32(zMaxCeil), (int32_t)(zMaxCeil == zMaxInt));
-            if (zMin < mn[2]) zMin = mn[2];
-            if (zMax > mx[2]) zMax = mx[2];
-            for (int32_t z = zMin; z < zMax; ++z) {
-                const float fz = (float)z;
-                bool ok = true;
+    T.np = (n.z < 0.0f) ? -n : n;
+    const float dTri = dot(T.np, v[0]);
+    T.dMin = (a.mode == 0) ? dTri - (T.np.x * 0.5f + T.np.y * 0.5f)
+                           : (dTri - vt_maxf(T.np.x, 0.0f)) - vt_maxf(T.np.y, 0.0f);
+    T.dMax = (a.mode == 0) ? T.dMin : (dTri - vt_minf(T.np.x, 0.0f)) - vt_minf(T.np.y, 0.0f);
+    T.nzInv = 1.0f / T.np.z;
+    return true;
+}
+
+/* One (x, y) column of the triangle's loop (voxelize.gs:181-231): the 2D
+ * edge test in the dominant plane, the z-range from the plane, the tests in
+ * the other two planes, and the write of every covered voxel (unswizzled). */
+__device__ __forceinline__ void vox_column(const VoxArgs& a, const VoxTri& T, int32_t x, int32_t y)
+{
+    const int32_t res[3] = { a.d.nx, a.d.ny, a.d.nz };
+    const float fx = (float)x, fy = (float)y;
+    const float d0 = T.dxy[0] + dot2(T.nxy[0], fx, fy);
+    const float d1 = T.dxy[1] + dot2(T.nxy[1], fx, fy);
+    const float d2 = T.dxy[2] + dot2(T.nxy[2], fx, fy);
+    if (!(d0 >= 0.0f && d1 >= 0.0f && d2 >= 0.0f)) return;
+    const float dnp = T.np.x * fx + T.np.y * fy;
+    const float zMinInt = (-dnp + T.dMin) * T.nzInv;
+    const float zMaxInt = (a.mode == 0) ? zMinInt : (-dnp + T.dMax) * T.nzInv;
+    const float zMinFloor = floorf(zMinInt), zMaxCeil = ceilf(zMaxInt);
+    int32_t zMin = wrap_add(cvt_i32(zMinFloor), -(int32_t)(zMinFloor == zMinInt));
+    int32_t zMax = wrap_add(cvt_i32(zMaxCeil), (int32_t)(zMaxCeil == zMaxInt));
+    if (zMin < T.mn[2]) zMin = T.mn[2];
+    if (zMax > T.mx[2]) zMax = T.mx[2];
+    for (int32_t z = zMin; z < zMax; ++z) {
+        const float fz = (float)z;
+        bool ok = true;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    ok = ok && (dyz[k] + dot2(nyz[k], fy, fz) >= 0.0f);
-                    ok = ok && (dzx[k] + dot2(nzx[k], fz, fx) >= 0.0f);
-                }
-                if (!ok) continue;
-                int32_t c0, c1, c2;
-                if (lut == 0) { c0 = z; c1 = x; c2 = y; }
-                else if (lut == 1) { c0 = y; c1 = z; c2 = x; }
-                else { c0 = x; c1 = y; c2 = z; }
-                if ((uint32_t)c0 >= (uint32_t)res[0] || (uint32_t)c1 >= (uint32_t)res[1] ||
-                    (uint32_t)c2 >= (uint32_t)res[2])
-                    continue;
-                if (a.bricks) a.bricks[brick_byte(c0, c1, c2)] = a.slot;
-                if (a.occ) a.occ[(size_t)c0 + (size_t)c1 * res[0] + (size_t)c2 * (size_t)res[0] * res[1]] = 1u;
-            }
+        for (int k = 0; k < 3; ++k) {
+            ok = ok && (T.dyz[k] + dot2(T.nyz[k], fy, fz) >= 0.0f);
+            ok = ok && (T.dzx[k] + dot2(T.nzx[k], fz, fx) >= 0.0f);
         }
+        if (!ok) continue;
+        int32_t c0, c1, c2;
+        if (T.lut == 0) { c0 = z; c1 = x; c2 = y; }
+        else if (T.lut == 1) { c0 = y; c1 = z; c2 = x; }
+        else { c0 = x; c1 = y; c2 = z; }
+        if ((uint32_t)c0 >= (uint32_t)res[0] || (uint32_t)c1 >= (uint32_t)res[1] || (uint32_t)c2 >= (uint32_t)res[2])
+            continue;
+        if (a.bricks) a.bricks[brick_byte(c0, c1, c2)] = a.slot;
+        if (a.occ) a.occ[(size_t)c0 + (size_t)c1 * res[0] + (size_t)c2 * (size_t)res[0] * res[1]] = 1u;
+    }
+}
+
+/* One thread per triangle.  Workgroup i works on triangle chunk
+ * (i % 8) * per + i / 8: workgroups go to the eight XCDs in turn, so each
+ * XCD takes one contiguous run of the triangle list (neighbouring triangles,
+ * whose voxels share 64-byte lines, are then written from one L2 instead of
+ * being merged from eight).  A triangle whose column loop exceeds
+ * a.big_area columns is listed for voxelize_big_kernel instead. */
+#define VOX_XCDS 8u
+__global__ __launch_bounds__(256) void voxelize_kernel(VoxArgs a)
+{
+    const uint32_t per = (gridDim.x + VOX_XCDS - 1u) / VOX_XCDS;
+    const uint64_t chunk = (uint64_t)(blockIdx.x % VOX_XCDS) * per + blockIdx.x / VOX_XCDS;
+    const int64_t tri = (int64_t)chunk * blockDim.x + threadIdx.x;
+    if (tri >= a.ntri) return;
+    VoxTri T;
+    if (!vox_setup(a, tri, T)) return;
+    const int64_t w = T.mx[0] > T.mn[0] ? T.mx[0] - T.mn[0] : 0, h = T.mx[1] > T.mn[1] ? T.mx[1] - T.mn[1] : 0;
+    if (a.big && w * h > a.big_area && tri < (int64_t)0xffffffff) {
+        const uint32_t slot = atomicAdd(a.big_count, 1u);
+        if (slot < a.big_cap) {
+            a.big[slot] = (uint32_t)tri;
+            return;
+        }
+    }
+    for (int32_t x = T.mn[0]; x < T.mx[0]; ++x)
+        for (int32_t y = T.mn[1]; y < T.mx[1]; ++y) vox_column(a, T, x, y);
+}
+
+/* The listed large triangles, one wave each: the lanes take the triangle's
+ * (x, y) columns in turn (consecutive lanes, consecutive y). */
+__global__ __launch_bounds__(256) void voxelize_big_kernel(VoxArgs a)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t n = *a.big_count < a.big_cap ? *a.big_count : a.big_cap;
+    for (uint32_t i = wave; i < n; i += waves) {
+        VoxTri T;
+        if (!vox_setup(a, (int64_t)a.big[i], T)) continue;
+        const int64_t w = T.mx[0] - T.mn[0], h = T.mx[1] - T.mn[1];
+        for (int64_t k = lane; k < w * h; k += 64) vox_column(a, T, T.mn[0] + (int32_t)(k / h), T.mn[1] + (int32_t)(k % h));
     }
 }
 
@@ -382,10 +448,17 @@ hipError_t launch_env_quads(const float* img, int32_t w, int32_t h, float4* quad
     return hipGetLastError();
 }
 
-hipError_t launch_voxelize(const VoxArgs& a, hipStream_t s)
+hipError_t launch_voxelize(const VoxArgs& a, int cus, hipStream_t s)
 {
     if (a.ntri <= 0) return hipSuccess;
-    voxelize_kernel<<<(unsigned)((a.ntri + 255) / 256), 256, 0, s>>>(a);
+    /* a multiple of the XCD count, so the chunk mapping covers every chunk */
+    const uint64_t g = (uint64_t)(a.ntri + 255) / 256;
+    const uint64_t g8 = (g + VOX_XCDS - 1) / VOX_XCDS * VOX_XCDS;
+    voxelize_kernel<<<(unsigned)g8, 256, 0, s>>>(a);
+    if (a.big) {
+        const uint64_t waves = (uint64_t)a.big_cap < (uint64_t)cus * 32u ? (uint64_t)a.big_cap : (uint64_t)cus * 32u;
+        voxelize_big_kernel<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(a);
+    }
     return hipGetLastError();
 }
 
