@@ -160,6 +160,13 @@ int vt_read_accum(vt_context* ctx, float* rgba);
 /* FP64 running sums (W*H*3 doubles) when vt_set_frame(fp64=1). */
 int vt_read_sum64(vt_context* ctx, double* rgb);
 int vt_get_stats(vt_context* ctx, vt_stats* out);
+/* Resume of a progressive render: write back a running mean (and FP64 sums)
+ * read earlier with vt_read_accum / vt_read_sum64 (same layouts, the frame
+ * of the same size), so that vt_render(ctx, seed, n, ...) continues the
+ * interrupted render exactly.  No counterpart in the reference (its
+ * accumulation lives in GL textures). */
+int vt_write_accum(vt_context* ctx, const float* rgba);
+int vt_write_sum64(vt_context* ctx, const double* rgb);
 /* Wavefront renders of at least 2 passes and 4M paths run their first and
  * second half of the passes on two streams of the device (1 = one stream).
  * Images are identical (the halves accumulate in pass order); the second

@@ -69,6 +69,11 @@ int vtr_load_volume(vtr_renderer* r, int32_t nx, int32_t ny, int32_t nz, const i
 int vtr_get_volume(vtr_renderer* r, int32_t* res, float* bmin, float* bmax, int32_t* offsets);
 int vtr_get_emissive(vtr_renderer* r, int32_t* out, int32_t capacity, int32_t* count);
 int vtr_save_image(vtr_renderer* r, const char* path);
+/* Checkpoint / resume of the progressive render (frame size, sample count,
+ * seed, running mean); load into a renderer with the saved one's scene,
+ * camera and settings.  No counterpart in the reference. */
+int vtr_save_checkpoint(vtr_renderer* r, const char* path);
+int vtr_load_checkpoint(vtr_renderer* r, const char* path);
 
 int vtr_get_render_settings(vtr_renderer* r, vtr_settings* out);
 /* Copy the settings in and apply them (Renderer::updateRenderSettings). */

@@ -78,6 +78,64 @@ def test_renderer_passes_bit_exact(vr, orc):
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
+def test_checkpoint_resume_equals_uninterrupted_render(vr, tmp_path):
+    """3 passes, checkpoint, a new renderer with the same scene and camera
+    loads it and renders 2 more: the image equals 5 uninterrupted passes;
+    the same through the context ABI with FP64 sums.  A checkpoint of
+    another frame size is refused."""
+    from voxeltoy_amd import api, workloads
+    offs, mat = small_volume()
+
+    def fresh():
+        r = make_renderer(vr, 64, 48)
+        r.load_volume(offs, mat)
+        r.focus_on_bounds()
+        r.orbit(math.radians(115), math.radians(40))
+        return r
+
+    ref = fresh()
+    ref.set_seed(5)
+    for _ in range(5):
+        ref.render()
+    a = fresh()
+    a.set_seed(5)
+    for _ in range(3):
+        a.render()
+    path = str(tmp_path / "progress.ckpt")
+    a.save_checkpoint(path)
+    b = fresh()
+    b.set_seed(999)                       # the checkpoint carries the seed
+    b.load_checkpoint(path)
+    assert b.number_samples() == 3
+    for _ in range(2):
+        b.render()
+    assert b.number_samples() == 5
+    assert np.array_equal(b.read_accum().view(np.uint32), ref.read_accum().view(np.uint32))
+    c = make_renderer(vr, 32, 48)
+    c.load_volume(offs, mat)
+    with pytest.raises(vr.RendererError):
+        c.load_checkpoint(path)
+    # the context ABI: running mean and FP64 sums written back
+    wl = workloads.small(1, width=48, height=40)
+    x = api.Context(0)
+    workloads.load(x, wl, fp64=True)
+    x.render(wl.seed, 0, 4)
+    whole, whole64 = x.read_accum(), x.read_sum64()
+    y = api.Context(0)
+    workloads.load(y, wl, fp64=True)
+    y.render(wl.seed, 0, 2)
+    acc, s64 = y.read_accum(), y.read_sum64()
+    z = api.Context(0)
+    workloads.load(z, wl, fp64=True)
+    z.write_accum(acc)
+    z.write_sum64(s64)
+    z.render(wl.seed, 2, 2)
+    assert np.array_equal(z.read_accum().view(np.uint32), whole.view(np.uint32))
+    assert np.array_equal(z.read_sum64().view(np.uint64), whole64.view(np.uint64))
+    for ctx in (x, y, z):
+        ctx.close()
+
+
 def test_renderer_finishes_at_max_samples(vr):
     offs, mat = small_volume(16)
     r = make_renderer(vr, 16, 16, bounces=1)

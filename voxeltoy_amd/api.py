@@ -121,6 +121,8 @@ SIGNATURES = {
     "vt_render": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
     "vt_read_accum": (C.c_int, [C.c_void_p, f32p]),
     "vt_read_sum64": (C.c_int, [C.c_void_p, f64p]),
+    "vt_write_accum": (C.c_int, [C.c_void_p, f32p]),
+    "vt_write_sum64": (C.c_int, [C.c_void_p, f64p]),
     "vt_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "vt_set_launch_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "vt_set_overlap": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -398,6 +400,19 @@ class Context:
         out = np.zeros((self.height, self.width, 3), np.float64)
         _check(lib().vt_read_sum64(self._c, out.reshape(-1)))
         return out
+
+    def write_accum(self, rgba: np.ndarray):
+        """Resume: the running mean read earlier with read_accum."""
+        a = np.ascontiguousarray(rgba, np.float32)
+        if a.size != self.height * self.width * 4:
+            raise ValueError("running mean of another frame size")
+        _check(lib().vt_write_accum(self._c, a.reshape(-1)))
+
+    def write_sum64(self, rgb: np.ndarray):
+        a = np.ascontiguousarray(rgb, np.float64)
+        if a.size != self.height * self.width * 3:
+            raise ValueError("FP64 sums of another frame size")
+        _check(lib().vt_write_sum64(self._c, a.reshape(-1)))
 
     def stats(self) -> dict:
         s = Stats()

@@ -1833,6 +1833,26 @@ int vt_read_sum64(vt_context* c, double* rgb)
     return VT_OK;
 }
 
+int vt_write_accum(vt_context* c, const float* rgba)
+{
+    if (!c || !rgba) return fail(VT_ERR_ARG, "null argument");
+    if (!c->d_accum) return fail(VT_ERR_STATE, "no frame");
+    VT_HIP(hipSetDevice(c->device));
+    VT_HIP(hipMemcpyAsync(c->d_accum, rgba, (size_t)c->W * c->H * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
+int vt_write_sum64(vt_context* c, const double* rgb)
+{
+    if (!c || !rgb) return fail(VT_ERR_ARG, "null argument");
+    if (!c->d_sum64) return fail(VT_ERR_STATE, "frame has no FP64 sums");
+    VT_HIP(hipSetDevice(c->device));
+    VT_HIP(hipMemcpyAsync(c->d_sum64, rgb, (size_t)c->W * c->H * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
 int vt_get_stats(vt_context* c, vt_stats* out)
 {
     if (!c || !out) return fail(VT_ERR_ARG, "null argument");

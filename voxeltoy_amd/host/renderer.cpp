@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <sstream>
 
 namespace voxeltoy {
@@ -364,6 +365,79 @@ void Renderer::saveImage(const std::string& file)
     }
     std::string err;
     if (!writeImage(file, px.data(), W, H, &err)) fail("saveImage: " + err);
+}
+
+/* Checkpoint file: 8-byte tag, then int32 width, height, sample count and a
+ * zero word, uint64 seed, then width x height x 4 floats (row 0 = bottom). */
+namespace {
+const char kCheckpointTag[8] = { 'V', 'T', 'C', 'K', 'P', 'T', '0', '1' };
+struct CheckpointHeader {
+    char tag[8];
+    int32_t width, height, samples, zero;
+    uint64_t seed;
+};
+}  // namespace
+
+bool Renderer::saveCheckpoint(const std::string& file)
+{
+    if (!m_initialized || !m_ctx || m_frameW <= 0) {
+        fail("saveCheckpoint: nothing rendered");
+        return false;
+    }
+    std::vector<float> px((size_t)m_frameW * m_frameH * 4);
+    if (vt_read_accum(m_ctx, px.data()) != VT_OK) {
+        fail(std::string("saveCheckpoint: ") + vt_last_error());
+        return false;
+    }
+    CheckpointHeader h;
+    std::memcpy(h.tag, kCheckpointTag, sizeof(h.tag));
+    h.width = m_frameW;
+    h.height = m_frameH;
+    h.samples = m_numberSamples;
+    h.zero = 0;
+    h.seed = m_seed;
+    std::ofstream f(file, std::ios::binary);
+    f.write((const char*)&h, sizeof(h));
+    f.write((const char*)px.data(), (std::streamsize)(px.size() * sizeof(float)));
+    if (!f) {
+        fail("saveCheckpoint: cannot write " + file);
+        return false;
+    }
+    return true;
+}
+
+bool Renderer::loadCheckpoint(const std::string& file)
+{
+    if (!m_initialized || !m_ctx) {
+        fail("loadCheckpoint: renderer not initialized");
+        return false;
+    }
+    std::ifstream f(file, std::ios::binary);
+    CheckpointHeader h;
+    if (!f.read((char*)&h, sizeof(h)) || std::memcmp(h.tag, kCheckpointTag, sizeof(h.tag)) != 0) {
+        fail("loadCheckpoint: " + file + " is not a checkpoint");
+        return false;
+    }
+    if (h.width != m_renderSettings.m_imageResolution.x || h.height != m_renderSettings.m_imageResolution.y ||
+        h.samples < 0) {
+        fail("loadCheckpoint: checkpoint frame does not match the renderer's image resolution");
+        return false;
+    }
+    std::vector<float> px((size_t)h.width * h.height * 4);
+    if (!f.read((char*)px.data(), (std::streamsize)(px.size() * sizeof(float)))) {
+        fail("loadCheckpoint: " + file + " is truncated");
+        return false;
+    }
+    fitFilmToImage();
+    if (!ensureFrame()) return false;
+    updateCamera();
+    if (vt_write_accum(m_ctx, px.data()) != VT_OK) {
+        fail(std::string("loadCheckpoint: ") + vt_last_error());
+        return false;
+    }
+    m_numberSamples = h.samples;
+    m_seed = h.seed;
+    return true;
 }
 
 /* Renderer::onMouseMove (renderer.cpp:658-669): pixel motion normalised by
@@ -720,6 +794,18 @@ int vtr_save_image(vtr_renderer* r, const char* path)
     if (!path) return VT_ERR_ARG;
     if (r && !r->r.context()) return VT_ERR_STATE;
     return guarded(r, VT_ERR_ARG, [&] { r->r.saveImage(path); });
+}
+
+int vtr_save_checkpoint(vtr_renderer* r, const char* path)
+{
+    if (!r || !path) return VT_ERR_ARG;
+    return r->r.saveCheckpoint(path) ? VT_OK : VT_ERR_STATE;
+}
+
+int vtr_load_checkpoint(vtr_renderer* r, const char* path)
+{
+    if (!r || !path) return VT_ERR_ARG;
+    return r->r.loadCheckpoint(path) ? VT_OK : VT_ERR_STATE;
 }
 
 int vtr_get_render_settings(vtr_renderer* r, vtr_settings* o)

@@ -110,6 +110,14 @@ public:
                                      std::vector<int32_t>& emissiveVoxelIndices);
     void saveImage(const std::string& file);
     void resetRender();
+    /* Checkpoint of the progressive render (no counterpart in the reference):
+     * frame size, sample count, RNG seed and the RGBA32F running mean.
+     * loadCheckpoint restores them into a renderer holding the saved one's
+     * scene, camera and settings (which the checkpoint does not carry), so
+     * that rendering on gives the image of the uninterrupted render.
+     * Failures go to the logger and return false. */
+    bool saveCheckpoint(const std::string& file);
+    bool loadCheckpoint(const std::string& file);
 
     /* Mouse and key input through the camera's active controller
      * (Camera::setCameraController: orbit or fly), pixel motion normalised

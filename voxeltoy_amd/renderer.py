@@ -58,6 +58,8 @@ SIGNATURES = {
     "vtr_get_volume": (C.c_int, [_R, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vtr_get_emissive": (C.c_int, [_R, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "vtr_save_image": (C.c_int, [_R, C.c_char_p]),
+    "vtr_save_checkpoint": (C.c_int, [_R, C.c_char_p]),
+    "vtr_load_checkpoint": (C.c_int, [_R, C.c_char_p]),
     "vtr_get_render_settings": (C.c_int, [_R, C.POINTER(Settings)]),
     "vtr_update_render_settings": (C.c_int, [_R, C.POINTER(Settings)]),
     "vtr_camera_get": (C.c_int, [_R, C.POINTER(CameraState)]),
@@ -210,6 +212,12 @@ class Renderer:
 
     def save_image(self, path: str):
         self._check(lib().vtr_save_image(self._r, str(path).encode()))
+
+    def save_checkpoint(self, path: str):
+        self._check(lib().vtr_save_checkpoint(self._r, str(path).encode()))
+
+    def load_checkpoint(self, path: str):
+        self._check(lib().vtr_load_checkpoint(self._r, str(path).encode()))
 
     def render_settings(self) -> Settings:
         s = Settings()
