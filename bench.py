@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # stream), from rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), each
 # factor being what the counter reads for a copy kernel of known size
 # (profiles/wf_traverse_kernel_full_1024x1024_16spp.md).
-TRAFFIC: dict = {("wf_traverse_kernel", 1): 480573290}
+TRAFFIC: dict = {("wf_traverse_kernel", 1): 481225141}
 
 
 def env_lookup_bytes(cu_w: int, cv_n: int) -> tuple[int, int]:
