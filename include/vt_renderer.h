@@ -69,6 +69,12 @@ int vtr_load_volume(vtr_renderer* r, int32_t nx, int32_t ny, int32_t nz, const i
 int vtr_get_volume(vtr_renderer* r, int32_t* res, float* bmin, float* bmax, int32_t* offsets);
 int vtr_get_emissive(vtr_renderer* r, int32_t* out, int32_t capacity, int32_t* count);
 int vtr_save_image(vtr_renderer* r, const char* path);
+/* CPUVoxelizer::voxelizeMesh (src/voxelize/cpuVoxelizer.h:5-13) on host
+ * threads, with the mesh placed like vt_voxelize places it (row-major model
+ * transform, resolution res[3]); occupancy (res[0] x res[1] x res[2] bytes,
+ * x fastest) gets 1 at every covered voxel.  mode 0 THIN, 1 FAT. */
+int vtr_cpu_voxelize(const float* vertices, int64_t n_vertices, const uint32_t* indices, int64_t n_triangles,
+                     const float* model_transform, const int32_t* res, int32_t mode, uint8_t* occupancy);
 /* Checkpoint / resume of the progressive render (frame size, sample count,
  * seed, running mean); load into a renderer with the saved one's scene,
  * camera and settings.  No counterpart in the reference. */

@@ -20,10 +20,16 @@ LIB_PATH = os.path.join(HERE, "build", "libvtoracle.so")
 
 
 def build(force: bool = False) -> str:
+    """make the library; one build at a time (test workers call this
+    concurrently), under a lock file next to it."""
+    import fcntl
+    os.makedirs(os.path.join(HERE, "build"), exist_ok=True)
     args = ["make", "-C", HERE]
     if force:
         args.insert(1, "-B")
-    subprocess.run(args, check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    with open(os.path.join(HERE, "build", ".lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        subprocess.run(args, check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     return LIB_PATH
 
 

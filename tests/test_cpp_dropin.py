@@ -22,3 +22,19 @@ def test_glwidget_calls_compile_and_run(vtapi, tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert r.stdout.strip() == "OK"
+
+
+def test_environment_image_formats(vtapi, tmp_path):
+    """readImage (the library's loadImage): Radiance RGBE with run-length
+    and flat scanlines in both row orders, 8-bit PPM and 16-bit PGM, decoded
+    to the expected floats; other formats and truncated files refused."""
+    lib_dir = os.path.dirname(vtapi.library_path())
+    exe = str(tmp_path / "image_io")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror",
+                    "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "voxeltoy_amd", "host"),
+                    os.path.join(ROOT, "tests", "cpp", "image_io.cpp"), "-o", exe,
+                    "-L", lib_dir, "-lvoxeltoy", f"-Wl,-rpath,{lib_dir}"], check=True, capture_output=True,
+                   text=True)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "OK"

@@ -293,3 +293,39 @@ def test_save_image_pfm_and_ppm(vr, tmp_path):
     assert np.abs(img.astype(int) - want.astype(int)).max() <= 1
     with pytest.raises(vr.RendererError):
         r.save_image(str(tmp_path / "a.exr"))
+
+
+def test_hdr_background_renders_like_the_same_pfm(vr, tmp_path):
+    """An environment image given as Radiance RGBE (.hdr) renders exactly
+    like its decoded floats given as PFM: the renderer reads both formats
+    into the same environment (image, CDF and all)."""
+    rng = np.random.default_rng(3)
+    h, w = 32, 64
+    sky = (rng.uniform(0.05, 2.0, (h, w, 3)) * np.linspace(2.0, 0.2, h)[:, None, None]).astype(np.float64)
+    sky[4:6, 40:43] = 900.0                                          # a sun
+    mx = sky.max(axis=-1)
+    m, e = np.frexp(mx)
+    rgbe = np.zeros((h, w, 4), np.uint8)
+    rgbe[..., :3] = np.floor(sky * (m * 256.0 / mx)[..., None]).astype(np.uint8)
+    rgbe[..., 3] = (e + 128).astype(np.uint8)
+    # what the format's reader decodes: (r, g, b) * 2^(e - 136), in float
+    decoded = (rgbe[..., :3].astype(np.float32) * np.ldexp(np.float32(1.0), rgbe[..., 3].astype(np.int32) - 136)[..., None]).astype(np.float32)
+    hdr = tmp_path / "sky.hdr"
+    hdr.write_bytes(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n" % (h, w) + rgbe.tobytes())
+    pfm = tmp_path / "sky.pfm"
+    pfm.write_bytes(b"PF\n%d %d\n-1.0\n" % (w, h) + decoded[::-1].astype("<f4").tobytes())   # PFM: bottom row first
+    offs, mat = small_volume(24)
+    imgs = []
+    for path in (hdr, pfm):
+        r = make_renderer(vr, 40, 30, bounces=2)
+        r.load_volume(offs, mat)
+        r.focus_on_bounds()
+        s = r.render_settings()
+        s.background_image = str(path).encode()
+        r.update_render_settings(s)
+        r.set_seed(7)
+        for _ in range(2):
+            r.render()
+        imgs.append(r.read_accum())
+    assert np.isfinite(imgs[0]).any() and imgs[0][..., :3].max() > 0
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))

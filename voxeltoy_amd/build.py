@@ -19,7 +19,8 @@ LIB_DIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIB_DIR, "libvoxeltoy.so")
 
 HIP_SOURCES = ["csrc/vt_trace.hip", "csrc/vt_volume.hip", "csrc/vt_cache.hip", "csrc/vt_api.hip"]
-CXX_SOURCES = ["host/vt_host_algos.cpp", "host/camera.cpp", "host/scene_io.cpp", "host/renderer.cpp"]
+CXX_SOURCES = ["host/vt_host_algos.cpp", "host/camera.cpp", "host/scene_io.cpp", "host/renderer.cpp",
+               "host/cpu_voxelizer.cpp"]
 HEADERS = ["csrc/vtmath.h", "csrc/vt_device.h", "csrc/vt_kernels.h", "csrc/vt_cache.h", "host/vt_host_algos.h",
            "host/camera.h", "host/scene_io.h", "host/renderer.h", "host/vt_math_types.h"]
 API_HEADERS = ["vt_api.h", "vt_renderer.h"]

@@ -198,7 +198,7 @@ void Renderer::updateRenderSettings()
         if (!s.m_backgroundImage.empty()) {
             int w = 0, h = 0;
             std::string err;
-            if (!readImagePFM(s.m_backgroundImage, img, w, h, &err)) {
+            if (!readImage(s.m_backgroundImage, img, w, h, &err)) {
                 fail("background image: " + err);
                 return;
             }

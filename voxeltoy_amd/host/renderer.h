@@ -52,7 +52,7 @@ struct RenderSettings {
     int m_viewport[4] = { 0, 0, 512, 512 };
     float m_wireframeOpacity = 0.0f;
     float m_wireframeThickness = 0.01f;
-    /* path of a float RGB .pfm environment image; empty = gradient */
+    /* path of an environment image (.pfm, .hdr/.pic, .ppm/.pgm); empty = gradient */
     std::string m_backgroundImage;
     V3f m_backgroundColor[2] = { V3f(153.0f / 255 * 2, 187.0f / 255 * 2, 201.0f / 255 * 2),
                                  V3f(77.0f / 255, 64.0f / 255, 50.0f / 255) };   /* top, bottom */
@@ -81,6 +81,23 @@ public:
 private:
     vt_context* m_ctx;
     Logger* m_logger;
+};
+
+/* CPUVoxelizer (src/voxelize/cpuVoxelizer.h:5-13): the same voxelization on
+ * host threads (0 = one per hardware thread).  The first form takes
+ * vertices already in voxel space (model transform and resolution applied,
+ * as the reference's caller does, import.cpp:98-106); the second places a
+ * mesh as GPUVoxelizer's vertex stage does.  Covered voxels of voxelStorage
+ * (x fastest) are set to 1; mode 0 THIN (the reference's setting), 1 FAT.
+ * Every triangle is voxelized: the reference's task split misses
+ * triangles when the mesh has fewer triangles than threads
+ * (cpuVoxelizer.cpp:289-299). */
+class CPUVoxelizer {
+public:
+    static void voxelizeMesh(const V3f* vertices, const unsigned int* indices, unsigned int numTriangles,
+                             const V3i& voxelDimensions, unsigned char* voxelStorage, int mode = 0, int threads = 0);
+    static void voxelizeMesh(const Mesh* mesh, const M44f& meshTransform, const V3i& resolution,
+                             unsigned char* voxelStorage, int mode = 0, int threads = 0);
 };
 
 /* computeMeshTransform (import.cpp:46-64): fit the bounds into the unit cube

@@ -490,4 +490,143 @@ bool readImagePFM(const std::string& file, std::vector<float>& rgb, int& w, int&
     return true;
 }
 
+/* Radiance RGBE (.hdr / .pic): "#?RADIANCE" or "#?RGBE" header lines up to
+ * an empty line, the resolution line "-Y h +X w" (rows top down) or "+Y h +X w"
+ * (bottom up), then the scanlines, each either run-length encoded per
+ * channel (starting 2, 2, w >> 8, w & 255; runs of count - 128 copies of one
+ * byte, or count literal bytes) or 4 flat bytes per pixel.  A pixel is
+ * (r, g, b) * 2^(e - 136), 0 for e = 0, the conversion of the format's
+ * reference reader (rgbe2float), which OpenImageIO's reader uses too. */
+static bool read_hdr(const std::vector<uint8_t>& buf, std::vector<float>& rgb, int& w, int& h, std::string* error)
+{
+    size_t pos = 0;
+    auto line = [&](std::string& t) {
+        t.clear();
+        while (pos < buf.size() && buf[pos] != '\n') t.push_back((char)buf[pos++]);
+        if (pos >= buf.size()) return false;
+        ++pos;
+        return true;
+    };
+    std::string t;
+    if (!line(t) || (t.rfind("#?RADIANCE", 0) != 0 && t.rfind("#?RGBE", 0) != 0)) {
+        if (error) *error = "not a Radiance HDR file";
+        return false;
+    }
+    for (;;) {
+        if (!line(t)) { if (error) *error = "truncated HDR header"; return false; }
+        if (t.empty()) break;
+        if (t.rfind("FORMAT=", 0) == 0 && t != "FORMAT=32-bit_rle_rgbe") {
+            if (error) *error = "unsupported HDR pixel format " + t;
+            return false;
+        }
+    }
+    if (!line(t)) { if (error) *error = "missing HDR resolution line"; return false; }
+    char ys[3] = { 0 }, xs[3] = { 0 };
+    if (std::sscanf(t.c_str(), "%2s %d %2s %d", ys, &h, xs, &w) != 4 || std::strcmp(xs, "+X") != 0 ||
+        (std::strcmp(ys, "-Y") != 0 && std::strcmp(ys, "+Y") != 0) || w <= 0 || h <= 0 || w > 65535 || h > 65535) {
+        if (error) *error = "unsupported HDR orientation or size: " + t;
+        return false;
+    }
+    const bool top_down = std::strcmp(ys, "-Y") == 0;
+    rgb.assign((size_t)w * h * 3, 0.0f);
+    std::vector<uint8_t> row((size_t)w * 4);
+    for (int y = 0; y < h; ++y) {
+        const bool rle = w >= 8 && w < 32768 && pos + 4 <= buf.size() && buf[pos] == 2 && buf[pos + 1] == 2 &&
+                         (((int)buf[pos + 2] << 8) | buf[pos + 3]) == w && !(buf[pos + 2] & 0x80);
+        if (rle) {
+            pos += 4;
+            for (int c = 0; c < 4; ++c)
+                for (int x = 0; x < w;) {
+                    if (pos >= buf.size()) { if (error) *error = "truncated HDR scanline"; return false; }
+                    int n = buf[pos++];
+                    if (n > 128) {
+                        n -= 128;
+                        if (x + n > w || pos >= buf.size()) { if (error) *error = "bad HDR run"; return false; }
+                        const uint8_t v = buf[pos++];
+                        for (int k = 0; k < n; ++k) row[(size_t)(x++) * 4 + c] = v;
+                    } else {
+                        if (n == 0 || x + n > w || pos + n > buf.size()) { if (error) *error = "bad HDR run"; return false; }
+                        for (int k = 0; k < n; ++k) row[(size_t)(x++) * 4 + c] = buf[pos++];
+                    }
+                }
+        } else {
+            if (pos + (size_t)w * 4 > buf.size()) { if (error) *error = "truncated HDR pixels"; return false; }
+            std::memcpy(row.data(), &buf[pos], (size_t)w * 4);
+            pos += (size_t)w * 4;
+        }
+        const int dst_row = top_down ? y : h - 1 - y;
+        for (int x = 0; x < w; ++x) {
+            const uint8_t* q = &row[(size_t)x * 4];
+            const float f = q[3] ? std::ldexp(1.0f, (int)q[3] - 136) : 0.0f;
+            float* o = &rgb[((size_t)dst_row * w + x) * 3];
+            o[0] = q[3] ? (float)q[0] * f : 0.0f;
+            o[1] = q[3] ? (float)q[1] * f : 0.0f;
+            o[2] = q[3] ? (float)q[2] * f : 0.0f;
+        }
+    }
+    return true;
+}
+
+/* Binary PPM / PGM (P6 / P5, maxval up to 65535): values / maxval, the
+ * normalised float conversion of OpenImageIO's integer formats. */
+static bool read_pnm(const std::vector<uint8_t>& buf, std::vector<float>& rgb, int& w, int& h, std::string* error)
+{
+    size_t pos = 0;
+    auto token = [&](std::string& t) {
+        for (;;) {
+            while (pos < buf.size() && std::isspace(buf[pos])) ++pos;
+            if (pos < buf.size() && buf[pos] == '#') {
+                while (pos < buf.size() && buf[pos] != '\n') ++pos;
+                continue;
+            }
+            break;
+        }
+        t.clear();
+        while (pos < buf.size() && !std::isspace(buf[pos])) t.push_back((char)buf[pos++]);
+        return !t.empty();
+    };
+    std::string magic, sw, sh, sm;
+    if (!token(magic) || (magic != "P6" && magic != "P5") || !token(sw) || !token(sh) || !token(sm)) {
+        if (error) *error = "not a binary PPM/PGM file";
+        return false;
+    }
+    ++pos;
+    w = std::atoi(sw.c_str());
+    h = std::atoi(sh.c_str());
+    const int maxval = std::atoi(sm.c_str());
+    const int ch = magic == "P6" ? 3 : 1, bytes = maxval > 255 ? 2 : 1;
+    if (w <= 0 || h <= 0 || maxval <= 0 || maxval > 65535 || buf.size() < pos + (size_t)w * h * ch * bytes) {
+        if (error) *error = "truncated or invalid PPM/PGM";
+        return false;
+    }
+    rgb.assign((size_t)w * h * 3, 0.0f);
+    for (size_t i = 0; i < (size_t)w * h; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const size_t src = pos + (i * ch + (ch == 3 ? c : 0)) * bytes;
+            const int v = bytes == 2 ? ((int)buf[src] << 8) | buf[src + 1] : buf[src];
+            rgb[i * 3 + c] = (float)v / (float)maxval;
+        }
+    return true;
+}
+
+bool readImage(const std::string& file, std::vector<float>& rgb, int& w, int& h, std::string* error)
+{
+    if (ends_with(file, ".pfm")) return readImagePFM(file, rgb, w, h, error);
+    const bool hdr = ends_with(file, ".hdr") || ends_with(file, ".pic");
+    const bool pnm = ends_with(file, ".ppm") || ends_with(file, ".pgm");
+    if (!hdr && !pnm) {
+        if (error) *error = "unsupported image extension (use .pfm, .hdr, .pic, .ppm or .pgm): " + file;
+        return false;
+    }
+    std::vector<uint8_t> buf;
+    if (!read_file(file, buf)) {
+        if (error) *error = "cannot open " + file;
+        return false;
+    }
+    std::string err;
+    const bool ok = hdr ? read_hdr(buf, rgb, w, h, &err) : read_pnm(buf, rgb, w, h, &err);
+    if (!ok && error) *error = err + ": " + file;
+    return ok;
+}
+
 }  // namespace voxeltoy

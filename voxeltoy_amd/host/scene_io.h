@@ -150,4 +150,11 @@ bool writeImage(const std::string& file, const float* rgba, int width, int heigh
 bool readImagePFM(const std::string& file, std::vector<float>& rgb, int& width, int& height,
                   std::string* error = nullptr);
 
+/* loadImage (src/renderer/image.cpp:28-59) for the formats this library
+ * reads without OpenImageIO: float PFM, Radiance RGBE (.hdr, .pic; flat or
+ * run-length encoded) and binary PPM/PGM (8 or 16 bits, values / maxval).
+ * RGB floats, row 0 = the top row. */
+bool readImage(const std::string& file, std::vector<float>& rgb, int& width, int& height,
+               std::string* error = nullptr);
+
 }  // namespace voxeltoy

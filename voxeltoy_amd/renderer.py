@@ -81,6 +81,8 @@ SIGNATURES = {
     "vtr_request_action": (C.c_int, [_R, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32, C.c_int32]),
     "vtr_focal_distance": (C.c_float, [_R]),
     "vtr_selected_voxel": (C.c_int, [_R, api.i32p]),
+    "vtr_cpu_voxelize": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32,
+                                   C.c_void_p]),
     "vtr_load_obj": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p, C.c_int64,
                                C.POINTER(C.c_int64), C.c_void_p, C.c_void_p]),
     "vtr_get_material_data": (C.c_int, [_R, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
@@ -121,6 +123,20 @@ def load_obj(path: str):
                           C.byref(nt), bmin.ctypes.data, bmax.ctypes.data) != 0:
         raise RendererError(f"cannot load {path}")
     return v, t, bmin, bmax
+
+
+def cpu_voxelize(verts, tris, M, res, mode: int = 0) -> np.ndarray:
+    """CPUVoxelizer on the host threads: occupancy (res z, y, x) with 1 at the
+    covered voxels of the mesh placed by the row-major transform M."""
+    v = np.ascontiguousarray(verts, np.float32)
+    t = np.ascontiguousarray(tris, np.uint32)
+    m = np.ascontiguousarray(M, np.float32).reshape(-1)
+    r = np.asarray(res, np.int32)
+    occ = np.zeros((int(r[2]), int(r[1]), int(r[0])), np.uint8)
+    if lib().vtr_cpu_voxelize(v.ctypes.data, len(v), t.ctypes.data, len(t), m.ctypes.data, r.ctypes.data, mode,
+                              occ.ctypes.data) != 0:
+        raise RendererError("cpu_voxelize: bad arguments")
+    return occ
 
 
 class Renderer:
