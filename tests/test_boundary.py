@@ -103,3 +103,28 @@ def test_set_environment_rejects_bad_shapes(vtapi):
                       (np.zeros((4, 8, 3), np.float32), cu, cv[:, None])]:
         with pytest.raises(ValueError):
             ctx.set_environment(image=img, cdf_u=u, cdf_v=v, integral=1.0)
+
+
+def test_traversal_kernels_fit_the_sgpr_allocation_of_two_workgroups_per_cu(tmp_path):
+    """The traversal kernels run two 1024-thread workgroups per CU (8 waves
+    per SIMD) and are capped at 80 SGPRs (amdgpu_num_sgpr): a build whose
+    allocation rose to 96 ran 35 % slower although the compiler still
+    reported 8 waves.  Compile vt_trace.hip for gfx950 and check every
+    traversal kernel's SGPRs plus VCC, FLAT_SCRATCH and XNACK_MASK (6) fit
+    in 80, and its VGPRs in 64."""
+    from voxeltoy_amd import build as vb
+    out = str(tmp_path / "vt_trace.s")
+    src = os.path.join(vb.PKG, "csrc", "vt_trace.hip")
+    subprocess.run([vb.HIPCC, *vb.HIP_FLAGS, "--cuda-device-only", "-S", src, "-o", out], check=True,
+                   capture_output=True, text=True)
+    s = open(out).read()
+    kernels = re.findall(r"\.amdhsa_kernel (_ZN2vt18wf_traverse_kernel\S+)", s)
+    assert len(kernels) == 6
+    for name in kernels:
+        i = s.index(".amdhsa_kernel " + name)
+        block = s[i:s.index(".end_amdhsa_kernel", i)]
+        sgpr = int(re.search(r"\.amdhsa_next_free_sgpr (\d+)", block).group(1))
+        vgpr = int(re.search(r"\.amdhsa_next_free_vgpr (\d+)", block).group(1))
+        assert sgpr + 6 <= 80, (name, sgpr)
+        if "ELb1EEEv" not in name:              # batch kernels: per-lane grid views, fewer waves
+            assert vgpr <= 64, (name, vgpr)
