@@ -77,6 +77,54 @@ def test_batch_equals_single_scene_renders(graph):
         c.close()
 
 
+def long_scene(nx, seed):
+    """A 6 x 5 x nx grid seen down its length (floor, sparse boxes, emitters
+    at the far end), 48 x 12 pixels, 2 spp, 2 bounces."""
+    from voxeltoy_amd import api, scenes, workloads
+    nz, ny = 6, 5
+    rng = np.random.default_rng(seed)
+    occ = rng.random((nz, ny, nx)) < 0.001
+    occ[:, 0, :] = True
+    recs = [scenes.lambert((0, 0, 0), (0.6, 0.6, 0.6)), scenes.lambert((6.0, 5.0, 4.0), (0.5, 0.5, 0.5))]
+    mat, offs = scenes.build_materials(recs)
+    id_offset = np.full(256, -1, np.int32)
+    id_offset[1:3] = offs
+    ids = np.where(occ, np.uint8(1), np.uint8(0))
+    ids[:, 1:3, nx - 8:] = 2
+    sc = scenes.Scene(ids, id_offset, mat)
+    sc.emissive = api.prune_emissive(sc.offsets(), scenes.emissive_list(ids, id_offset, mat))
+    bmin, bmax = workloads.reference_bounds(sc.shape)
+    eye = np.array([bmin[0] - 20.0, 0.8 * bmax[1], 0.1 * bmax[2]], np.float32)
+    cam = scenes.CameraSetup(eye, np.array([bmax[0], 0.3 * bmax[1], 0.0], np.float32), 0.06)
+    return workloads.Workload(f"long{nx}", sc, cam, 48, 12, spp=2, bounces=2, seed=seed, bmin=bmin, bmax=bmax,
+                              env_top=(1.2, 1.2, 1.3), env_bottom=(0.2, 0.2, 0.2))
+
+
+@pytest.mark.parametrize("nx", [1016, 1017])
+def test_batch_with_a_long_grid_equals_single_renders(nx):
+    """The merged launch marches with packed coordinates only when every
+    merged scene's extents are at most 1016: a 1016-long grid keeps that
+    loop, a 1017-long one switches the whole merged launch to the unpacked
+    loop.  Either way each scene equals its own render."""
+    from voxeltoy_amd import api, workloads
+    wls = mixed_scenes()[:3] + [long_scene(nx, 41)]
+    seeds = [wl.seed for wl in wls]
+    n0 = [0] * len(wls)
+    ref = singles(api, workloads, wls, seeds, n0)
+    ctxs = []
+    for wl in wls:
+        c = api.Context(0)
+        workloads.load(c, wl)
+        ctxs.append(c)
+    b = api.Batch(ctxs)
+    b.render(seeds, [wl.spp for wl in wls], n0=n0, graph=True)
+    for i, c in enumerate(ctxs):
+        assert np.array_equal(c.read_accum().view(np.uint32), ref[i].view(np.uint32)), f"scene {i} differs"
+    b.close()
+    for c in ctxs:
+        c.close()
+
+
 def test_batch_chunked_and_stats():
     """Path capacity smaller than the batch: chunks cut across scenes."""
     from voxeltoy_amd import api, workloads

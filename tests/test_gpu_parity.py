@@ -281,12 +281,16 @@ def test_render_large_grid_bit_exact(ctx, orc):
         assert stats[k] == cnt[k], k
 
 
-def test_render_grid_of_maximum_extent_bit_exact(ctx, orc):
-    """A 4096-voxel-long grid (the largest extent the volume accepts): hit
-    records, light targets and queue entries carry coordinates up to 4096
-    (the first voxel past the grid); emissive voxels sit near the far end."""
+@pytest.mark.parametrize("nx", [1016, 1017, 4096])
+def test_render_grid_of_maximum_extent_bit_exact(ctx, orc, nx):
+    """Long grids: 1016 voxels (the largest extent the packed-coordinate
+    loop takes, its 10-bit fields reaching 1020 at the first voxel past the
+    grid), 1017 (the smallest that takes the unpacked loop) and 4096 (the
+    largest extent the volume accepts).  Hit records, light targets and
+    queue entries carry coordinates up to nx (the first voxel past the
+    grid); emissive voxels sit near the far end."""
     from voxeltoy_amd import api, scenes, workloads
-    nz, ny, nx = 6, 5, 4096
+    nz, ny = 6, 5
     rng = np.random.default_rng(33)
     occ = rng.random((nz, ny, nx)) < 0.0005
     occ[:, 0, :] = True                                      # a floor the whole length
@@ -303,7 +307,7 @@ def test_render_grid_of_maximum_extent_bit_exact(ctx, orc):
     assert len(sc.emissive) > 0 and (sc.emissive % nx).max() >= nx - 8
     bmin, bmax = workloads.reference_bounds(sc.shape)
     # looking down the length from just before the near end: rays march up
-    # to the full 4096 voxels and leave through the far end
+    # to the full length and leave through the far end
     eye = np.array([bmin[0] - 20.0, 0.8 * bmax[1], 0.1 * bmax[2]], np.float32)
     cam = scenes.CameraSetup(eye, np.array([bmax[0], 0.3 * bmax[1], 0.0], np.float32), 0.06)
     wl = workloads.Workload("long", sc, cam, 96, 24, spp=2, bounces=3, bmin=bmin, bmax=bmax,
