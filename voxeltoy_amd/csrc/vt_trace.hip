@@ -791,8 +791,48 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
     const uint32_t n = PHASE == PH_PRIMARY ? P.cnt[sel] : P.cnt[sel] / 2u;
     if (blockIdx.x == 0 && threadIdx.x == 0) P.cnt[2 + (sel ^ 1)] = 0u;
     Counters cnt = { 0, 0, 0, 0, 0, 0 };
+    /* Shadow and continuation rounds: each batch of a workgroup's paths is
+     * sorted so that the paths whose continuation ray left the grid (the sky
+     * branch) come after those that hit a voxel or the ground (the shading
+     * branch).  A wave then runs one of the two branches with most of its
+     * lanes instead of both with some of them; in the first bounce about
+     * half the continuation rays leave the grid.  Every path is shaded from
+     * its own state and hit, so the results do not change, only the order
+     * of the appended rays. */
+    constexpr bool SORT = !CACHE && PHASE != PH_PRIMARY;
+    constexpr uint32_t NW = WF_SHADE_THREADS / 64;
+    __shared__ uint32_t s_cls[NW][3];
+    __shared__ uint32_t s_src[SORT ? WF_SHADE_THREADS : 1];
+    __shared__ uint4 s_hit[SORT ? WF_SHADE_THREADS : 1];
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t j = base + threadIdx.x;
+        uint32_t j = base + threadIdx.x;
+        uint4 hraw = make_uint4(0u, 0u, 0u, 0u);
+        if (SORT) {
+            uint32_t cls = 2u;                              /* no path */
+            if (j < n) {
+                hraw = ((const uint4*)P.hit)[j];
+                Hit hh;
+                cls = unpack_hit(make_uint2(hraw.z, hraw.w), hh) == KIND_MISS ? 1u : 0u;
+            }
+            /* counting sort over the workgroup: class, then wave, then lane */
+            const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+            uint32_t rank = 0u;
+            for (uint32_t k = 0; k < 3u; ++k) {
+                const unsigned long long m = __ballot(cls == k);
+                if (lane == 0) s_cls[wv][k] = (uint32_t)__popcll(m);
+                if (cls == k) rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            }
+            __syncthreads();
+            uint32_t pos = rank;
+            for (uint32_t k = 0; k < 3u; ++k)
+                for (uint32_t w = 0; w < NW; ++w)
+                    if (k < cls || (k == cls && w < wv)) pos += s_cls[w][k];
+            s_src[pos] = threadIdx.x;
+            s_hit[pos] = hraw;
+            __syncthreads();
+            j = base + s_src[threadIdx.x];
+            hraw = s_hit[threadIdx.x];
+        }
         bool keep = false, query = false;
         uint32_t sid = 0;
         f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, rn = ro;
@@ -812,7 +852,7 @@ __global__ __launch_bounds__(WF_SHADE_THREADS) __attribute__((amdgpu_waves_per_e
             /* the shadow and continuation results of a path are adjacent: one 16-byte load */
             uint2 hv;
             if (PHASE != PH_PRIMARY) {
-                const uint4 hp = ((const uint4*)P.hit)[j];
+                const uint4 hp = SORT ? hraw : ((const uint4*)P.hit)[j];
                 hv = make_uint2(hp.z, hp.w);
                 /* the shadow ray's result (never ends the path): it picks the
                  * radiance with or without the light sample (store_cont) */
