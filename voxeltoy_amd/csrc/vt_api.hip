@@ -63,6 +63,17 @@ void dfree(T*& p)
     p = nullptr;
 }
 
+/* A call's temporary device buffer, freed when the call returns, the early
+ * error returns of VT_HIP included. */
+template <typename T>
+struct DevTmp {
+    T* p = nullptr;
+    DevTmp() = default;
+    DevTmp(const DevTmp&) = delete;
+    DevTmp& operator=(const DevTmp&) = delete;
+    ~DevTmp() { dfree(p); }
+};
+
 }  // namespace
 
 /* ---- staging memory of vt_trace_rays_noalloc -------------------------------------
@@ -553,20 +564,19 @@ int vt_set_volume(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const int32
     c->n_slots = ns;
     r = check_emissive(emissive, n_emissive, nvox, occ);
     if (r != VT_OK) return r;
-    int32_t* d_dense = nullptr;
-    uint8_t* d_lut = nullptr;
-    uint32_t* d_bad = nullptr;
-    VT_HIP(hipMalloc(&d_dense, (size_t)nvox * 4u));
-    VT_HIP(hipMalloc(&d_lut, lut.size()));
-    VT_HIP(hipMalloc(&d_bad, 4));
-    VT_HIP(hipMemcpyAsync(d_dense, offsets, (size_t)nvox * 4u, hipMemcpyHostToDevice, c->stream));
-    VT_HIP(hipMemcpyAsync(d_lut, lut.data(), lut.size(), hipMemcpyHostToDevice, c->stream));
-    VT_HIP(vt::launch_fill_u32(d_bad, 1, 0u, c->stream));
-    VT_HIP(vt::launch_brick_from_offsets(c->dims, d_dense, d_lut, mat_size, c->d_bricks, d_bad, c->stream));
+    DevTmp<int32_t> d_dense;
+    DevTmp<uint8_t> d_lut;
+    DevTmp<uint32_t> d_bad;
+    VT_HIP(hipMalloc(&d_dense.p, (size_t)nvox * 4u));
+    VT_HIP(hipMalloc(&d_lut.p, lut.size()));
+    VT_HIP(hipMalloc(&d_bad.p, 4));
+    VT_HIP(hipMemcpyAsync(d_dense.p, offsets, (size_t)nvox * 4u, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipMemcpyAsync(d_lut.p, lut.data(), lut.size(), hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_fill_u32(d_bad.p, 1, 0u, c->stream));
+    VT_HIP(vt::launch_brick_from_offsets(c->dims, d_dense.p, d_lut.p, mat_size, c->d_bricks, d_bad.p, c->stream));
     uint32_t bad = 0;
-    VT_HIP(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, c->stream));
+    VT_HIP(hipMemcpyAsync(&bad, d_bad.p, 4, hipMemcpyDeviceToHost, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
-    dfree(d_dense); dfree(d_lut); dfree(d_bad);
     if (bad) return fail(VT_ERR_ARG, "grid conversion found an unmapped material offset");
     return finish_volume(c, emissive, n_emissive, occ, true);
 }
@@ -599,15 +609,14 @@ int vt_set_volume_u8(vt_context* c, int32_t nx, int32_t ny, int32_t nz, const ui
     }
     r = check_emissive(emissive, n_emissive, nvox, occ);
     if (r != VT_OK) return r;
-    uint8_t* d_dense = nullptr;
-    uint8_t* d_remap = nullptr;
-    VT_HIP(hipMalloc(&d_dense, (size_t)nvox));
-    VT_HIP(hipMalloc(&d_remap, 256));
-    VT_HIP(hipMemcpyAsync(d_dense, ids, (size_t)nvox, hipMemcpyHostToDevice, c->stream));
-    VT_HIP(hipMemcpyAsync(d_remap, remap, 256, hipMemcpyHostToDevice, c->stream));
-    VT_HIP(vt::launch_brick_from_ids(c->dims, d_dense, d_remap, c->d_bricks, c->stream));
+    DevTmp<uint8_t> d_dense;
+    DevTmp<uint8_t> d_remap;
+    VT_HIP(hipMalloc(&d_dense.p, (size_t)nvox));
+    VT_HIP(hipMalloc(&d_remap.p, 256));
+    VT_HIP(hipMemcpyAsync(d_dense.p, ids, (size_t)nvox, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipMemcpyAsync(d_remap.p, remap, 256, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_brick_from_ids(c->dims, d_dense.p, d_remap.p, c->d_bricks, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
-    dfree(d_dense); dfree(d_remap);
     return finish_volume(c, emissive, n_emissive, occ, false);
 }
 
@@ -631,15 +640,14 @@ int vt_read_volume(vt_context* c, int32_t* offsets)
     int32_t table[256];
     table[0] = -1;
     for (int s = 1; s < 256; ++s) table[s] = c->slot_off[s];
-    int32_t* d_dense = nullptr;
-    int32_t* d_table = nullptr;
-    VT_HIP(hipMalloc(&d_dense, (size_t)nvox * 4u));
-    VT_HIP(hipMalloc(&d_table, sizeof(table)));
-    VT_HIP(hipMemcpyAsync(d_table, table, sizeof(table), hipMemcpyHostToDevice, c->stream));
-    VT_HIP(vt::launch_dense_from_bricks(c->dims, c->d_bricks, d_table, d_dense, c->stream));
-    VT_HIP(hipMemcpyAsync(offsets, d_dense, (size_t)nvox * 4u, hipMemcpyDeviceToHost, c->stream));
+    DevTmp<int32_t> d_dense;
+    DevTmp<int32_t> d_table;
+    VT_HIP(hipMalloc(&d_dense.p, (size_t)nvox * 4u));
+    VT_HIP(hipMalloc(&d_table.p, sizeof(table)));
+    VT_HIP(hipMemcpyAsync(d_table.p, table, sizeof(table), hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_dense_from_bricks(c->dims, c->d_bricks, d_table.p, d_dense.p, c->stream));
+    VT_HIP(hipMemcpyAsync(offsets, d_dense.p, (size_t)nvox * 4u, hipMemcpyDeviceToHost, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
-    dfree(d_dense); dfree(d_table);
     return VT_OK;
 }
 
@@ -1741,20 +1749,18 @@ int vt_autotune(vt_context* c, uint32_t passes, vt_tuning* out)
     VT_HIP(hipSetDevice(c->device));
     const size_t acc_bytes = (size_t)c->W * c->H * sizeof(float4);
     const size_t sum_bytes = c->d_sum64 ? (size_t)c->W * c->H * 3 * sizeof(double) : 0;
-    void* save_acc = nullptr;
-    void* save_sum = nullptr;
+    DevTmp<void> save_acc;
+    DevTmp<void> save_sum;
     {
-        hipError_t he = hipMalloc(&save_acc, acc_bytes);
-        if (he == hipSuccess && sum_bytes) he = hipMalloc(&save_sum, sum_bytes);
+        hipError_t he = hipMalloc(&save_acc.p, acc_bytes);
+        if (he == hipSuccess && sum_bytes) he = hipMalloc(&save_sum.p, sum_bytes);
         if (he == hipSuccess)
-            he = hipMemcpyAsync(save_acc, c->d_accum, acc_bytes, hipMemcpyDeviceToDevice, c->stream);
+            he = hipMemcpyAsync(save_acc.p, c->d_accum, acc_bytes, hipMemcpyDeviceToDevice, c->stream);
         if (he == hipSuccess && sum_bytes)
-            he = hipMemcpyAsync(save_sum, c->d_sum64, sum_bytes, hipMemcpyDeviceToDevice, c->stream);
+            he = hipMemcpyAsync(save_sum.p, c->d_sum64, sum_bytes, hipMemcpyDeviceToDevice, c->stream);
         if (he != hipSuccess) {
             (void)hipGetLastError();
             (void)hipStreamSynchronize(c->stream);
-            dfree(save_acc);
-            dfree(save_sum);
             return fail(VT_ERR_HIP, std::string("autotune: saving the accumulation failed: ") + hipGetErrorString(he));
         }
     }
@@ -1788,13 +1794,19 @@ int vt_autotune(vt_context* c, uint32_t passes, vt_tuning* out)
             pick = k;
         }
     }
-    VT_HIP(hipMemcpyAsync(c->d_accum, save_acc, acc_bytes, hipMemcpyDeviceToDevice, c->stream));
-    if (sum_bytes) VT_HIP(hipMemcpyAsync(c->d_sum64, save_sum, sum_bytes, hipMemcpyDeviceToDevice, c->stream));
-    VT_HIP(hipStreamSynchronize(c->stream));
-    dfree(save_acc);
-    dfree(save_sum);
+    /* the accumulation and the context's settings are restored before any
+     * error is reported */
+    hipError_t he = hipMemcpyAsync(c->d_accum, save_acc.p, acc_bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (he == hipSuccess && sum_bytes)
+        he = hipMemcpyAsync(c->d_sum64, save_sum.p, sum_bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
     c->stats_enabled = was_stats;
     c->timed = false;
+    if (he != hipSuccess) {
+        (void)hipGetLastError();
+        c->render_mode = mode0; c->dda_budget = b0; c->refill_min = r0; c->refill_chunk = k0;
+        return fail(VT_ERR_HIP, std::string("autotune: restoring the accumulation failed: ") + hipGetErrorString(he));
+    }
     if (rc != VT_OK) {
         c->render_mode = mode0; c->dda_budget = b0; c->refill_min = r0; c->refill_chunk = k0;
         return rc;
@@ -1897,18 +1909,17 @@ int vt_trace_rays(vt_context* c, int32_t n, const float* o, const float* dir, in
     if (n == 0) return VT_OK;
     VT_HIP(hipSetDevice(c->device));
     vt::TraceArgs a = trace_args(c);
-    float* d_o = nullptr;
-    float* d_d = nullptr;
-    int32_t* d_r = nullptr;
-    VT_HIP(hipMalloc(&d_o, (size_t)n * 12));
-    VT_HIP(hipMalloc(&d_d, (size_t)n * 12));
-    VT_HIP(hipMalloc(&d_r, (size_t)n * 32));
-    VT_HIP(hipMemcpyAsync(d_o, o, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
-    VT_HIP(hipMemcpyAsync(d_d, dir, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
-    VT_HIP(vt::launch_trace_rays(a, n, d_o, d_d, d_r, c->stream));
-    VT_HIP(hipMemcpyAsync(rec, d_r, (size_t)n * 32, hipMemcpyDeviceToHost, c->stream));
+    DevTmp<float> d_o;
+    DevTmp<float> d_d;
+    DevTmp<int32_t> d_r;
+    VT_HIP(hipMalloc(&d_o.p, (size_t)n * 12));
+    VT_HIP(hipMalloc(&d_d.p, (size_t)n * 12));
+    VT_HIP(hipMalloc(&d_r.p, (size_t)n * 32));
+    VT_HIP(hipMemcpyAsync(d_o.p, o, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipMemcpyAsync(d_d.p, dir, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_trace_rays(a, n, d_o.p, d_d.p, d_r.p, c->stream));
+    VT_HIP(hipMemcpyAsync(rec, d_r.p, (size_t)n * 32, hipMemcpyDeviceToHost, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
-    dfree(d_o); dfree(d_d); dfree(d_r);
     return VT_OK;
 }
 
@@ -1968,28 +1979,28 @@ int vt_voxelize(vt_context* c, const float* verts, int64_t nv, const uint32_t* i
     a.mode = mode;
     a.slot = (uint8_t)slot;
     a.bricks = c->d_bricks;
-    float* d_v = nullptr;
-    uint32_t* d_i = nullptr;
-    uint8_t* d_occ = nullptr;
+    DevTmp<float> d_v;
+    DevTmp<uint32_t> d_i;
+    DevTmp<uint8_t> d_occ;
     const int64_t nvox = (int64_t)c->dims.nx * c->dims.ny * c->dims.nz;
     if (ntri > 0) {
-        VT_HIP(hipMalloc(&d_v, (size_t)nv * 12));
-        VT_HIP(hipMalloc(&d_i, (size_t)ntri * 12));
-        VT_HIP(hipMemcpyAsync(d_v, verts, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream));
-        VT_HIP(hipMemcpyAsync(d_i, idx, (size_t)ntri * 12, hipMemcpyHostToDevice, c->stream));
+        VT_HIP(hipMalloc(&d_v.p, (size_t)nv * 12));
+        VT_HIP(hipMalloc(&d_i.p, (size_t)ntri * 12));
+        VT_HIP(hipMemcpyAsync(d_v.p, verts, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream));
+        VT_HIP(hipMemcpyAsync(d_i.p, idx, (size_t)ntri * 12, hipMemcpyHostToDevice, c->stream));
     }
     if (occupancy) {
-        VT_HIP(hipMalloc(&d_occ, (size_t)nvox));
-        VT_HIP(vt::launch_fill_u8(d_occ, (uint64_t)nvox, 0, c->stream));
+        VT_HIP(hipMalloc(&d_occ.p, (size_t)nvox));
+        VT_HIP(vt::launch_fill_u8(d_occ.p, (uint64_t)nvox, 0, c->stream));
     }
-    a.verts = d_v; a.idx = d_i; a.occ = d_occ;
+    a.verts = d_v.p; a.idx = d_i.p; a.occ = d_occ.p;
     /* the list of large triangles (a wave each) and its counter */
-    uint32_t* d_big = nullptr;
+    DevTmp<uint32_t> d_big;
     if (ntri > 0) {
-        VT_HIP(hipMalloc(&d_big, ((size_t)ntri + 1) * 4));
-        VT_HIP(vt::launch_fill_u32(d_big + ntri, 1, 0u, c->stream));
-        a.big = d_big;
-        a.big_count = d_big + ntri;
+        VT_HIP(hipMalloc(&d_big.p, ((size_t)ntri + 1) * 4));
+        VT_HIP(vt::launch_fill_u32(d_big.p + ntri, 1, 0u, c->stream));
+        a.big = d_big.p;
+        a.big_count = d_big.p + ntri;
         a.big_cap = (uint32_t)std::min<int64_t>(ntri, 0x7fffffff);   /* beyond it, triangles keep their thread */
         a.big_area = VT_VOX_BIG_AREA;
     }
@@ -2002,9 +2013,8 @@ int vt_voxelize(vt_context* c, const float* verts, int64_t nv, const uint32_t* i
         int r = refresh_occupancy(c);
         if (r != VT_OK) return r;
     }
-    if (occupancy) VT_HIP(hipMemcpyAsync(occupancy, d_occ, (size_t)nvox, hipMemcpyDeviceToHost, c->stream));
+    if (occupancy) VT_HIP(hipMemcpyAsync(occupancy, d_occ.p, (size_t)nvox, hipMemcpyDeviceToHost, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
-    dfree(d_v); dfree(d_i); dfree(d_occ); dfree(d_big);
     return VT_OK;
 }
 
