@@ -718,8 +718,14 @@ int vt_set_environment(vt_context* c, const vt_environment* env)
         }
     }
     /* earlier renders may still read the old tables */
-    VT_HIP(hipStreamSynchronize(c->stream));
-    if (c->stream2) VT_HIP(hipStreamSynchronize(c->stream2));
+    hipError_t hs = hipStreamSynchronize(c->stream);
+    if (hs == hipSuccess && c->stream2) hs = hipStreamSynchronize(c->stream2);
+    if (hs != hipSuccess) {
+        (void)hipGetLastError();
+        dfree(img); dfree(cdfu); dfree(cdfv); dfree(quad);
+        return fail(VT_ERR_HIP, std::string("environment upload: waiting for earlier renders failed: ") +
+                                    hipGetErrorString(hs));
+    }
     dfree(c->d_img); dfree(c->d_quad); dfree(c->d_cdfu); dfree(c->d_cdfv);
     c->d_img = img; c->d_quad = quad; c->d_cdfu = cdfu; c->d_cdfv = cdfv;
     e.image_rgb = nullptr; e.cdf_u = nullptr; e.cdf_v = nullptr;
@@ -1893,12 +1899,11 @@ int vt_trace_first_hit(vt_context* c, uint64_t seed, uint32_t frame, int32_t* re
     vt::TraceArgs a = trace_args(c);
     a.seed = seed; a.n0 = frame;
     const size_t bytes = (size_t)c->W * c->H * 8 * 4;
-    int32_t* d = nullptr;
-    VT_HIP(hipMalloc(&d, bytes));
-    VT_HIP(vt::launch_first_hit(a, d, c->stream));
-    VT_HIP(hipMemcpyAsync(rec, d, bytes, hipMemcpyDeviceToHost, c->stream));
+    DevTmp<int32_t> d;
+    VT_HIP(hipMalloc(&d.p, bytes));
+    VT_HIP(vt::launch_first_hit(a, d.p, c->stream));
+    VT_HIP(hipMemcpyAsync(rec, d.p, bytes, hipMemcpyDeviceToHost, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
-    dfree(d);
     return VT_OK;
 }
 
@@ -2064,16 +2069,15 @@ int vt_device_math(vt_context* c, int32_t fn, int32_t n, const float* a, const f
     if (!c || n < 0 || (n > 0 && (!a || !b || !out))) return fail(VT_ERR_ARG, "bad argument");
     if (n == 0) return VT_OK;
     VT_HIP(hipSetDevice(c->device));
-    float *da = nullptr, *db = nullptr, *dout = nullptr;
-    VT_HIP(hipMalloc(&da, (size_t)n * 4));
-    VT_HIP(hipMalloc(&db, (size_t)n * 4));
-    VT_HIP(hipMalloc(&dout, (size_t)n * 4));
-    VT_HIP(hipMemcpyAsync(da, a, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-    VT_HIP(hipMemcpyAsync(db, b, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-    VT_HIP(vt::launch_device_math(fn, n, da, db, dout, c->stream));
-    VT_HIP(hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    DevTmp<float> da, db, dout;
+    VT_HIP(hipMalloc(&da.p, (size_t)n * 4));
+    VT_HIP(hipMalloc(&db.p, (size_t)n * 4));
+    VT_HIP(hipMalloc(&dout.p, (size_t)n * 4));
+    VT_HIP(hipMemcpyAsync(da.p, a, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(hipMemcpyAsync(db.p, b, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    VT_HIP(vt::launch_device_math(fn, n, da.p, db.p, dout.p, c->stream));
+    VT_HIP(hipMemcpyAsync(out, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
     VT_HIP(hipStreamSynchronize(c->stream));
-    dfree(da); dfree(db); dfree(dout);
     return VT_OK;
 }
 
@@ -2081,20 +2085,19 @@ int vt_copy_probe(vt_context* c, uint64_t bytes, int32_t iters, double* kernel_m
 {
     if (!c || bytes < 16 || bytes % 16 || iters < 1 || !kernel_ms) return fail(VT_ERR_ARG, "bad argument");
     VT_HIP(hipSetDevice(c->device));
-    uint8_t *a = nullptr, *b = nullptr;
-    VT_HIP(hipMalloc(&a, bytes));
-    VT_HIP(hipMalloc(&b, bytes));
-    VT_HIP(vt::launch_fill_u8(a, bytes, 1, c->stream));
-    VT_HIP(vt::launch_copy(a, b, bytes, c->stream));
+    DevTmp<uint8_t> a, b;
+    VT_HIP(hipMalloc(&a.p, bytes));
+    VT_HIP(hipMalloc(&b.p, bytes));
+    VT_HIP(vt::launch_fill_u8(a.p, bytes, 1, c->stream));
+    VT_HIP(vt::launch_copy(a.p, b.p, bytes, c->stream));
     VT_HIP(hipEventRecord(c->ev0, c->stream));
-    for (int32_t i = 0; i < iters; ++i) VT_HIP(vt::launch_copy(a, b, bytes, c->stream));
+    for (int32_t i = 0; i < iters; ++i) VT_HIP(vt::launch_copy(a.p, b.p, bytes, c->stream));
     VT_HIP(hipEventRecord(c->ev1, c->stream));
     VT_HIP(hipEventSynchronize(c->ev1));
     float ms = 0.0f;
     VT_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->timed = false;
     *kernel_ms = ms / iters;
-    dfree(a); dfree(b);
     return VT_OK;
 }
 
