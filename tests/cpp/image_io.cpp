@@ -115,6 +115,11 @@ int main(int argc, char** argv)
         CHECK(!readImage(dir + "/xyz.hdr", img, w, h));
         put(dir + "/short.ppm", text("P6 4 4 255\n\x01\x02"));
         CHECK(!readImage(dir + "/short.ppm", img, w, h));
+        // a header asking for 65535 x 65535 pixels of a few-byte file: refused
+        // before any pixel buffer is sized from it
+        put(dir + "/huge.hdr", text("#?RADIANCE\n\n-Y 65535 +X 65535\n\x02\x02\xff\xff"));
+        err.clear();
+        CHECK(!readImage(dir + "/huge.hdr", img, w, h, &err) && err.find("truncated") != std::string::npos);
     }
     if (fails) return 1;
     std::printf("OK\n");

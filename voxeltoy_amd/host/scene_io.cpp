@@ -528,6 +528,14 @@ static bool read_hdr(const std::vector<uint8_t>& buf, std::vector<float>& rgb, i
         return false;
     }
     const bool top_down = std::strcmp(ys, "-Y") == 0;
+    /* the shortest scanline a file can hold (run-length: a 4-byte header and
+     * at least one 2-byte run per 127 pixels of each channel; flat: 4 bytes
+     * per pixel), so a header cannot ask for more pixels than the file has */
+    const size_t min_row = (w >= 8 && w < 32768) ? 4u + 8u * (size_t)((w + 126) / 127) : (size_t)w * 4u;
+    if (buf.size() - pos < (size_t)h * min_row) {
+        if (error) *error = "truncated HDR pixels";
+        return false;
+    }
     rgb.assign((size_t)w * h * 3, 0.0f);
     std::vector<uint8_t> row((size_t)w * 4);
     for (int y = 0; y < h; ++y) {
