@@ -198,6 +198,11 @@ def test_vox_errors_are_reported(tmp_path):
     (tmp_path / "v.vox").write_bytes(b"VOX " + struct.pack("<i", 151) + b"\0" * 12)
     with pytest.raises(vr.RendererError, match="version"):
         r.load_vox_file(str(tmp_path / "v.vox"))
+    # a SIZE chunk beyond what the volume takes is refused before any grid is sized from it
+    for size in ((5000, 4, 4), (4096, 4096, 4096)):
+        write_vox(tmp_path / "big.vox", size, [(0, 0, 0, 1)])
+        with pytest.raises(vr.RendererError, match="SIZE"):
+            r.load_vox_file(str(tmp_path / "big.vox"))
     # a failed load leaves the previous volume in place
     off, _, _ = r.volume()
     assert off.shape == (16, 16, 16)

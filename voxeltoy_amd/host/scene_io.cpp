@@ -257,6 +257,12 @@ bool MagicaVoxelLoader::load(const std::string& filePath, std::vector<int32_t>& 
     }
     if (!rd.ok) { m_error = "truncated file"; return false; }
     if (sx <= 0 || sy <= 0 || sz <= 0) { m_error = "missing or invalid SIZE chunk"; return false; }
+    /* what the volume takes: at most 4096 voxels per axis, and linear voxel
+     * indices (the emissive list) that fit in 31 bits */
+    if (sx > 4096 || sy > 4096 || sz > 4096 || (uint64_t)sx * (uint64_t)sy * (uint64_t)sz >= (1ull << 31)) {
+        m_error = "SIZE chunk beyond the volume's limits";
+        return false;
+    }
     if (!custom) defaultPalette(palette);
 
     /* axis conversion: grid (x, y, z) = file (x, z, y) */
