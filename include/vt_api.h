@@ -29,6 +29,10 @@
 extern "C" {
 #endif
 
+/* Status codes; vt_last_error() holds the message of the calling thread's
+ * last failure.  No C++ exception leaves the library: a host allocation that
+ * cannot be made returns VT_ERR_LIMIT ("out of memory"), any other internal
+ * exception VT_ERR_STATE. */
 #define VT_OK 0
 #define VT_ERR_ARG -1
 #define VT_ERR_HIP -2

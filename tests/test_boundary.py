@@ -128,3 +128,25 @@ def test_traversal_kernels_fit_the_sgpr_allocation_of_two_workgroups_per_cu(tmp_
         assert sgpr + 6 <= 80, (name, sgpr)
         if "ELb1EEEv" not in name:              # batch kernels: per-lane grid views, fewer waves
             assert vgpr <= 64, (name, vgpr)
+
+
+def test_no_exception_leaves_the_c_abi(vtapi):
+    """A host allocation sized from the caller's input that cannot be made
+    returns VT_ERR_LIMIT instead of throwing through the C ABI: in a child
+    process whose address space is capped at 8 GiB, vtr_cpu_voxelize is asked
+    to copy 2^34 vertices (192 GiB)."""
+    import sys
+    code = f"""
+import ctypes as C, resource, numpy as np
+lib = C.CDLL({vtapi.library_path()!r})
+resource.setrlimit(resource.RLIMIT_AS, (8 << 30, 8 << 30))
+f = lib.vtr_cpu_voxelize
+f.restype = C.c_int
+f.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+v = np.zeros(3, np.float32); m = np.eye(4, dtype=np.float32); res = np.array([4, 4, 4], np.int32)
+occ = np.zeros(64, np.uint8)
+print(f(v.ctypes.data, 1 << 34, None, 0, m.ctypes.data, res.ctypes.data, 0, occ.ctypes.data))
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.strip() == "-4"

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <thread>
 #include <vector>
+#include <new>
 
 namespace voxeltoy {
 namespace {
@@ -176,17 +177,25 @@ void CPUVoxelizer::voxelizeMesh(const Mesh* mesh, const M44f& M, const V3i& reso
 int vtr_cpu_voxelize(const float* vertices, int64_t n_vertices, const uint32_t* indices, int64_t n_triangles,
                      const float* model_transform, const int32_t* res, int32_t mode, uint8_t* occupancy)
 {
-    using namespace voxeltoy;
-    if (!model_transform || !res || !occupancy || n_vertices < 0 || n_triangles < 0 || n_triangles > 0xffffffffLL ||
-        (n_vertices > 0 && !vertices) || (n_triangles > 0 && !indices) || (mode != 0 && mode != 1) || res[0] <= 0 ||
-        res[1] <= 0 || res[2] <= 0)
-        return VT_ERR_ARG;
-    for (int64_t i = 0; i < 3 * n_triangles; ++i)
-        if ((int64_t)indices[i] >= n_vertices) return VT_ERR_ARG;
-    Mesh mesh(std::vector<float>(vertices, vertices + 3 * n_vertices),
-              std::vector<uint32_t>(indices, indices + 3 * n_triangles));
-    M44f M;
-    std::memcpy(M.data(), model_transform, sizeof(float) * 16);
-    CPUVoxelizer::voxelizeMesh(&mesh, M, V3i(res[0], res[1], res[2]), occupancy, mode);
-    return VT_OK;
+    /* no C++ exception leaves the C ABI: a failed allocation or thread
+     * start becomes an error code */
+    try {
+        using namespace voxeltoy;
+        if (!model_transform || !res || !occupancy || n_vertices < 0 || n_triangles < 0 || n_triangles > 0xffffffffLL ||
+            (n_vertices > 0 && !vertices) || (n_triangles > 0 && !indices) || (mode != 0 && mode != 1) || res[0] <= 0 ||
+            res[1] <= 0 || res[2] <= 0)
+            return VT_ERR_ARG;
+        for (int64_t i = 0; i < 3 * n_triangles; ++i)
+            if ((int64_t)indices[i] >= n_vertices) return VT_ERR_ARG;
+        Mesh mesh(std::vector<float>(vertices, vertices + 3 * n_vertices),
+                  std::vector<uint32_t>(indices, indices + 3 * n_triangles));
+        M44f M;
+        std::memcpy(M.data(), model_transform, sizeof(float) * 16);
+        CPUVoxelizer::voxelizeMesh(&mesh, M, V3i(res[0], res[1], res[2]), occupancy, mode);
+        return VT_OK;
+    } catch (const std::bad_alloc&) {
+        return VT_ERR_LIMIT;
+    } catch (...) {
+        return VT_ERR_STATE;
+    }
 }

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <sstream>
 
 namespace voxeltoy {
@@ -692,13 +693,30 @@ static int guarded(vtr_renderer* r, int err, F&& f)
     return r->r.failed() ? err : VT_OK;
 }
 
+/* No C++ exception leaves the C ABI: an entry point's body runs in no_throw,
+ * and a failed host allocation (containers sized from files or the caller's
+ * input) or any other exception becomes an error code. */
+template <typename F>
+static int no_throw(F&& f) noexcept
+{
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return VT_ERR_LIMIT;
+    } catch (...) {
+        return VT_ERR_STATE;
+    }
+}
+
 extern "C" {
 
 int vtr_create(int device, vtr_renderer** out)
 {
-    if (!out) return VT_ERR_ARG;
-    *out = new vtr_renderer(device);
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!out) return VT_ERR_ARG;
+        *out = new vtr_renderer(device);
+        return VT_OK;
+    });
 }
 
 void vtr_destroy(vtr_renderer* r) { delete r; }
@@ -708,170 +726,202 @@ vt_context* vtr_context(vtr_renderer* r) { return r ? r->r.context() : nullptr; 
 
 int vtr_initialize(vtr_renderer* r, const char* shader_path)
 {
-    return guarded(r, VT_ERR_STATE, [&] { r->r.initialize(shader_path ? shader_path : ""); });
+    return no_throw([&]() -> int {
+        return guarded(r, VT_ERR_STATE, [&] { r->r.initialize(shader_path ? shader_path : ""); });
+    });
 }
 
 int vtr_resize_frame(vtr_renderer* r, int32_t w, int32_t h, int32_t vx, int32_t vy, int32_t vw, int32_t vh)
 {
-    if (!r || w <= 0 || h <= 0 || w > 16384 || h > 16384) return VT_ERR_ARG;
-    r->r.resizeFrame(w, h, vx, vy, vw, vh);
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r || w <= 0 || h <= 0 || w > 16384 || h > 16384) return VT_ERR_ARG;
+        r->r.resizeFrame(w, h, vx, vy, vw, vh);
+        return VT_OK;
+    });
 }
 
 int vtr_render(vtr_renderer* r)
 {
-    if (!r) return VT_ERR_ARG;
-    if (!r->r.context()) return VT_ERR_STATE;
-    Renderer::RenderResult rr = Renderer::RR_FINISHED_RENDERING;
-    const int e = guarded(r, VT_ERR_STATE, [&] { rr = r->r.render(); });
-    if (e != VT_OK) return e;
-    return rr == Renderer::RR_SAMPLES_PENDING ? 0 : 1;
+    return no_throw([&]() -> int {
+        if (!r) return VT_ERR_ARG;
+        if (!r->r.context()) return VT_ERR_STATE;
+        Renderer::RenderResult rr = Renderer::RR_FINISHED_RENDERING;
+        const int e = guarded(r, VT_ERR_STATE, [&] { rr = r->r.render(); });
+        if (e != VT_OK) return e;
+        return rr == Renderer::RR_SAMPLES_PENDING ? 0 : 1;
+    });
 }
 
 int vtr_reset_render(vtr_renderer* r)
 {
-    if (!r) return VT_ERR_ARG;
-    r->r.resetRender();
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r) return VT_ERR_ARG;
+        r->r.resetRender();
+        return VT_OK;
+    });
 }
 
 int vtr_set_seed(vtr_renderer* r, uint64_t seed)
 {
-    if (!r) return VT_ERR_ARG;
-    r->r.setSeed(seed);
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r) return VT_ERR_ARG;
+        r->r.setSeed(seed);
+        return VT_OK;
+    });
 }
 
 int32_t vtr_number_samples(vtr_renderer* r) { return r ? r->r.numberSamples() : VT_ERR_ARG; }
 
 int vtr_load_vox_file(vtr_renderer* r, const char* path)
 {
-    if (!path) return VT_ERR_ARG;
-    return guarded(r, VT_ERR_ARG, [&] { r->r.loadVoxFile(path); });
+    return no_throw([&]() -> int {
+        if (!path) return VT_ERR_ARG;
+        return guarded(r, VT_ERR_ARG, [&] { r->r.loadVoxFile(path); });
+    });
 }
 
 int vtr_load_mesh(vtr_renderer* r, const char* path)
 {
-    if (!path) return VT_ERR_ARG;
-    return guarded(r, VT_ERR_ARG, [&] { r->r.loadMesh(path); });
+    return no_throw([&]() -> int {
+        if (!path) return VT_ERR_ARG;
+        return guarded(r, VT_ERR_ARG, [&] { r->r.loadMesh(path); });
+    });
 }
 
 int vtr_load_volume(vtr_renderer* r, int32_t nx, int32_t ny, int32_t nz, const int32_t* offsets,
                     const float* mat, int32_t mat_size, const int32_t* emissive, int32_t n_emissive)
 {
-    if (!r || nx <= 0 || ny <= 0 || nz <= 0 || !offsets || mat_size < 0 || n_emissive < 0 ||
-        (mat_size > 0 && !mat) || (n_emissive > 0 && !emissive))
-        return VT_ERR_ARG;
-    const size_t n = (size_t)nx * ny * nz;
-    std::vector<int32_t> o(offsets, offsets + n), e(emissive, emissive + n_emissive);
-    std::vector<float> m(mat, mat + mat_size);
-    return guarded(r, VT_ERR_ARG, [&] { r->r.loadVolume(voxeltoy::V3i(nx, ny, nz), o, m, e); });
+    return no_throw([&]() -> int {
+        if (!r || nx <= 0 || ny <= 0 || nz <= 0 || !offsets || mat_size < 0 || n_emissive < 0 ||
+            (mat_size > 0 && !mat) || (n_emissive > 0 && !emissive))
+            return VT_ERR_ARG;
+        const size_t n = (size_t)nx * ny * nz;
+        std::vector<int32_t> o(offsets, offsets + n), e(emissive, emissive + n_emissive);
+        std::vector<float> m(mat, mat + mat_size);
+        return guarded(r, VT_ERR_ARG, [&] { r->r.loadVolume(voxeltoy::V3i(nx, ny, nz), o, m, e); });
+    });
 }
 
 int vtr_get_volume(vtr_renderer* r, int32_t* res, float* bmin, float* bmax, int32_t* offsets)
 {
-    if (!r) return VT_ERR_ARG;
-    const voxeltoy::V3i v = r->r.volumeResolution();
-    const voxeltoy::Box3f b = r->r.volumeBounds();
-    if (res) { res[0] = v.x; res[1] = v.y; res[2] = v.z; }
-    if (bmin) { bmin[0] = b.min.x; bmin[1] = b.min.y; bmin[2] = b.min.z; }
-    if (bmax) { bmax[0] = b.max.x; bmax[1] = b.max.y; bmax[2] = b.max.z; }
-    if (offsets) std::memcpy(offsets, r->r.hostOffsets().data(), r->r.hostOffsets().size() * sizeof(int32_t));
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r) return VT_ERR_ARG;
+        const voxeltoy::V3i v = r->r.volumeResolution();
+        const voxeltoy::Box3f b = r->r.volumeBounds();
+        if (res) { res[0] = v.x; res[1] = v.y; res[2] = v.z; }
+        if (bmin) { bmin[0] = b.min.x; bmin[1] = b.min.y; bmin[2] = b.min.z; }
+        if (bmax) { bmax[0] = b.max.x; bmax[1] = b.max.y; bmax[2] = b.max.z; }
+        if (offsets) std::memcpy(offsets, r->r.hostOffsets().data(), r->r.hostOffsets().size() * sizeof(int32_t));
+        return VT_OK;
+    });
 }
 
 int vtr_get_emissive(vtr_renderer* r, int32_t* out, int32_t cap, int32_t* count)
 {
-    if (!r || !count) return VT_ERR_ARG;
-    const std::vector<int32_t>& e = r->r.hostEmissive();
-    *count = (int32_t)e.size();
-    if (out) std::memcpy(out, e.data(), std::min<size_t>(e.size(), (size_t)std::max(0, cap)) * sizeof(int32_t));
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r || !count) return VT_ERR_ARG;
+        const std::vector<int32_t>& e = r->r.hostEmissive();
+        *count = (int32_t)e.size();
+        if (out) std::memcpy(out, e.data(), std::min<size_t>(e.size(), (size_t)std::max(0, cap)) * sizeof(int32_t));
+        return VT_OK;
+    });
 }
 
 int vtr_save_image(vtr_renderer* r, const char* path)
 {
-    if (!path) return VT_ERR_ARG;
-    if (r && !r->r.context()) return VT_ERR_STATE;
-    return guarded(r, VT_ERR_ARG, [&] { r->r.saveImage(path); });
+    return no_throw([&]() -> int {
+        if (!path) return VT_ERR_ARG;
+        if (r && !r->r.context()) return VT_ERR_STATE;
+        return guarded(r, VT_ERR_ARG, [&] { r->r.saveImage(path); });
+    });
 }
 
 int vtr_save_checkpoint(vtr_renderer* r, const char* path)
 {
-    if (!r || !path) return VT_ERR_ARG;
-    return r->r.saveCheckpoint(path) ? VT_OK : VT_ERR_STATE;
+    return no_throw([&]() -> int {
+        if (!r || !path) return VT_ERR_ARG;
+        return r->r.saveCheckpoint(path) ? VT_OK : VT_ERR_STATE;
+    });
 }
 
 int vtr_load_checkpoint(vtr_renderer* r, const char* path)
 {
-    if (!r || !path) return VT_ERR_ARG;
-    return r->r.loadCheckpoint(path) ? VT_OK : VT_ERR_STATE;
+    return no_throw([&]() -> int {
+        if (!r || !path) return VT_ERR_ARG;
+        return r->r.loadCheckpoint(path) ? VT_OK : VT_ERR_STATE;
+    });
 }
 
 int vtr_get_render_settings(vtr_renderer* r, vtr_settings* o)
 {
-    if (!r || !o) return VT_ERR_ARG;
-    const voxeltoy::RenderSettings& s = r->r.renderSettings();
-    o->max_bounces = s.m_pathtracerMaxNumBounces;
-    o->max_samples = s.m_pathtracerMaxSamples;
-    o->width = s.m_imageResolution.x;
-    o->height = s.m_imageResolution.y;
-    for (int k = 0; k < 4; ++k) o->viewport[k] = s.m_viewport[k];
-    o->wireframe_opacity = s.m_wireframeOpacity;
-    o->wireframe_thickness = s.m_wireframeThickness;
-    r->bg = s.m_backgroundImage;
-    o->background_image = r->bg.c_str();
-    for (int k = 0; k < 3; ++k) {
-        o->background_top[k] = s.m_backgroundColor[0][k];
-        o->background_bottom[k] = s.m_backgroundColor[1][k];
-    }
-    o->background_rotation_degrees = s.m_backgroundRotationDegrees;
-    o->exposure = s.m_exposure;
-    o->gamma = s.m_gamma;
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r || !o) return VT_ERR_ARG;
+        const voxeltoy::RenderSettings& s = r->r.renderSettings();
+        o->max_bounces = s.m_pathtracerMaxNumBounces;
+        o->max_samples = s.m_pathtracerMaxSamples;
+        o->width = s.m_imageResolution.x;
+        o->height = s.m_imageResolution.y;
+        for (int k = 0; k < 4; ++k) o->viewport[k] = s.m_viewport[k];
+        o->wireframe_opacity = s.m_wireframeOpacity;
+        o->wireframe_thickness = s.m_wireframeThickness;
+        r->bg = s.m_backgroundImage;
+        o->background_image = r->bg.c_str();
+        for (int k = 0; k < 3; ++k) {
+            o->background_top[k] = s.m_backgroundColor[0][k];
+            o->background_bottom[k] = s.m_backgroundColor[1][k];
+        }
+        o->background_rotation_degrees = s.m_backgroundRotationDegrees;
+        o->exposure = s.m_exposure;
+        o->gamma = s.m_gamma;
+        return VT_OK;
+    });
 }
 
 int vtr_update_render_settings(vtr_renderer* r, const vtr_settings* i)
 {
-    if (!r || !i) return VT_ERR_ARG;
-    if (i->max_bounces < 0 || i->width <= 0 || i->height <= 0 || !(i->gamma != 0.0f)) return VT_ERR_ARG;
-    voxeltoy::RenderSettings& s = r->r.renderSettings();
-    const bool resized = i->width != s.m_imageResolution.x || i->height != s.m_imageResolution.y;
-    s.m_pathtracerMaxNumBounces = i->max_bounces;
-    s.m_pathtracerMaxSamples = i->max_samples;
-    s.m_wireframeOpacity = i->wireframe_opacity;
-    s.m_wireframeThickness = i->wireframe_thickness;
-    s.m_backgroundImage = i->background_image ? i->background_image : "";
-    for (int k = 0; k < 3; ++k) {
-        s.m_backgroundColor[0][k] = i->background_top[k];
-        s.m_backgroundColor[1][k] = i->background_bottom[k];
-    }
-    s.m_backgroundRotationDegrees = i->background_rotation_degrees;
-    s.m_exposure = i->exposure;
-    s.m_gamma = i->gamma;
-    return guarded(r, VT_ERR_ARG, [&] {
-        if (resized)
-            r->r.resizeFrame(i->width, i->height, i->viewport[0], i->viewport[1], i->viewport[2], i->viewport[3]);
-        for (int k = 0; k < 4; ++k) s.m_viewport[k] = i->viewport[k];
-        r->r.updateRenderSettings();
+    return no_throw([&]() -> int {
+        if (!r || !i) return VT_ERR_ARG;
+        if (i->max_bounces < 0 || i->width <= 0 || i->height <= 0 || !(i->gamma != 0.0f)) return VT_ERR_ARG;
+        voxeltoy::RenderSettings& s = r->r.renderSettings();
+        const bool resized = i->width != s.m_imageResolution.x || i->height != s.m_imageResolution.y;
+        s.m_pathtracerMaxNumBounces = i->max_bounces;
+        s.m_pathtracerMaxSamples = i->max_samples;
+        s.m_wireframeOpacity = i->wireframe_opacity;
+        s.m_wireframeThickness = i->wireframe_thickness;
+        s.m_backgroundImage = i->background_image ? i->background_image : "";
+        for (int k = 0; k < 3; ++k) {
+            s.m_backgroundColor[0][k] = i->background_top[k];
+            s.m_backgroundColor[1][k] = i->background_bottom[k];
+        }
+        s.m_backgroundRotationDegrees = i->background_rotation_degrees;
+        s.m_exposure = i->exposure;
+        s.m_gamma = i->gamma;
+        return guarded(r, VT_ERR_ARG, [&] {
+            if (resized)
+                r->r.resizeFrame(i->width, i->height, i->viewport[0], i->viewport[1], i->viewport[2], i->viewport[3]);
+            for (int k = 0; k < 4; ++k) s.m_viewport[k] = i->viewport[k];
+            r->r.updateRenderSettings();
+        });
     });
 }
 
 int vtr_camera_get(vtr_renderer* r, vtr_camera_state* o)
 {
-    if (!r || !o) return VT_ERR_ARG;
-    const voxeltoy::CameraParameters& p = r->r.camera().parameters();
-    for (int k = 0; k < 3; ++k) { o->eye[k] = p.eye()[k]; o->target[k] = p.target()[k]; }
-    o->fov_y = p.fovY();
-    o->near_d = p.nearDistance();
-    o->far_d = p.farDistance();
-    o->focal_length = p.focalLength();
-    o->focal_distance = p.focalDistance();
-    o->lens_radius = p.lensRadius();
-    o->film[0] = p.filmSize().x;
-    o->film[1] = p.filmSize().y;
-    o->lens_model = (int32_t)p.lensModel();
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r || !o) return VT_ERR_ARG;
+        const voxeltoy::CameraParameters& p = r->r.camera().parameters();
+        for (int k = 0; k < 3; ++k) { o->eye[k] = p.eye()[k]; o->target[k] = p.target()[k]; }
+        o->fov_y = p.fovY();
+        o->near_d = p.nearDistance();
+        o->far_d = p.farDistance();
+        o->focal_length = p.focalLength();
+        o->focal_distance = p.focalDistance();
+        o->lens_radius = p.lensRadius();
+        o->film[0] = p.filmSize().x;
+        o->film[1] = p.filmSize().y;
+        o->lens_model = (int32_t)p.lensModel();
+        return VT_OK;
+    });
 }
 
 #define VTR_CAM(body)                \
@@ -882,24 +932,32 @@ int vtr_camera_get(vtr_renderer* r, vtr_camera_state* o)
 
 int vtr_camera_set_eye_target(vtr_renderer* r, const float* e, const float* t)
 {
-    if (!e || !t) return VT_ERR_ARG;
-    VTR_CAM(r->r.camera().controller().setEyeTarget(voxeltoy::V3f(e[0], e[1], e[2]), voxeltoy::V3f(t[0], t[1], t[2])))
+    return no_throw([&]() -> int {
+        if (!e || !t) return VT_ERR_ARG;
+        VTR_CAM(r->r.camera().controller().setEyeTarget(voxeltoy::V3f(e[0], e[1], e[2]), voxeltoy::V3f(t[0], t[1], t[2])))
+    });
 }
 int vtr_camera_look_at(vtr_renderer* r, const float* t)
 {
-    if (!t) return VT_ERR_ARG;
-    VTR_CAM(r->r.camera().controller().lookAt(voxeltoy::V3f(t[0], t[1], t[2])))
+    return no_throw([&]() -> int {
+        if (!t) return VT_ERR_ARG;
+        VTR_CAM(r->r.camera().controller().lookAt(voxeltoy::V3f(t[0], t[1], t[2])))
+    });
 }
 int vtr_camera_set_distance_from_target(vtr_renderer* r, float d)
 {
-    VTR_CAM(r->r.camera().controller().setDistanceFromTarget(d))
+    return no_throw([&]() -> int {
+        VTR_CAM(r->r.camera().controller().setDistanceFromTarget(d))
+    });
 }
 int vtr_camera_orbit(vtr_renderer* r, float theta, float phi) { VTR_CAM(r->r.camera().controller().orbit(theta, phi)) }
 int vtr_camera_focus_on_bounds(vtr_renderer* r) { VTR_CAM(r->r.camera().controller().focusOnBounds(r->r.volumeBounds())) }
 int vtr_camera_set_lens_model(vtr_renderer* r, int32_t m)
 {
-    if (m < 0 || m > 2) return VT_ERR_ARG;
-    VTR_CAM(r->r.camera().setLensModel((voxeltoy::CameraParameters::CameraLensModel)m))
+    return no_throw([&]() -> int {
+        if (m < 0 || m > 2) return VT_ERR_ARG;
+        VTR_CAM(r->r.camera().setLensModel((voxeltoy::CameraParameters::CameraLensModel)m))
+    });
 }
 int vtr_camera_set_focal_length(vtr_renderer* r, float l) { VTR_CAM(r->r.camera().setFocalLength(l)) }
 int vtr_camera_set_film_size(vtr_renderer* r, float w, float h) { VTR_CAM(r->r.camera().setFilmSize(w, h)) }
@@ -909,86 +967,106 @@ int vtr_camera_set_fov_y(vtr_renderer* r, float f) { VTR_CAM(r->r.camera().setFo
 
 int vtr_camera_set_controller(vtr_renderer* r, int32_t mode)
 {
-    if (!r || (mode != 0 && mode != 1)) return VT_ERR_ARG;
-    r->r.camera().setCameraController(mode == 1 ? voxeltoy::Camera::CCM_FLY : voxeltoy::Camera::CCM_ORBIT);
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r || (mode != 0 && mode != 1)) return VT_ERR_ARG;
+        r->r.camera().setCameraController(mode == 1 ? voxeltoy::Camera::CCM_FLY : voxeltoy::Camera::CCM_ORBIT);
+        return VT_OK;
+    });
 }
 
 int vtr_camera_block(vtr_renderer* r, vt_camera* out)
 {
-    if (!r || !out) return VT_ERR_ARG;
-    *out = r->r.cameraBlock();
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r || !out) return VT_ERR_ARG;
+        *out = r->r.cameraBlock();
+        return VT_OK;
+    });
 }
 
 int vtr_on_mouse_move(vtr_renderer* r, int32_t dx, int32_t dy, int32_t buttons)
 {
-    if (!r) return VT_ERR_ARG;
-    return r->r.onMouseMove(dx, dy, buttons) ? 1 : 0;
+    return no_throw([&]() -> int {
+        if (!r) return VT_ERR_ARG;
+        return r->r.onMouseMove(dx, dy, buttons) ? 1 : 0;
+    });
 }
 
 int vtr_on_key_press(vtr_renderer* r, int32_t key)
 {
-    if (!r) return VT_ERR_ARG;
-    return r->r.onKeyPress(key) ? 1 : 0;
+    return no_throw([&]() -> int {
+        if (!r) return VT_ERR_ARG;
+        return r->r.onKeyPress(key) ? 1 : 0;
+    });
 }
 
 int vtr_request_action(vtr_renderer* r, float x, float y, float dx, float dy, int32_t action, int32_t restart)
 {
-    if (!r || action < 0 || action > 3) return VT_ERR_ARG;
-    r->r.requestAction(x, y, dx, dy, (voxeltoy::Action::PICKING_ACTION)action, restart != 0);
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r || action < 0 || action > 3) return VT_ERR_ARG;
+        r->r.requestAction(x, y, dx, dy, (voxeltoy::Action::PICKING_ACTION)action, restart != 0);
+        return VT_OK;
+    });
 }
 
 float vtr_focal_distance(vtr_renderer* r) { return r ? r->r.focalDistance() : -1.0f; }
 
 int vtr_selected_voxel(vtr_renderer* r, int32_t* xyz)
 {
-    if (!r || !xyz) return VT_ERR_ARG;
-    const voxeltoy::V3i v = r->r.selectedVoxel();
-    xyz[0] = v.x; xyz[1] = v.y; xyz[2] = v.z;
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r || !xyz) return VT_ERR_ARG;
+        const voxeltoy::V3i v = r->r.selectedVoxel();
+        xyz[0] = v.x; xyz[1] = v.y; xyz[2] = v.z;
+        return VT_OK;
+    });
 }
 
 int vtr_load_obj(const char* path, float* vertices, int64_t vcap, int64_t* nv, uint32_t* indices, int64_t icap,
                  int64_t* nt, float* bmin, float* bmax)
 {
-    if (!path || !nv || !nt) return VT_ERR_ARG;
-    std::vector<float> v;
-    std::vector<uint32_t> idx;
-    std::string err;
-    if (!voxeltoy::MeshLoader::loadFromOBJ(path, v, idx, &err)) return VT_ERR_ARG;
-    *nv = (int64_t)(v.size() / 3);
-    *nt = (int64_t)(idx.size() / 3);
-    if (vertices && vcap >= *nv) std::memcpy(vertices, v.data(), v.size() * sizeof(float));
-    if (indices && icap >= *nt) std::memcpy(indices, idx.data(), idx.size() * sizeof(uint32_t));
-    const voxeltoy::Box3f b = voxeltoy::computeBounds(v.data(), v.size() / 3);
-    if (bmin) { bmin[0] = b.min.x; bmin[1] = b.min.y; bmin[2] = b.min.z; }
-    if (bmax) { bmax[0] = b.max.x; bmax[1] = b.max.y; bmax[2] = b.max.z; }
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!path || !nv || !nt) return VT_ERR_ARG;
+        std::vector<float> v;
+        std::vector<uint32_t> idx;
+        std::string err;
+        if (!voxeltoy::MeshLoader::loadFromOBJ(path, v, idx, &err)) return VT_ERR_ARG;
+        *nv = (int64_t)(v.size() / 3);
+        *nt = (int64_t)(idx.size() / 3);
+        if (vertices && vcap >= *nv) std::memcpy(vertices, v.data(), v.size() * sizeof(float));
+        if (indices && icap >= *nt) std::memcpy(indices, idx.data(), idx.size() * sizeof(uint32_t));
+        const voxeltoy::Box3f b = voxeltoy::computeBounds(v.data(), v.size() / 3);
+        if (bmin) { bmin[0] = b.min.x; bmin[1] = b.min.y; bmin[2] = b.min.z; }
+        if (bmax) { bmax[0] = b.max.x; bmax[1] = b.max.y; bmax[2] = b.max.z; }
+        return VT_OK;
+    });
 }
 
 int vtr_get_material_data(vtr_renderer* r, float* out, int32_t cap, int32_t* size)
 {
-    if (!r || !size) return VT_ERR_ARG;
-    const std::vector<float>& m = r->r.hostMaterials();
-    *size = (int32_t)m.size();
-    if (out) std::memcpy(out, m.data(), std::min<size_t>(m.size(), (size_t)std::max(0, cap)) * sizeof(float));
-    return VT_OK;
+    return no_throw([&]() -> int {
+        if (!r || !size) return VT_ERR_ARG;
+        const std::vector<float>& m = r->r.hostMaterials();
+        *size = (int32_t)m.size();
+        if (out) std::memcpy(out, m.data(), std::min<size_t>(m.size(), (size_t)std::max(0, cap)) * sizeof(float));
+        return VT_OK;
+    });
 }
 
 int vtr_update_material_color(vtr_renderer* r, uint32_t off, const float* rgb)
 {
-    if (!r || !rgb) return VT_ERR_ARG;
-    if ((size_t)off + 3 > r->r.hostMaterials().size()) return VT_ERR_ARG;
-    return guarded(r, VT_ERR_STATE, [&] { r->r.updateMaterialColor(off, rgb); });
+    return no_throw([&]() -> int {
+        if (!r || !rgb) return VT_ERR_ARG;
+        if ((size_t)off + 3 > r->r.hostMaterials().size()) return VT_ERR_ARG;
+        return guarded(r, VT_ERR_STATE, [&] { r->r.updateMaterialColor(off, rgb); });
+    });
 }
 
 int vtr_update_material_value(vtr_renderer* r, uint32_t off, float v)
 {
-    if (!r) return VT_ERR_ARG;
-    if ((size_t)off >= r->r.hostMaterials().size()) return VT_ERR_ARG;
-    return guarded(r, VT_ERR_STATE, [&] { r->r.updateMaterialValue(off, v); });
+    return no_throw([&]() -> int {
+        if (!r) return VT_ERR_ARG;
+        if ((size_t)off >= r->r.hostMaterials().size()) return VT_ERR_ARG;
+        return guarded(r, VT_ERR_STATE, [&] { r->r.updateMaterialValue(off, v); });
+    });
 }
 
 }  // extern "C"
