@@ -20,7 +20,9 @@ The same line carries the metric's other half (leave out with --no-extras):
 after another, and "time_to_rmse", the 1920x1080 scene at one sample per
 pixel per frame, 256 frames of the plain tracer against the radiance cache
 and plain path truncation given the same time, scored against a
-16384-sample FP64 reference.
+16384-sample FP64 reference.  It also carries "voxelize", the 200k-triangle
+mesh voxelized into 512^3 and rendered at 1920x1080 (10 steps; the same
+workload as --workload voxelize).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras]
 """
@@ -114,7 +116,7 @@ def main():
     ap.add_argument("--only", choices=["plain", "cache", "truncation"], default=None,
                     help="cache workload: run --frames frames of this contender alone (for a kernel trace)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="full workload only: leave out the batch and time-to-RMSE objects")
+                    help="full workload only: leave out the batch, time-to-RMSE and voxelize objects")
     ap.add_argument("--cache-bounce", type=int, default=2, help="cache workload: bounce at which paths end")
     ap.add_argument("--train-paths", type=int, default=65536,
                     help="cache workload: training paths per frame (65536 give about 58k samples per frame)")
@@ -236,15 +238,17 @@ def main():
         out["time_to_rmse"] = rmse_race(api, workloads, frames=args.frames, ref_spp=args.ref_spp,
                                         cache_bounce=args.cache_bounce, train_paths=args.train_paths,
                                         train_extra=args.train_extra)
+        # the voxelize-then-render workload (200k triangles into 512^3)
+        out["voxelize"] = voxelize_race(api, workloads, steps=10, warmup=3)
     print(json.dumps(out))
 
 
-def bench_voxelize(args, api, workloads):
+def voxelize_race(api, workloads, steps: int = 10, warmup: int = 3, mode: int = 1, streams: int = 1) -> dict:
     """A seeded displaced icosphere of 200 000 triangles (5 % radial noise)
     voxelized THIN into an empty 512^3 grid (computeMeshTransform placement),
     then rendered at 1920x1080, 8 spp, 6 bounces under a gradient sky.  A
     step is one voxelization plus one 8-pass render; the voxelizer kernel and
-    the render are also timed on their own."""
+    the render are also timed on their own.  Returns the measurements."""
     import math as _m
     import numpy as np
     from voxeltoy_amd import scenes
@@ -263,8 +267,8 @@ def bench_voxelize(args, api, workloads):
     ctx.set_environment(wl.env_top, wl.env_bottom)
     ctx.set_settings(max_bounces=bounces)
     ctx.set_frame(W, H)
-    ctx.set_render_mode(args.mode)
-    ctx.set_overlap(args.streams)
+    ctx.set_render_mode(mode)
+    ctx.set_overlap(streams)
 
     # first voxelization (untimed) and the statistics launch
     occ = ctx.voxelize(verts, tris, M, mode=0, material_offset=0, want_occupancy=True)
@@ -275,14 +279,14 @@ def bench_voxelize(args, api, workloads):
     ctx.render(wl.seed, 0, spp)
     st = ctx.stats()
     ctx.set_stats(False)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         ctx.voxelize(verts, tris, M, mode=0, material_offset=0, want_occupancy=False)
         ctx.reset_accum()
         ctx.render(wl.seed, 0, spp)
     ctx.synchronize()
     vox_ms, ren_ms = [], []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         ctx.voxelize(verts, tris, M, mode=0, material_offset=0, want_occupancy=False)
         vox_ms.append(ctx.stats()["kernel_ms"])
         ctx.reset_accum()
@@ -290,7 +294,7 @@ def bench_voxelize(args, api, workloads):
         ren_ms.append(ctx.stats()["kernel_ms"])
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    ms_per_step = wall * 1000.0 / args.steps
+    ms_per_step = wall * 1000.0 / steps
     v_ms = sum(vox_ms) / len(vox_ms)
     r_ms = sum(ren_ms) / len(ren_ms)
     rays = st["rays"]
@@ -306,41 +310,54 @@ def bench_voxelize(args, api, workloads):
     bctx.set_volume_u8(np.zeros((res, res, res), np.uint8), id_offset, mat)
     b_cov = int(bctx.voxelize(bv, bt, bM, mode=0, material_offset=0, want_occupancy=True).sum())
     b_ms = []
-    for _ in range(max(3, args.steps)):
+    for _ in range(max(3, steps)):
         bctx.voxelize(bv, bt, bM, mode=0, material_offset=0, want_occupancy=False)
         b_ms.append(bctx.stats()["kernel_ms"])
     bctx.close()
     b_ms = sum(b_ms) / len(b_ms)
+    ctx.close()
+    return {"workload": "200k-triangle displaced icosphere voxelized (THIN) into 512^3, then 1920x1080, "
+                        "8 spp, 6 bounces, gradient sky; a step = voxelization (host arrays uploaded) + render",
+            "steps": steps, "ms_per_step": round(ms_per_step, 3),
+            "mrays_per_s": round(rays / (ms_per_step / 1000.0) / 1e6, 2),
+            "triangles": int(len(tris)), "covered_voxels": covered, "rays_per_step": int(rays),
+            "voxelize_kernel_ms": round(v_ms, 4),
+            "voxelize_mtri_per_s": round(len(tris) / (v_ms / 1000.0) / 1e6, 1),
+            "voxelize_algorithmic_bytes": int(vox_bytes), "voxelize_achieved_gbs": round(vox_gbs, 1),
+            "render_kernel_ms": round(r_ms, 3),
+            "render_mrays_per_s": round(rays / (r_ms / 1000.0) / 1e6, 2), "streams": streams,
+            "bunny_512": {"triangles": int(len(bt)), "covered_voxels": b_cov,
+                          "voxelize_kernel_ms": round(b_ms, 4),
+                          "voxelize_mtri_per_s": round(len(bt) / (b_ms / 1000.0) / 1e6, 2)}}
+
+
+def bench_voxelize(args, api, workloads):
+    """The voxelize-then-render workload (voxelize_race) as the bench line."""
+    r = voxelize_race(api, workloads, steps=args.steps, warmup=args.warmup, mode=args.mode, streams=args.streams)
+    cfg = {k: v for k, v in r.items() if k not in ("steps", "ms_per_step", "mrays_per_s", "voxelize_algorithmic_bytes",
+                                                    "voxelize_achieved_gbs")}
     out = {
         "metric": METRIC,
-        "value": round(rays / (ms_per_step / 1000.0) / 1e6, 2),
+        "value": r["mrays_per_s"],
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "200k-triangle displaced icosphere voxelized (THIN) into 512^3, then 1920x1080, "
-                               "8 spp, 6 bounces, gradient sky; a step = voxelization (host arrays uploaded) + render",
-                   "triangles": int(len(tris)), "covered_voxels": covered, "rays_per_step": int(rays),
-                   "voxelize_kernel_ms": round(v_ms, 3),
-                   "voxelize_mtri_per_s": round(len(tris) / (v_ms / 1000.0) / 1e6, 1),
-                   "render_kernel_ms": round(r_ms, 3),
-                   "render_mrays_per_s": round(rays / (r_ms / 1000.0) / 1e6, 2), "streams": args.streams,
-                   "bunny_512": {"triangles": int(len(bt)), "covered_voxels": b_cov,
-                                 "voxelize_kernel_ms": round(b_ms, 4),
-                                 "voxelize_mtri_per_s": round(len(bt) / (b_ms / 1000.0) / 1e6, 2)}},
-        "roofline": {"bound": "hbm", "kernel": "voxelize_kernel", "achieved": round(vox_gbs, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(vox_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": int(vox_bytes), "avg_launch_ms": round(v_ms, 4)},
+        "config": cfg,
+        "roofline": {"bound": "hbm", "kernel": "voxelize_kernel", "achieved": r["voxelize_achieved_gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r["voxelize_achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": r["voxelize_algorithmic_bytes"],
+                     "avg_launch_ms": r["voxelize_kernel_ms"]},
         "cpu_baseline": None,
     }
     print(json.dumps(out))
-    ctx.close()
 
 
 def batch_race(api, workloads, steps: int = 5, warmup: int = 1) -> dict:
